@@ -1,0 +1,291 @@
+"""The FP64 CPU oracle checked against the reference's own known answers.
+
+Every expected value below is either a fixture of gplepage/vegas'
+tests/test_vegas.py or doc/source/*.out (cited file:line), a published
+known-answer vector, or an independent closed form / brute-force restatement
+chosen so that a plausible slip in the oracle (dropped term, wrong index or
+sign, transposed operand) fails.
+"""
+import math
+
+import numpy as np
+import pytest
+from numpy.testing import assert_allclose
+
+
+# ---------------------------------------------------------------- Philox ----
+# Random123 known-answer vectors for philox4x32_10 (kat_vectors, SC'11).
+KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", KAT)
+def test_philox_kat(orc, ctr, key, want):
+    assert [int(v) for v in orc.philox(ctr, key)] == list(want)
+
+
+def _py_philox(ctr, key):
+    """Independent pure-Python Philox4x32-10 (Salmon et al. SC'11, eq. for
+    the 4x32 bijection) used to pin the counter layout of the uniforms."""
+    c = list(ctr); k = list(key); M = 0xFFFFFFFF
+    for _ in range(10):
+        p0 = 0xD2511F53 * c[0]; p1 = 0xCD9E8D57 * c[2]
+        c = [((p1 >> 32) ^ c[1] ^ k[0]) & M, p1 & M, ((p0 >> 32) ^ c[3] ^ k[1]) & M, p0 & M]
+        k = [(k[0] + 0x9E3779B9) & M, (k[1] + 0xBB67AE85) & M]
+    return c
+
+
+def test_uniform_layout(orc):
+    seed, it, p0, n, dim = 0x1234567890ABCDEF, 7, (1 << 32) - 3, 6, 5
+    U = orc.uniforms(seed, it, p0, n, dim)
+    U32 = orc.uniforms(seed, it, p0, n, dim, f32=True)
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    for i in range(n):
+        p = p0 + i
+        for d in range(dim):
+            w = _py_philox((p & 0xFFFFFFFF, p >> 32, it, d // 2), key)
+            e = d % 2
+            want = ((w[2 * e] >> 5) * 2.0 ** 26 + (w[2 * e + 1] >> 6)) * 2.0 ** -53
+            assert U[i, d] == want
+            w = _py_philox((p & 0xFFFFFFFF, p >> 32, it, d // 4), key)
+            assert U32[i, d] == (w[d % 4] >> 8) * 2.0 ** -24
+    big = orc.uniforms(1, 0, 0, 20000, 3)
+    assert big.min() >= 0 and big.max() < 1
+    assert abs(big.mean() - 0.5) < 0.01
+
+
+# ------------------------------------------------------------ AdaptiveMap ----
+def test_make_uniform_is_linspace(orc):
+    m = orc.OracleMap([[0.1, 0.7], [-3.0, 17.0 / 3.0]], ninc=[997, 13])
+    for d, (a, b, n) in enumerate([(0.1, 0.7, 997), (-3.0, 17.0 / 3.0, 13)]):
+        assert np.array_equal(m.grid[d, :n + 1], np.linspace(a, b, n + 1))
+        assert np.array_equal(m.inc[d, :n], np.diff(np.linspace(a, b, n + 1)))
+
+
+def test_init_fixture(orc):
+    # tests/test_vegas.py:39-63
+    m = orc.OracleMap([[0, 1], [2, 4]])
+    assert_allclose(m.grid, [[0, 1], [2, 4]]); assert_allclose(m.inc, [[1], [2]])
+    m = orc.OracleMap([[0, 1], [-2, 4]], ninc=2)
+    assert_allclose(m.grid, [[0, 0.5, 1.], [-2., 1., 4.]])
+    assert_allclose(m.inc, [[0.5, 0.5], [3., 3.]])
+    m = orc.OracleMap([[0, 0.4, 1], [-2, 0., 4]], ninc=4)
+    assert_allclose(m.grid, [[0, 0.2, 0.4, 0.7, 1.], [-2., -1., 0., 2., 4.]])
+    assert_allclose(m.inc, [[0.2, 0.2, 0.3, 0.3], [1, 1, 2, 2]])
+    m = orc.OracleMap([[0, 1], [2, 2.5, 4]])
+    assert list(m.ninc) == [1, 2]
+    assert_allclose(m.grid[1, :3], [2, 2.5, 4]); assert_allclose(m.inc[1, :2], [.5, 1.5])
+
+
+def test_map_fixture(orc):
+    # tests/test_vegas.py:76-95 (includes the y == 1 branch, pyx:357-359)
+    m = orc.OracleMap([[0, 1, 3], [-2, 0, 6]])
+    y = np.array([[0, 0], [0.25, 0.25], [0.5, 0.5], [0.75, 0.75], [1.0, 1.0]])
+    x, jac = m.map(y)
+    assert_allclose(x, [[0, -2], [0.5, -1], [1, 0], [2, 3], [3, 6]])
+    assert_allclose(jac, [8, 8, 48, 48, 48])
+    assert_allclose(np.prod(m.jac1d(y), axis=1), jac)
+
+
+def test_invmap_fixture(orc):
+    # tests/test_vegas.py:97-112
+    m = orc.OracleMap([[0., 1., 3.], [-2., 0., 6.]])
+    x = np.array([[0., -2.], [0.5, -1], [1, 0], [2, 3], [3, 6]])
+    y, jac = m.invmap(x)
+    assert_allclose(y, [[0, 0], [0.25, 0.25], [0.5, 0.5], [0.75, 0.75], [1.0, 1.0]])
+    assert_allclose(jac, [8., 8, 48, 48, 48])
+    rng = np.random.default_rng(3)
+    m = orc.OracleMap([np.sort(rng.uniform(0, 1, 31)), np.sort(rng.uniform(-5, 2, 17))])
+    yy = rng.uniform(0, 1, (500, 2))
+    xx, jj = m.map(yy)
+    y2, j2 = m.invmap(xx)
+    assert_allclose(y2, yy, rtol=1e-12, atol=1e-14)
+    assert_allclose(j2, jj, rtol=1e-12)
+
+
+def test_training_histogram_bruteforce(orc):
+    rng = np.random.default_rng(11)
+    m = orc.OracleMap([[0, 1], [0, 2], [-1, 1]], ninc=[7, 10, 5])
+    y = rng.uniform(0, 1, (2000, 3))
+    y[:5] = 0.0; y[5:9] = 1.0; y[9, 1] = 0.7
+    f = rng.normal(size=2000)
+    m.add_training_data(y, f)
+    for d, n in enumerate([7, 10, 5]):
+        ok = (y[:, d] > 0) & (y[:, d] < 1)
+        k = np.floor(y[ok, d] * n).astype(int)
+        sf = np.bincount(k, weights=np.abs(f[ok]), minlength=n)
+        nf = np.bincount(k, minlength=n)
+        assert_allclose(m.sum_f[d, :n], sf, rtol=1e-13)
+        assert np.array_equal(m.n_f[d, :n], np.where(nf > 0, nf.astype(float), orc.TINY))
+
+
+def test_adapt_noop_cases(orc):
+    # tests/test_vegas.py:146-159
+    m = orc.OracleMap([[0, 2], [-1, 1]], ninc=2)
+    y = np.array([[0.25, 0.25], [0.75, 0.75]])
+    m.add_training_data(y, np.prod(m.jac1d(y), axis=1))
+    m.adapt(alpha=1.5)
+    assert_allclose(m.grid, [[0., 1., 2.], [-1, 0, 1]])
+    m = orc.OracleMap([[0, 1.5, 2], [-1, 0, 1]])
+    m.add_training_data(y, np.prod(m.jac1d(y), axis=1))
+    m.adapt(alpha=0.0)
+    assert_allclose(m.grid, [[0, 1.5, 2], [-1, 0, 1]])
+
+
+def _gauss_train(orc, grid, F, nitn, alpha):
+    g = 1. / 3. ** 0.5
+    yg = [(1 - g) / 4., (1 + g) / 4, (3 - g) / 4, (3 + g) / 4.]
+    m = orc.OracleMap(grid, ninc=2)
+    if len(grid) == 1:
+        y = np.array([[a] for a in yg])
+    else:
+        y = np.array([[a, b] for a in yg for b in yg])
+    for _ in range(nitn):
+        x, jac = m.map(y)
+        m.add_training_data(y, F(x) * jac)
+        m.adapt(alpha=alpha)
+    return m
+
+
+def test_adapt_closed_forms(orc):
+    # tests/test_vegas.py:169-207: 2-pt Gauss abscissas make training exact
+    m = _gauss_train(orc, [[0, 2]], lambda x: x[:, 0] ** 2, 60, 2.)
+    assert_allclose(m.grid, [[0, 2. / 2. ** (1. / 3.), 2.]])
+    F = lambda x: x[:, 0] * x[:, 1] ** 2  # noqa: E731
+    want = [[0, 2. * 2. ** (-0.5), 2.], [0, 4. * 2 ** (-1. / 3.), 4.]]
+    assert_allclose(_gauss_train(orc, [[0, 2], [0, 4]], F, 60, 2.).grid, want)
+    assert_allclose(_gauss_train(orc, [[0, 2], [0, 4]], F, 20, -2.).grid, want)
+
+
+def test_adapt_regrid_preserves_jacobian(orc):
+    # alpha = 0 regrid (pyx:133-137, tests:50-57): the new grid reproduces the
+    # old piecewise-linear map at every new node, for any ninc.
+    rng = np.random.default_rng(5)
+    nodes = np.concatenate([[0.], np.sort(rng.uniform(0, 3, 9)), [3.]])
+    m = orc.OracleMap([nodes])
+    old = orc.OracleMap(m)
+    m.adapt(ninc=40)
+    yn = np.arange(41)[:, None] / 40.
+    x_old, _ = old.map(yn)
+    assert_allclose(m.grid[0, :41], x_old[:, 0], rtol=1e-13, atol=1e-13)
+
+
+def test_adapt_shared_buffer_quirk(orc):
+    # pyx:534: with alpha > 0 and no training data, the buffer written for one
+    # axis leaks into the next.  Axes of equal ninc stay uniform.
+    m = orc.OracleMap([[0, 1], [0, 1]], ninc=[6, 6])
+    m.adapt(alpha=0.5)
+    assert_allclose(m.grid, [np.linspace(0, 1, 7)] * 2, rtol=1e-14)
+    m = orc.OracleMap([[0, 1], [0, 1]], ninc=[4, 6])
+    m.adapt(alpha=0.5)
+    assert_allclose(m.grid[0, :5], np.linspace(0, 1, 5), rtol=1e-14)
+    assert not np.allclose(m.grid[1, :7], np.linspace(0, 1, 7))
+
+
+# --------------------------------------------------------------- geometry ----
+def test_geometry_fixtures(orc):
+    g = orc.geometry(2, neval=234)  # tests:561-575
+    assert list(g["ninc"]) == [20, 20] and list(g["nstrat"]) == [5, 5]
+    g = orc.geometry(2, neval=1000, nstrat=[1, 1])  # tests:577-582
+    assert list(g["ninc"]) == [100, 100] and g["min_neval_hcube"] == 1000
+    g = orc.geometry(2, nstrat=[10, 11])  # tests:588-595
+    assert list(g["ninc"]) == [80, 88] and g["neval"] == 880 and g["min_neval_hcube"] == 2
+    g = orc.geometry(2, nstrat=[22, 13])  # tests:727-729
+    assert g["neval"] == 22 * 13 * 2 / 0.25 and g["min_neval_hcube"] == 2
+    g = orc.geometry(2, nstrat=[7, 9], neval=2000)  # tests:730-734
+    assert list(g["ninc"]) == [196, 198] and g["min_neval_hcube"] == 7
+    with pytest.raises(ValueError):
+        orc.geometry(2, nstrat=[7, 9], neval=20)
+    assert list(orc.geometry(2, neval=3100)["nstrat"]) == [20, 19]  # tests:741-743
+    assert list(orc.geometry(2, neval=3500)["nstrat"]) == [21, 20]  # tests:747-749
+    with pytest.raises(ValueError):
+        orc.geometry(2, nstrat=[10, 12], neval=120 * 4)
+    g = orc.geometry(2, neval=3.5e8)  # tests:756-760 alignment
+    assert np.all(np.round(g["nstrat"] / g["ninc"]) * g["ninc"] == g["nstrat"])
+    g = orc.geometry(2, neval=300)
+    assert np.all(np.round(g["ninc"] / g["nstrat"]) * g["nstrat"] == g["ninc"])
+    g = orc.geometry(5, neval=1000)  # doc/source/eg10.out:3-5
+    assert list(g["nstrat"]) == [3, 3, 3, 2, 2] and list(g["ninc"]) == [99, 99, 99, 100, 100]
+    assert g["nhcube"] == 108
+    assert orc.geometry(20, neval=2e6)["nhcube"] == 2 ** 17  # doc/source/eg6a.out:1
+
+
+@pytest.mark.parametrize("dim,neval,nstrat,ninc,nh", [
+    (4, 1e5, [11, 11, 10, 10], [990, 990, 1000, 1000], 12100),
+    (8, 1e7, [6] * 6 + [5] * 2, [996] * 6 + [1000] * 2, 1166400),
+    (8, 2e7, [7, 7] + [6] * 6, [994] * 2 + [996] * 6, 2286144),
+    (9, 2e7, [6] + [5] * 8, [996] + [1000] * 8, 2343750),
+])
+def test_geometry_configs(orc, dim, neval, nstrat, ninc, nh):
+    g = orc.geometry(dim, neval=neval)
+    assert list(g["nstrat"]) == nstrat and list(g["ninc"]) == ninc and g["nhcube"] == nh
+
+
+# ------------------------------------------------------------------ RAvg ----
+def test_ravg_closed_form(orc):
+    # tests/test_vegas.py:216-226
+    mean, sdev, chi2 = orc.ravg([1., 2., 3.], [1., 4., 9.])
+    assert_allclose(mean, 1.346938775510204, rtol=1e-15)
+    assert_allclose(sdev, 0.8571428571428571, rtol=1e-15)
+    assert_allclose(chi2, 0.5306122448979592, rtol=1e-15)
+    assert_allclose(orc.gammaQ(2 / 2., chi2 / 2.), 0.7669711269557102, rtol=1e-14)
+    from scipy.special import gammaincc
+    for a, x in [(0.5, 0.3), (4.5, 1.0), (7.0, 20.0), (100.0, 90.0)]:
+        assert_allclose(orc.gammaQ(a, x), gammaincc(a, x), rtol=1e-12)
+    m, s, c = orc.ravg([1., 2., 3.], [1., 4., 9.], weighted=False)
+    assert_allclose(m, 2.0); assert_allclose(s, math.sqrt(14.) / 3.)
+    assert_allclose(c, 2.0 / (14. / 3.))
+
+
+# ------------------------------------------------------------- iteration ----
+def test_constant_integrand_guard(orc):
+    # tests/test_vegas.py:1272-1296: EPSILON guard makes sdev ratios exact
+    I = orc.OracleIntegrator([(0, 1)], neval=100, alpha=0)
+    g7 = lambda x: 7 * np.ones(x.shape[0])  # noqa: E731
+    m, v, _ = I.run(g7, 4)
+    mean, sdev, _ = orc.ravg(m, v)
+    s = np.sqrt(v)
+    assert round(s[0] / s[1] - 1.0, 7) == 0
+    assert round(s[0] / sdev - 2.0, 7) == 0
+    assert round(mean - 7.0, 7) == 0
+    I = orc.OracleIntegrator([(0, 1)], neval=100, alpha=0, adapt=False)
+    m, v, _ = I.run(g7, 4)
+    mean, sdev, _ = orc.ravg(m, v, weighted=False)
+    s = np.sqrt(v)
+    assert round(s[0] / s[1] - 1.0, 7) == 0 and round(s[0] / sdev - 2.0, 7) == 0
+    I = orc.OracleIntegrator([(0, 1)], neval=100)
+    m, v, _ = I.run(lambda x: np.zeros(x.shape[0]), 4)
+    assert orc.ravg(m, v)[0] == 0.0
+
+
+def test_volume_exact(orc):
+    # tests/test_vegas.py:774-781: f = 2 on [-1,1]x[0,4] -> 16 with tiny sdev
+    I = orc.OracleIntegrator([(-1, 1), (0, 4)], neval=1000)
+    m, v, _ = I.run(lambda x: 2 * np.ones(x.shape[0]), 10)
+    mean, sdev, _ = orc.ravg(m, v)
+    assert_allclose(mean, 16, rtol=1e-12) and sdev < 1e-6
+
+
+def test_first_iteration_allocation(orc):
+    # uniform sigf: n_h = int(neval_frac*neval/nhcube) + min_neval_hcube
+    I = orc.OracleIntegrator([(0, 1)] * 4, neval=1e5)
+    o, sf, nf = I.iterate(lambda x: np.ones(x.shape[0]))
+    # neval_hcube_range starts at [min_neval_hcube]*2 (pyx:1682, 1699-1702)
+    assert o.last_neval == 96800 and o.nh_min == 2 and o.nh_max == 8
+    assert nf[nf > 1].sum() == 4 * 96800
+
+
+def test_gaussian_4d_analytic(orc):
+    # config-1 integrand (tests/test_vegas.py:1202-1215 with erf normalisation)
+    N = (100 / math.pi) ** 2 / math.erf(5) ** 4
+    f = orc.GaussMix([N], [100.], [[0.5] * 4])
+    I = orc.OracleIntegrator([(0, 1)] * 4, neval=2e4)
+    m, v, _ = I.run(f, 10)
+    mean, sdev, chi2 = orc.ravg(m, v)
+    assert abs(mean - 1.0) < 5 * sdev
+    assert orc.gammaQ(9 / 2., chi2 / 2.) > 1e-3
+    assert sdev < 2e-3
