@@ -1,0 +1,188 @@
+/*
+ * vgx.h -- C-ABI of libvgx.so, the MI355X (gfx950) vegas+ hot path.
+ *
+ * Plain pointers and sizes only.  Every pointer argument documented as
+ * "device" is device memory of the current HIP device; "host" pointers are
+ * read synchronously before the call returns.  `stream` is a hipStream_t
+ * (NULL = the null stream).  All kernels are enqueued asynchronously on that
+ * stream; calls on one vgx_ctx are stream-ordered and not thread-safe.
+ * No entry point returns memory the caller must free.  Every function returns
+ * an int status: VGX_OK (0) or a negative VGX_E* code (see vgx_strerror).
+ *
+ * Layouts (row-major, as the reference's memoryviews, _vegas.pxd:21-31):
+ *   grid   double[dim][ld]      nodes of axis d in grid[d][0..ninc[d]];
+ *                               increments are grid[d][i+1]-grid[d][i]
+ *                               (the reference stores them separately,
+ *                               computed by exactly that subtraction)
+ *   sum_f  double[dim][ld-1]    training sums   (AdaptiveMap.sum_f)
+ *   n_f    double[dim][ld-1]    training counts (AdaptiveMap.n_f)
+ *   y, x   T[n][dim]            points, T = double or float
+ */
+#ifndef VGX_H
+#define VGX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VGX_MAXDIM 64
+
+enum {
+    VGX_OK = 0,
+    VGX_EINVAL = -1,   /* bad argument (the reference's ValueError) */
+    VGX_EHIP = -2,     /* HIP runtime error */
+    VGX_ENAN = -3,     /* integrand evaluated to NaN (pyx:2133-2134) */
+    VGX_ENOMEM = -4,   /* device allocation failed (MemoryError) */
+    VGX_ERANGE = -5,   /* size beyond what the kernels support */
+    VGX_ENODEV = -6    /* no gfx950 device */
+};
+
+const char* vgx_strerror(int status);
+int vgx_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* AdaptiveMap.map(y, x, jac, ny)              -- _vegas.pyx:310-360
+ * ninc: host int32[dim].  grid: device.  y in, x/jac out: device.        */
+int vgx_map_f64(int dim, const int32_t* ninc, const double* grid, int ld,
+                const double* y, double* x, double* jac, int64_t n, void* stream);
+/* Same map with float y/x/jac (index and interpolation computed in FP64). */
+int vgx_map_f32(int dim, const int32_t* ninc, const double* grid, int ld,
+                const float* y, float* x, float* jac, int64_t n, void* stream);
+
+/* AdaptiveMap.invmap(x, y, jac, nx)           -- _vegas.pyx:362-416      */
+int vgx_invmap_f64(int dim, const int32_t* ninc, const double* grid, int ld,
+                   const double* x, double* y, double* jac, int64_t n, void* stream);
+
+/* AdaptiveMap.jac1d(y)                        -- _vegas.pyx:265-295
+ * jac out: device double[n][dim].                                        */
+int vgx_jac1d_f64(int dim, const int32_t* ninc, const double* grid, int ld,
+                  const double* y, double* jac, int64_t n, void* stream);
+
+/* ---------------------------------------------------------------------- */
+typedef struct vgx_ctx vgx_ctx;
+/* Scratch owner for the current device (scan temporaries, tile partials,
+ * staging).  Released by vgx_ctx_destroy.                                */
+int vgx_ctx_create(vgx_ctx** out);
+int vgx_ctx_destroy(vgx_ctx* ctx);
+
+/* AdaptiveMap.add_training_data(y, f, ny)     -- _vegas.pyx:421-464
+ * Adds |f[i]| and 1 to sum_f/n_f at bin floor(y[i][d]*ninc[d]) for every
+ * axis d with 0 < y < 1.  ordered=1 sums every bin in point order, bit-for-
+ * bit the reference's sequential loop; ordered=0 uses LDS-privatised
+ * atomics (same counts, sums equal to rounding).                        */
+int vgx_train_hist(vgx_ctx* ctx, int dim, const int32_t* ninc, int ld,
+                   const double* y, const double* f, int64_t n,
+                   double* sum_f, double* n_f, int ordered, void* stream);
+
+/* AdaptiveMap.adapt(alpha, ninc)              -- _vegas.pyx:467-594
+ * has_training=0 is the reference's sum_f None.  new_grid: device
+ * double[dim][new_ld], new_ld >= max(new_ninc)+1.  exact=1 runs the
+ * reference's sequential sums and CDF walk in one lane per axis (order of
+ * operations identical to the reference); exact=0 uses block-parallel
+ * reductions and a parallel CDF inversion (agrees to ~1e-13 relative).  */
+int vgx_adapt(vgx_ctx* ctx, int dim, const int32_t* ninc, const double* grid, int ld,
+              int has_training, const double* sum_f, const double* n_f, double alpha,
+              const int32_t* new_ninc, double* new_grid, int new_ld, int exact,
+              void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Integrator: one vegas+ run of nitn iterations enqueued without host
+ * synchronisation.  Geometry (pyx:1331-1446) is computed by the caller. */
+enum { VGX_PREC_F64 = 0, VGX_PREC_F32 = 1 };
+enum { VGX_FCN_GAUSSMIX = 1, VGX_FCN_DIAGRIDGE = 2, VGX_FCN_BUFFER = 3 };
+enum {
+    VGX_FLAG_EXACT_ADAPT = 1,  /* vgx_adapt(exact=1) inside the loop */
+    VGX_FLAG_EXACT_DIV = 2     /* y = (c+u)/s by IEEE division (as pyx:1741) */
+};
+
+typedef struct {
+    int32_t dim;
+    int32_t ld;                      /* grid row stride, >= max(ninc)+1 */
+    int32_t ninc[VGX_MAXDIM];
+    int32_t nstrat[VGX_MAXDIM];
+    int64_t nhcube;
+    int64_t neval;
+    double neval_frac;               /* raw Integrator.neval_frac */
+    double alpha;
+    double beta;
+    int32_t adapt;
+    int32_t adapt_to_errors;
+    int64_t min_neval_hcube;
+    int64_t max_neval_hcube;
+    uint64_t seed;
+    uint32_t iteration;              /* RNG stream of the first iteration */
+    int32_t nitn;
+    int32_t precision;               /* VGX_PREC_* : integrand arithmetic */
+    int32_t flags;                   /* VGX_FLAG_* */
+    int32_t weighted;                /* RAvg weighted average (adapt) */
+    int32_t reserved;
+} vgx_config;
+
+/* Device integrands.  GAUSSMIX: f(x) = sum_k A[k] exp(-B[k] |x - c[k]|^2)
+ * with c double[ncomp][dim].  DIAGRIDGE: the same mixture with centres on
+ * the main diagonal, c[k] = t[k]*(1,...,1) (c holds t[ncomp]); evaluated
+ * with |x-c|^2 = |x|^2 - 2 t sum(x) + dim t^2.  BUFFER: f values supplied
+ * by the caller (device double, indexed by global point index - fbase).  */
+typedef struct {
+    int32_t kind;
+    int32_t ncomp;
+    const double* A;                 /* device */
+    const double* B;                 /* device */
+    const double* c;                 /* device */
+    const double* fbuf;              /* BUFFER only */
+    int64_t fbase;
+} vgx_integrand;
+
+/* Caller-owned device state, updated in place. */
+typedef struct {
+    double* grid;                    /* dim x ld */
+    double* sigf;                    /* nhcube (stratification weights) */
+    double* sum_sigf;                /* [1] Integrator.sum_sigf */
+    double* sum_f;                   /* dim x (ld-1) training scratch */
+    double* n_f;                     /* dim x (ld-1) */
+    double* itn_mean;                /* [nitn] per-iteration estimate */
+    double* itn_var;                 /* [nitn] */
+    double* ravg;                    /* [4] mean, sdev, chi2, sum_w (FP64) */
+    int64_t* itn_neval;              /* [nitn] last_neval */
+    int64_t* itn_nhrange;            /* [2*nitn] neval_hcube_range */
+    int32_t* status;                 /* [1] 0 or VGX_ENAN */
+} vgx_state;
+
+/* Integrator.__call__ for a device integrand   -- _vegas.pyx:2086-2227.
+ * Runs cfg->nitn iterations: allocation (pyx:1657-1706), stratified y,
+ * map, integrand, per-hcube two-pass sums with the EPSILON guard
+ * (pyx:2138-2193), training histogram, sigf/sum_sigf update, adapt and the
+ * running average, all on device.  Host synchronisation is the caller's. */
+int vgx_integrate(vgx_ctx* ctx, const vgx_config* cfg, const vgx_integrand* fcn,
+                  vgx_state* st, void* stream);
+
+/* Staged iteration for host-driven (Python) integrands, in three steps:
+ * vgx_iter_begin    allocation + scan; writes n_h (device int32[nhcube]) so
+ *                   the caller can cut batches at hcube boundaries exactly
+ *                   as _random_batch does (pyx:1692-1710);
+ * vgx_iter_sample   x (and optionally y, wgt, hcube) for the hcubes
+ *                   [h0, h1) (pyx:1731-1759);
+ * vgx_iter_accumulate  per-hcube sums and training data for the same batch
+ *                   given f (device double[n]) (pyx:2136-2197);
+ * vgx_iter_end      closes the iteration: estimate, sum_sigf, running
+ *                   average and adapt (pyx:2199-2217).  it = iteration
+ *                   index within cfg->nitn.                              */
+int vgx_iter_begin(vgx_ctx* ctx, const vgx_config* cfg, vgx_state* st, int it,
+                   int32_t* n_h, int64_t* total_host, void* stream);
+int vgx_iter_sample(vgx_ctx* ctx, const vgx_config* cfg, vgx_state* st, int it,
+                    int64_t h0, int64_t h1, int64_t p0, int64_t p1,
+                    double* x, double* y, double* wgt, int64_t* hcube, void* stream);
+int vgx_iter_accumulate(vgx_ctx* ctx, const vgx_config* cfg, vgx_state* st, int it,
+                        int64_t h0, int64_t h1, int64_t p0, int64_t p1,
+                        const double* f, void* stream);
+int vgx_iter_end(vgx_ctx* ctx, const vgx_config* cfg, vgx_state* st, int it,
+                 void* stream);
+
+/* Last-kernel timing of the fused sampler (HIP events on `stream`), ms. */
+int vgx_last_fused_ms(vgx_ctx* ctx, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -289,11 +289,9 @@ class OracleIntegrator:
                      max_mem=max_mem)
         self.__dict__.update(g)
         self.map = OracleMap([list(r) for r in region])
+        # Integrator.set regrids with map.adapt(ninc=...) (pyx:1385-1386)
         if not np.all(self.map.ninc == g["ninc"]):
-            if np.all(self.map.ninc == 1):
-                self.map.make_uniform(g["ninc"])
-            else:
-                self.map.adapt(ninc=g["ninc"])
+            self.map.adapt(ninc=g["ninc"])
         self.neval_frac, self.alpha, self.beta = neval_frac, alpha, beta
         self.adapt_, self.adapt_to_errors = adapt, adapt_to_errors
         self.min_neval_batch, self.max_neval_hcube = min_neval_batch, max_neval_hcube

@@ -1,0 +1,186 @@
+"""GPU vegas+ iterations against the FP64 oracle and analytic values.
+
+FP64 parity mode (precision='f64', exact=True) feeds the device and the
+oracle the same Philox uniforms, so the first iteration's y, x, J and weights
+are identical; the estimate, variance, sigf and training histogram agree to
+1e-12 relative (the integrand's exp differs in the last bit).  Later
+iterations are compared statistically: a one-ulp sigf difference may flip an
+int() in the allocation and re-deal the random numbers.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def va():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import vegas_amd
+    return vegas_amd
+
+
+def gauss4(va, orc):
+    N = (100 / math.pi) ** 2 / math.erf(5) ** 4
+    return va.GaussianMixture([N], [100.], [[0.5] * 4]), orc.GaussMix([N], [100.], [[0.5] * 4])
+
+
+def double_peak(va, dim=8, sigma=0.02):
+    return va.GaussianMixture.isotropic([[0.33] * dim, [0.67] * dim], sigma)
+
+
+@pytest.mark.parametrize("neval,nstrat", [(20000, None), (3000, [5, 5, 4, 4]), (2e5, None)])
+def test_first_iteration_parity(va, orc, neval, nstrat):
+    f, fo = gauss4(va, orc)
+    integ = va.Integrator([[0, 1]] * 4, neval=neval, nstrat=nstrat, seed=11, precision='f64',
+                          exact=True, alpha=0.0)
+    r = integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * 4, neval=neval, nstrat=nstrat, seed=11, alpha=0.0)
+    out, sf, nf = o.iterate(fo)
+    assert integ.last_neval == out.last_neval
+    m = r.itn_results[0]
+    assert abs(m.mean - out.mean) <= 1e-12 * abs(out.mean)
+    assert abs(m.var - out.var) <= 1e-10 * out.var
+    np.testing.assert_allclose(integ.sigf, o.sigf, rtol=1e-11)
+    assert abs(integ.sum_sigf - o.sigf.sum()) <= 1e-10 * o.sigf.sum()
+
+
+def test_training_and_adapt_parity(va, orc):
+    # one iteration with adaptation: histogram -> adapt on device vs oracle
+    f, fo = gauss4(va, orc)
+    integ = va.Integrator([[0, 1]] * 4, neval=50000, seed=5, precision='f64', exact=True)
+    integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * 4, neval=50000, seed=5)
+    o.run(fo, 1)
+    np.testing.assert_allclose(integ.map.grid, o.map.grid, rtol=1e-11, atol=1e-13)
+
+
+def test_fused_equals_staged(va, orc):
+    """Same seed and precision: the device integrand and a numpy lbatch,
+    rbatch, torch and non-batch integrand see the same points."""
+    import torch
+    f, _ = gauss4(va, orc)
+    N, B = f.A[0], f.B[0]
+
+    @va.lbatchintegrand
+    def fl(x):
+        return N * np.exp(-B * ((x - 0.5) ** 2).sum(1))
+
+    @va.rbatchintegrand
+    def fr(x):
+        return N * np.exp(-B * ((x - 0.5) ** 2).sum(0))
+
+    @va.gpu_lbatchintegrand
+    def ft(x):
+        return N * torch.exp(-B * ((x - 0.5) ** 2).sum(1))
+
+    def fs(x):
+        return N * math.exp(-B * sum((xi - 0.5) ** 2 for xi in x))
+
+    res = []
+    for fn, kw in [(f, {}), (fl, dict(min_neval_batch=3000)), (fr, {}), (ft, {}), (fs, {})]:
+        neval = 4000 if fn is fs else 30000
+        integ = va.Integrator([[0, 1]] * 4, neval=neval, seed=3, precision='f64', **kw)
+        res.append((neval, integ(fn, nitn=3)))
+    ref = [r for n, r in res if n == 30000][0]
+    for n, r in res:
+        if n != 30000:
+            continue
+        for a, b in zip(r.itn_results, ref.itn_results):
+            assert abs(a.mean - b.mean) <= 1e-12 * abs(b.mean)
+
+
+def test_constant_integrand_guard(va):
+    # tests/test_vegas.py:1272-1296
+    @va.batchintegrand
+    def g(x):
+        return 7 * np.ones(x.shape[0], float)
+    integ = va.Integrator([(0, 1)], neval=100, alpha=0)
+    res = integ(g, nitn=4)
+    assert round(res.itn_results[0].sdev / res.itn_results[1].sdev - 1.0, 7) == 0
+    assert round(res.itn_results[0].sdev / res.sdev - 2.0, 7) == 0
+    assert round(res.mean - 7.0, 7) == 0
+    res = integ(g, nitn=4, neval=1e2, adapt=False)
+    assert round(res.itn_results[0].sdev / res.itn_results[1].sdev - 1.0, 7) == 0
+    assert round(res.itn_results[0].sdev / res.sdev - 2.0, 7) == 0
+
+    @va.batchintegrand
+    def z(x):
+        return np.zeros(x.shape[0], float)
+    assert va.Integrator([(0, 1)])(z, nitn=4, neval=100).mean == 0.0
+
+
+def test_volume(va):
+    # tests/test_vegas.py:774-781 with a device integrand (constant via B=0)
+    f = va.GaussianMixture([2.0], [0.0], [[0.0, 0.0]])
+    r = va.Integrator([[-1, 1], [0, 4]], seed=1)(f)
+    assert abs(r.mean - 16) < 1e-6 * 16 and r.sdev < 1e-6
+
+
+def test_nan_raises(va):
+    @va.batchintegrand
+    def f(x):
+        v = np.ones(x.shape[0])
+        v[3] = np.nan
+        return v
+    with pytest.raises(ValueError):
+        va.Integrator([[0, 1]] * 2, neval=1000)(f, nitn=2)
+
+    @va.batchintegrand
+    def g(x):
+        return 1 / 0.
+    with pytest.raises(ZeroDivisionError):
+        va.Integrator([[0, 1]] * 2, neval=1000)(g, nitn=2)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_config1_statistics(va, prec):
+    # 4-D Gaussian, neval 1e5, nitn 10
+    N = (100 / math.pi) ** 2 / math.erf(5) ** 4
+    f = va.GaussianMixture([N], [100.], [[0.5] * 4])
+    integ = va.Integrator([[0, 1]] * 4, neval=1e5, seed=2026, precision=prec)
+    integ(f, nitn=10)
+    r = integ(f, nitn=10)
+    assert abs(r.mean - 1.0) < 5 * r.sdev
+    assert r.Q > 1e-3
+    assert r.sdev < 2e-4
+
+
+def test_config2_statistics(va):
+    # 8-D double peak, neval 1e7, nitn 15 (FP32 integrand)
+    f = double_peak(va)
+    integ = va.Integrator([[0, 1]] * 8, neval=1e7, seed=7)
+    assert list(integ.nstrat) == [6] * 6 + [5] * 2
+    r = integ(f, nitn=15)
+    rr = integ(f, nitn=5)
+    assert abs(rr.mean - 1.0) < 5 * rr.sdev
+    assert rr.Q > 1e-3
+    assert abs(r.mean - 1.0) < 5 * max(r.sdev, 1e-3)
+    # device running average agrees with the host one
+    dev = integ._last_device_ravg
+    assert abs(dev[0] - rr.mean) <= 1e-12 * abs(rr.mean)
+    assert abs(dev[1] - rr.sdev) <= 1e-9 * rr.sdev
+
+
+def test_random_batch_integral(va):
+    # tests/test_vegas.py:1070-1087
+    f = va.GaussianMixture([1.0], [100.], [[0.5] * 4])
+    integ = va.Integrator(4 * [[0, 1]], seed=9)
+    integ(f, nitn=10, neval=1000)
+    result = integ(f, nitn=1, neval=1000, adapt=False)
+    integral = 0.0
+    for x, wgt in integ.random_batch():
+        integral += wgt.dot(f(x))
+    assert abs(result.mean - integral) < 5 * result.sdev
+
+
+def test_seed_determinism(va):
+    f = double_peak(va, dim=4, sigma=0.05)
+    a = va.Integrator([[0, 1]] * 4, neval=2e5, seed=42)(f, nitn=4)
+    b = va.Integrator([[0, 1]] * 4, neval=2e5, seed=42)(f, nitn=4)
+    for x, y in zip(a.itn_results[:1], b.itn_results[:1]):
+        assert x.mean == y.mean and x.sdev == y.sdev

@@ -1,0 +1,265 @@
+// vgx_adapt.hip -- AdaptiveMap.adapt (src/vegas/_vegas.pyx:467-594) on device.
+//
+// Per axis: average the training data, smooth and compress it (alpha > 0),
+// then invert the cumulative distribution onto new_ninc equal-weight
+// increments.  Two implementations:
+//   exact     one lane walks each axis with the reference's order of
+//             operations (sequential sums, the while-loop CDF walk);
+//   parallel  one 1024-lane workgroup per axis: tree reductions, an LDS
+//             prefix sum and a binary search per new node.  Node i lands in
+//             the first old increment j with prefix P_j >= i*F, exactly the
+//             increment the sequential walk stops in (its running
+//             accumulator equals P_j - i*F); only rounding differs.
+#include "vgx_internal.h"
+
+namespace vgx {
+
+struct AdaptArgs {
+    int dim;
+    int ld;       // old grid stride
+    int nb;       // ld - 1, stride of sum_f / n_f
+    int new_ld;
+    int has_training;
+    int chain;    // exact mode: one lane for all axes, buffer shared (pyx:534)
+    double alpha;
+    int32_t ninc[VGX_MAXDIM];
+    int32_t new_ninc[VGX_MAXDIM];
+    const double* grid;
+    const double* sum_f;
+    const double* n_f;
+    double* new_grid;
+    double* buf;  // exact mode scratch: avg and tmp, 2 x nb per axis (chain: one pair)
+};
+
+// Fill the padding of one output row and nothing else.
+__device__ inline void pad_row(double* ng, int nn, int new_ld) {
+    for (int i = nn + 1; i < new_ld; ++i) ng[i] = ng[nn];
+}
+
+__device__ void adapt_axis_exact(const AdaptArgs& a, int d, double* avg, double* tmp) {
+    const int old = a.ninc[d], nn = a.new_ninc[d];
+    const double* g = a.grid + (int64_t)d * a.ld;
+    double* ng = a.new_grid + (int64_t)d * a.new_ld;
+    if (a.alpha != 0 && old > 1) {
+        if (a.has_training) {
+            for (int i = 0; i < old; ++i) {
+                double nf = a.n_f[(int64_t)d * a.nb + i];
+                avg[i] = nf > 0 ? a.sum_f[(int64_t)d * a.nb + i] / nf : 0.;
+            }
+        }
+        if (a.alpha > 0) {
+            tmp[0] = fabs(7. * avg[0] + avg[1]) / 8.;
+            tmp[old - 1] = fabs(7. * avg[old - 1] + avg[old - 2]) / 8.;
+            double s = tmp[0] + tmp[old - 1];
+            for (int i = 1; i < old - 1; ++i) {
+                tmp[i] = fabs(6. * avg[i] + avg[i - 1] + avg[i + 1]) / 8.;
+                s += tmp[i];
+            }
+            if (s > 0) {
+                for (int i = 0; i < old; ++i) avg[i] = tmp[i] / s + kTiny;
+            } else {
+                for (int i = 0; i < old; ++i) avg[i] = kTiny;
+            }
+            for (int i = 0; i < old; ++i)
+                if (avg[i] > 0 && avg[i] <= 0.99999999)
+                    avg[i] = pow(-(1 - avg[i]) / log(avg[i]), a.alpha);
+        }
+    }
+    ng[0] = g[0];
+    ng[nn] = g[old];
+    double acc = 0., f_ninc = 0.;
+    int j = -1;
+    for (int i = 0; i < old; ++i) f_ninc += avg[i];
+    f_ninc /= (double)nn;
+    int i = 1;
+    for (; i < nn; ++i) {
+        bool stopped = false;
+        while (acc < f_ninc) {
+            j += 1;
+            if (j < old) acc += avg[j];
+            else { stopped = true; break; }
+        }
+        if (stopped) break;
+        acc -= f_ninc;
+        ng[i] = g[j + 1] - (acc / avg[j]) * (g[j + 1] - g[j]);
+    }
+    // The reference leaves nodes after an early stop unset (round-off only);
+    // pin them to the upper limit.
+    for (; i < nn; ++i) ng[i] = g[old];
+    pad_row(ng, nn, a.new_ld);
+}
+
+__global__ void adapt_exact_kernel(AdaptArgs a) {
+    if (threadIdx.x != 0) return;
+    if (a.chain) {
+        double* avg = a.buf;
+        double* tmp = a.buf + a.nb;
+        for (int i = 0; i < a.nb; ++i) avg[i] = 1.0;
+        for (int d = 0; d < a.dim; ++d) adapt_axis_exact(a, d, avg, tmp);
+    } else {
+        const int d = blockIdx.x;
+        double* avg = a.buf + (int64_t)d * 2 * a.nb;
+        double* tmp = avg + a.nb;
+        for (int i = 0; i < a.nb; ++i) avg[i] = 1.0;
+        adapt_axis_exact(a, d, avg, tmp);
+    }
+}
+
+// Deterministic block-wide sum (fixed tree over lanes, then waves).
+__device__ double block_sum(double v, double* red) {
+    v = wave_sum_fixed(v);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < nw; ++k) t += red[k];
+        red[32] = t;
+    }
+    __syncthreads();
+    t = red[32];
+    __syncthreads();
+    return t;
+}
+
+constexpr int kAdaptLdsMax = 4096;  // old increments handled in LDS (2 x 32 KB)
+
+__global__ __launch_bounds__(1024) void adapt_parallel_kernel(AdaptArgs a) {
+    __shared__ double av[kAdaptLdsMax];
+    __shared__ double P[kAdaptLdsMax];
+    __shared__ double red[40];
+    const int d = blockIdx.x;
+    const int old = a.ninc[d], nn = a.new_ninc[d];
+    const double* g = a.grid + (int64_t)d * a.ld;
+    double* ng = a.new_grid + (int64_t)d * a.new_ld;
+    const int T = blockDim.x;
+
+    for (int i = threadIdx.x; i < old; i += T) {
+        double v = 1.0;
+        if (a.alpha != 0 && old > 1 && a.has_training) {
+            double nf = a.n_f[(int64_t)d * a.nb + i];
+            v = nf > 0 ? a.sum_f[(int64_t)d * a.nb + i] / nf : 0.;
+        }
+        av[i] = v;
+    }
+    __syncthreads();
+    if (a.alpha > 0 && old > 1) {
+        double part = 0.0;
+        for (int i = threadIdx.x; i < old; i += T) {
+            double t;
+            if (i == 0) t = fabs(7. * av[0] + av[1]) / 8.;
+            else if (i == old - 1) t = fabs(7. * av[old - 1] + av[old - 2]) / 8.;
+            else t = fabs(6. * av[i] + av[i - 1] + av[i + 1]) / 8.;
+            P[i] = t;
+            part += t;
+        }
+        const double s = block_sum(part, red);
+        for (int i = threadIdx.x; i < old; i += T) {
+            double v = s > 0 ? P[i] / s + kTiny : kTiny;
+            if (v > 0 && v <= 0.99999999) v = pow(-(1 - v) / log(v), a.alpha);
+            av[i] = v;
+        }
+        __syncthreads();
+    }
+    // inclusive prefix sums of av (chunked Hillis-Steele; fixed order)
+    double carry = 0.0;
+    for (int base = 0; base < old; base += T) {
+        const int i = base + threadIdx.x;
+        double v = i < old ? av[i] : 0.0;
+        for (int o = 1; o < T; o <<= 1) {
+            double add = 0.0;
+            if (i < old) P[i] = v;
+            __syncthreads();
+            if ((int)threadIdx.x >= o && i < old) add = P[i - o];
+            __syncthreads();
+            v += add;
+        }
+        if (i < old) P[i] = carry + v;
+        __syncthreads();
+        const int last = (base + T < old ? base + T : old) - 1;
+        carry = P[last];
+        __syncthreads();
+    }
+    const double total = P[old - 1];
+    const double F = total / (double)nn;
+    if (threadIdx.x == 0) {
+        ng[0] = g[0];
+        ng[nn] = g[old];
+    }
+    for (int i = 1 + threadIdx.x; i < nn; i += T) {
+        const double target = (double)i * F;
+        int lo = 0, hi = old;  // first j with P[j] >= target
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (P[mid] >= target) hi = mid; else lo = mid + 1;
+        }
+        double xv;
+        if (lo >= old) {
+            xv = g[old];
+        } else {
+            const int j = lo;
+            const double acc = P[j] - target;
+            xv = g[j + 1] - (acc / av[j]) * (g[j + 1] - g[j]);
+        }
+        ng[i] = xv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) pad_row(ng, nn, a.new_ld);
+}
+
+}  // namespace vgx
+
+using namespace vgx;
+
+extern "C" int vgx_adapt(vgx_ctx* ctx, int dim, const int32_t* ninc, const double* grid, int ld,
+                         int has_training, const double* sum_f, const double* n_f, double alpha,
+                         const int32_t* new_ninc, double* new_grid, int new_ld, int exact,
+                         void* stream) {
+    if (dim < 1 || dim > VGX_MAXDIM || !ninc || !new_ninc || !grid || !new_grid) return VGX_EINVAL;
+    if (grid == new_grid) return VGX_EINVAL;
+    if (has_training && (!sum_f || !n_f)) return VGX_EINVAL;
+    AdaptArgs a{};
+    a.dim = dim;
+    a.ld = ld;
+    a.nb = ld - 1;
+    a.new_ld = new_ld;
+    a.has_training = has_training;
+    a.alpha = alpha;
+    a.grid = grid;
+    a.sum_f = sum_f;
+    a.n_f = n_f;
+    a.new_grid = new_grid;
+    int max_new = 0, min_new = 1 << 30, max_old = 0;
+    for (int d = 0; d < dim; ++d) {
+        if (ninc[d] < 1 || ninc[d] + 1 > ld) return VGX_EINVAL;
+        a.ninc[d] = ninc[d];
+        a.new_ninc[d] = new_ninc[d];
+        max_new = new_ninc[d] > max_new ? new_ninc[d] : max_new;
+        min_new = new_ninc[d] < min_new ? new_ninc[d] : min_new;
+        max_old = ninc[d] > max_old ? ninc[d] : max_old;
+    }
+    if (min_new < 1) return VGX_EINVAL;
+    if (max_new + 1 > new_ld) return VGX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    // pyx:519-530: every axis collapses to its end points
+    if (max_new == 1) {
+        a.alpha = 0.0;  // the CDF walk with one increment writes only the ends
+        exact = 1;
+    }
+    // pyx:534: with alpha > 0 and no training data the shared buffer carries
+    // one axis' smoothed weights into the next; only the chained walk
+    // reproduces that, and only when axes differ in ninc does it matter.
+    a.chain = (!has_training && alpha > 0) ? 1 : 0;
+    if (!exact && max_old > kAdaptLdsMax) exact = 1;
+    if (exact || a.chain) {
+        if (!ctx) return VGX_EINVAL;
+        size_t need = (size_t)(a.chain ? 1 : dim) * 2 * (size_t)a.nb * sizeof(double);
+        int rc = ctx->adapt_tmp.ensure(need);
+        if (rc) return rc;
+        a.buf = (double*)ctx->adapt_tmp.ptr;
+        hipLaunchKernelGGL(adapt_exact_kernel, dim3(a.chain ? 1 : dim), dim3(64), 0, st, a);
+    } else {
+        hipLaunchKernelGGL(adapt_parallel_kernel, dim3(dim), dim3(1024), 0, st, a);
+    }
+    return check_launch();
+}

@@ -1,0 +1,76 @@
+// vgx_ctx.hip -- context, error strings and device queries of libvgx.
+#include <string.h>
+
+#include "vgx_internal.h"
+
+namespace vgx {
+
+int device_info(DeviceInfo* out) {
+    int dev = 0;
+    VGX_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    VGX_HIP(hipGetDeviceProperties(&p, dev));
+    if (strncmp(p.gcnArchName, "gfx950", 6) != 0) return VGX_ENODEV;
+    out->device = dev;
+    out->cus = p.multiProcessorCount;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        lds = 65536;
+    out->max_lds = (size_t)lds;
+    return VGX_OK;
+}
+
+}  // namespace vgx
+
+extern "C" const char* vgx_strerror(int status) {
+    switch (status) {
+        case VGX_OK: return "ok";
+        case VGX_EINVAL: return "invalid argument";
+        case VGX_EHIP: return "HIP runtime error";
+        case VGX_ENAN: return "integrand evaluates to nan";
+        case VGX_ENOMEM: return "device allocation failed";
+        case VGX_ERANGE: return "size beyond kernel limits";
+        case VGX_ENODEV: return "no gfx950 device";
+        default: return "unknown vgx status";
+    }
+}
+
+extern "C" int vgx_version(void) { return 1; }
+
+extern "C" int vgx_ctx_create(vgx_ctx** out) {
+    if (!out) return VGX_EINVAL;
+    vgx_ctx* c = new vgx_ctx();
+    int rc = vgx::device_info(&c->info);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    c->device = c->info.device;
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return VGX_EHIP;
+    }
+    *out = c;
+    return VGX_OK;
+}
+
+extern "C" int vgx_ctx_destroy(vgx_ctx* c) {
+    if (!c) return VGX_OK;
+    (void)hipDeviceSynchronize();
+    vgx::Scratch* all[] = {&c->hist_cnt, &c->hist_sorted, &c->hist_start, &c->nh, &c->off,
+                           &c->tile_h, &c->blocksum, &c->tile_slots, &c->blk_part,
+                           &c->strad_part, &c->wf_stage, &c->misc, &c->adapt_tmp, &c->errv};
+    for (auto* s : all) s->release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c;
+    return VGX_OK;
+}
+
+extern "C" int vgx_last_fused_ms(vgx_ctx* c, float* ms) {
+    if (!c || !ms) return VGX_EINVAL;
+    if (!c->have_timing) return VGX_EINVAL;
+    VGX_HIP(hipEventSynchronize(c->ev1));
+    VGX_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return VGX_OK;
+}

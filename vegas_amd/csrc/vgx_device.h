@@ -1,0 +1,141 @@
+// vgx_device.h -- device helpers shared by the libvgx kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vgx.h"
+
+namespace vgx {
+
+// Constants of the reference (src/vegas/_vegas.pyx:34-36).
+constexpr double kTiny = 1e-257;
+constexpr double kEpsilon = 2.220446049250313080847263336181640625e-16 * 1e4;
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Counter layout shared
+// with the sampler: ctr = (p lo, p hi, iteration, block), key = (seed lo, hi).
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        U4 n;
+        n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+        n.y = (uint32_t)p1;
+        n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+        n.w = (uint32_t)p0;
+        c = n;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// 24-bit uniform in [0,1) from one word (exact in float and double).
+__host__ __device__ inline double u24(uint32_t w) { return (double)(w >> 8) * (1.0 / 16777216.0); }
+// 53-bit uniform in [0,1) from two words.
+__host__ __device__ inline double u53(uint32_t a, uint32_t b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------------------------
+// Division of 32-bit unsigned n by an invariant divisor d >= 1 with one
+// multiply-high (Granlund & Montgomery, PLDI'94, section 4):
+//   l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+//   q = (mulhi(m, n) + n) >> l      (33-bit intermediate; exact for all n).
+struct DivMagic {
+    uint32_t m, l, d, pad;
+};
+
+inline DivMagic make_divmagic(uint32_t d) {
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    uint64_t m = (((1ull << 32) * ((1ull << l) - d)) / d) + 1;
+    return DivMagic{(uint32_t)m, l, d, 0};
+}
+
+__host__ __device__ inline uint32_t divq(uint32_t n, const DivMagic& g) {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t hi = __umulhi(n, g.m);
+#else
+    uint32_t hi = (uint32_t)(((uint64_t)n * g.m) >> 32);
+#endif
+    return (uint32_t)(((uint64_t)hi + n) >> g.l);
+}
+
+// ---------------------------------------------------------------------------
+// Compensated (double-double) accumulation: Knuth's TwoSum.  The library is
+// built with -ffp-contract=off, so these are exact error-free transforms.
+struct DD {
+    double hi, lo;
+};
+
+__host__ __device__ inline void dd_add(DD& a, double b) {
+    double s = a.hi + b;
+    double bp = s - a.hi;
+    double e = (a.hi - (s - bp)) + (b - bp);
+    a.hi = s;
+    a.lo += e;
+}
+
+__host__ __device__ inline DD dd_sum(DD a, DD b) {
+    double s = a.hi + b.hi;
+    double bp = s - a.hi;
+    double e = (a.hi - (s - bp)) + (b.hi - bp);
+    e += a.lo + b.lo;
+    double hi = s + e;
+    return DD{hi, e - (hi - s)};
+}
+
+__host__ __device__ inline double dd_value(DD a) { return a.hi + a.lo; }
+
+// ---------------------------------------------------------------------------
+// Wave-level helpers (wave64).
+__device__ inline int lane_id() { return (int)__lane_id(); }
+
+__device__ inline double wave_sum_fixed(double v) {
+    // Butterfly in a fixed order: deterministic for a given lane assignment.
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ inline DD wave_sum_dd(DD v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        DD w{__shfl_xor(v.hi, o, kWave), __shfl_xor(v.lo, o, kWave)};
+        // combine in lane-order-independent form: smaller lane index first
+        v = (lane_id() & o) ? dd_sum(w, v) : dd_sum(v, w);
+    }
+    return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Segmented inclusive scan across the 64 lanes.  head = 1 when the lane
+// starts a new segment.  Order of additions is fixed (Hillis-Steele).
+__device__ inline double wave_segscan(double v, int head) {
+    int lane = lane_id();
+    int hflag = head;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        double pv = __shfl_up(v, o, kWave);
+        int ph = __shfl_up(hflag, o, kWave);
+        if (lane >= o && !hflag) {
+            v = pv + v;
+        }
+        if (lane >= o) hflag |= ph;
+    }
+    return v;
+}
+
+}  // namespace vgx

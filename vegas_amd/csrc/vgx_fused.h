@@ -1,0 +1,437 @@
+// vgx_fused.h -- the fused vegas+ sampler kernel (template; instantiated per
+// dimension and integrand in vgx_fused_*.hip).
+//
+// One wave owns a tile of kT consecutive points of the iteration (kK per
+// lane, lane-contiguous).  Per point: Philox4x32-10 uniforms -> stratified y
+// (pyx:1734-1741) -> map with the grid in LDS (pyx:347-359) -> weight
+// J*(dv_y/n_h) (pyx:1749) -> integrand -> w f.  Per hcube: the reference's
+// two-pass sums with the EPSILON guard (pyx:2146-2183) as wave-level
+// segmented scans over the tile; hcubes cut by a tile edge are finished by
+// the straddler kernels.  Training data (w f n)^2 (pyx:2173) goes into an
+// LDS-privatised histogram (FP64 sums, packed u16 counts).
+#pragma once
+#include "vgx_iter.h"
+
+namespace vgx {
+
+// Per-axis constants of the sampler, staged in LDS by the fused kernel.
+struct AxisC {
+    uint32_t m, l, nstrat, ninc;  // DivMagic of nstrat, strata and increments
+    double inv_s;                 // 1 / nstrat
+};
+
+// Generates uniforms, y, x and J for point p of hcube h (R32: 24-bit
+// uniforms, one Philox block per 4 axes; else 53-bit, one per 2 axes).
+template <int DIM, bool R32>
+__device__ __forceinline__ void eval_point(const IterGeom& g, const AxisC* __restrict__ ax,
+                                           const double* __restrict__ G, uint32_t h, uint32_t p,
+                                           double* x, double& J, int* kbin) {
+    const int dim = DIM > 0 ? DIM : g.dim;
+    uint32_t t = h;
+    U4 w{0, 0, 0, 0};
+    J = 1.0;
+#pragma unroll
+    for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
+        if (DIM == 0 && d >= dim) break;
+        const AxisC a = ax[d];
+        const uint32_t q = (uint32_t)(((uint64_t)__umulhi(t, a.m) + t) >> a.l);
+        const uint32_t c = t - q * a.nstrat;
+        t = q;
+        double u;
+        if (R32) {
+            if ((d & 3) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
+            const int e = d & 3;
+            u = u24(e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w);
+        } else {
+            if ((d & 1) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
+            u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);
+        }
+        const double yv = g.exact_div ? ((double)c + u) / (double)a.nstrat : ((double)c + u) * a.inv_s;
+        const int ni = (int)a.ninc;
+        const double tt = fmin(yv * (double)ni, (double)ni);
+        const int k = (int)tt;
+        const double dl = tt - (double)k;
+        const double* gd = G + d * g.ld;
+        if (k < ni) {
+            const double inc = gd[k + 1] - gd[k];
+            x[d] = gd[k] + inc * dl;
+            J *= inc * (double)ni;
+        } else {
+            x[d] = gd[ni];
+            J *= (gd[ni] - gd[ni - 1]) * (double)ni;
+        }
+        kbin[d] = (yv > 0 && yv < 1) ? (k < ni ? k : ni - 1) : -1;
+    }
+}
+
+__host__ __device__ inline AxisC make_axis(const IterGeom& g, int d) {
+    return AxisC{g.dm[d].m, g.dm[d].l, (uint32_t)g.nstrat[d], (uint32_t)g.ninc[d], g.inv_nstrat[d]};
+}
+
+// Two-value version of seg_prefix (shares the segment flags).
+__device__ __forceinline__ void seg_prefix2(const int (&key)[kK], double (&a)[kK], double (&b)[kK],
+                                            int lane) {
+#pragma unroll
+    for (int j = 1; j < kK; ++j)
+        if (key[j] == key[j - 1]) {
+            a[j] += a[j - 1];
+            b[j] += b[j - 1];
+        }
+    int prev_last = __shfl_up(key[kK - 1], 1, kWave);
+    if (lane == 0) prev_last = -1;
+    int hflag = key[kK - 1] != prev_last;
+    double va = a[kK - 1], vb = b[kK - 1];
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const double pa = __shfl_up(va, o, kWave);
+        const double pb = __shfl_up(vb, o, kWave);
+        const int ph = __shfl_up(hflag, o, kWave);
+        if (lane >= o && !hflag) {
+            va = pa + va;
+            vb = pb + vb;
+        }
+        if (lane >= o) hflag |= ph;
+    }
+    const double ca = __shfl_up(va, 1, kWave), cb = __shfl_up(vb, 1, kWave);
+    const bool carry = lane > 0 && key[0] == prev_last;
+#pragma unroll
+    for (int j = 0; j < kK; ++j)
+        if (carry && key[j] == key[0]) {
+            a[j] += ca;
+            b[j] += cb;
+        }
+}
+
+// Sum of v over the segment of equal keys up to and including each element
+// (lane-contiguous elements, keys non-decreasing across the wave).
+__device__ __forceinline__ void seg_prefix(const int (&key)[kK], const double (&v)[kK],
+                                           double (&P)[kK], int lane) {
+    P[0] = v[0];
+#pragma unroll
+    for (int j = 1; j < kK; ++j) P[j] = (key[j] == key[j - 1]) ? P[j - 1] + v[j] : v[j];
+    int prev_last = __shfl_up(key[kK - 1], 1, kWave);
+    if (lane == 0) prev_last = -1;
+    const int head = key[kK - 1] != prev_last;
+    const double incl = wave_segscan(P[kK - 1], head);
+    double prev_incl = __shfl_up(incl, 1, kWave);
+    const double cin = (lane > 0 && key[0] == prev_last) ? prev_incl : 0.0;
+#pragma unroll
+    for (int j = 0; j < kK; ++j)
+        if (key[j] == key[0]) P[j] += cin;
+}
+
+template <int DIM, class Fcn>
+__global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedPtrs io, Fcn fcn) {
+    constexpr bool R32 = Fcn::kRng32;
+    extern __shared__ __align__(16) unsigned char smem[];
+    Scalars* sc = io.sc;
+    if (sc->status) return;  // NaN seen earlier in this run: nothing more is done
+    const int dim = DIM > 0 ? DIM : g.dim;
+    const uint32_t ntiles = sc->b_ntiles, p0 = sc->b_p0, p1 = sc->b_p1, hend = sc->b_h1;
+    const int tot = dim * g.nb;
+
+    // ---- LDS carve-up ----
+    unsigned char* cur = smem;
+    AxisC* ax = reinterpret_cast<AxisC*>(cur);
+    if (threadIdx.x < (unsigned)dim) ax[threadIdx.x] = io.axes[threadIdx.x];
+    cur += (size_t)VGX_MAXDIM * sizeof(AxisC);
+    double* Lg = reinterpret_cast<double*>(cur);
+    for (int i = threadIdx.x; i < dim * g.ld; i += blockDim.x) Lg[i] = io.grid[i];
+    const double* G = Lg;
+    cur += (size_t)dim * g.ld * sizeof(double);
+    double* Ls = nullptr;
+    uint32_t* Lc = nullptr;
+    if (g.hist_lds) {
+        Ls = reinterpret_cast<double*>(cur);
+        cur += (size_t)tot * sizeof(double);
+        Lc = reinterpret_cast<uint32_t*>(cur);
+        cur += (size_t)((tot + 1) / 2) * sizeof(uint32_t);
+        for (int i = threadIdx.x; i < tot; i += blockDim.x) Ls[i] = 0.0;
+        for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) Lc[i] = 0u;
+    }
+    cur = reinterpret_cast<unsigned char*>(((uintptr_t)cur + 15) & ~(uintptr_t)15);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nwave = blockDim.x >> 6;
+    uint32_t* offs = reinterpret_cast<uint32_t*>(cur) + wave * kNoff;
+    cur += (size_t)nwave * kNoff * sizeof(uint32_t);
+    double* Lred = reinterpret_cast<double*>(((uintptr_t)cur + 15) & ~(uintptr_t)15);
+    __syncthreads();
+
+    const uint32_t chunk = (ntiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t tb = blockIdx.x * chunk;
+    const uint32_t te = min(tb + chunk, ntiles);
+    const uint32_t nrounds = (chunk + nwave - 1) / nwave;
+
+    DD am{0.0, 0.0}, av{0.0, 0.0};
+    double wmax = 0.0;
+    int nan_seen = 0;
+    const double eps = kEpsilon;
+
+    for (uint32_t r = 0; r < nrounds; ++r) {
+        const uint32_t t = tb + r * nwave + wave;
+        if (t < te) {
+            // ------------------------------------------------ one wave tile
+            const uint32_t plo = p0 + t * kT;
+            const uint32_t phi = min(plo + (uint32_t)kT, p1);
+            const uint32_t hf = (uint32_t)io.tile_h[t];
+            for (int i = lane; i < kNoff; i += kWave) {
+                uint32_t hh = hf + i;
+                offs[i] = io.off[hh < hend ? hh : hend];
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // hcube (local index) of this lane's first point
+            const uint32_t pl0 = plo + lane * kK;
+            int li = 0;
+            {
+                int lo = 0, hi = kNoff - 2;  // largest i with offs[i] <= pl0
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (offs[mid] <= pl0) lo = mid; else hi = mid - 1;
+                }
+                li = lo;
+            }
+            // Points are evaluated one at a time (no unrolling: one point's
+            // registers live at once); results rotate through a register
+            // window so element j ends in key[j], wf[j], nn[j].
+            int key[kK];
+            double wf[kK];
+            uint32_t nn[kK];
+#pragma unroll
+            for (int j = 0; j < kK; ++j) {
+                key[j] = kKeySentinel;
+                wf[j] = 0.0;
+                nn[j] = 1;
+            }
+            int cur_i = -1;
+            double wdiv = 0.0;
+            uint32_t ncur = 1;
+#pragma unroll 1
+            for (int j = 0; j < kK; ++j) {
+                const uint32_t pj = pl0 + j;
+                int kj = kKeySentinel;
+                double vj = 0.0;
+                uint32_t nj = 1;
+                if (pj < phi) {
+                    while (li < kNoff - 2 && offs[li + 1] <= pj) ++li;
+                    if (li != cur_i) {
+                        cur_i = li;
+                        ncur = offs[li + 1] - offs[li];
+                        wdiv = g.dv_y / (double)ncur;
+                    }
+                    kj = li;
+                    nj = ncur;
+                    double x[DIM > 0 ? DIM : VGX_MAXDIM];
+                    int kb[DIM > 0 ? DIM : VGX_MAXDIM];
+                    double J;
+                    eval_point<DIM, R32>(g, ax, G, hf + li, pj, x, J, kb);
+                    const double w = J * wdiv;
+                    const double f = fcn.template eval<DIM>(x, dim, pj);
+                    if (f != f) nan_seen = 1;
+                    vj = w * f;
+                    const double tn = vj * (double)ncur;
+                    wmax = fmax(wmax, fabs(tn));
+                    if (g.train_points) {
+                        const double c2 = tn * tn;
+#pragma unroll
+                        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
+                            if (DIM == 0 && d >= dim) break;
+                            const int k = kb[d];
+                            if (k < 0) continue;
+                            const int idx = d * g.nb + k;
+                            if (g.hist_lds) {
+                                unsafeAtomicAdd(&Ls[idx], c2);
+                                atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
+                            } else {
+                                unsafeAtomicAdd(&io.sum_f[idx], c2);
+                                unsafeAtomicAdd(&io.n_f[idx], 1.0);
+                            }
+                        }
+                    }
+                }
+                // rotate: element j enters at the back of the window
+#pragma unroll
+                for (int q = 0; q < kK - 1; ++q) {
+                    key[q] = key[q + 1];
+                    wf[q] = wf[q + 1];
+                    nn[q] = nn[q + 1];
+                }
+                key[kK - 1] = kj;
+                wf[kK - 1] = vj;
+                nn[kK - 1] = nj;
+            }
+            // ---- pass 1: S per segment (pyx:2147-2152)
+            double P[kK];
+            seg_prefix(key, wf, P, lane);
+            int endf[kK];
+            const int next_first = __shfl_down(key[0], 1, kWave);
+#pragma unroll
+            for (int j = 0; j < kK - 1; ++j) endf[j] = key[j + 1] != key[j];
+            endf[kK - 1] = (lane == kWave - 1) || (next_first != key[kK - 1]);
+            double vlast = P[kK - 1];
+            int fl = endf[kK - 1];
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                double nv = __shfl_down(vlast, o, kWave);
+                int nf = __shfl_down(fl, o, kWave);
+                if (!fl && lane + o < kWave) {
+                    vlast = nv;
+                    fl = nf;
+                }
+            }
+            double S[kK];
+            {
+                double c = vlast;
+#pragma unroll
+                for (int j = kK - 1; j >= 0; --j) {
+                    if (j < kK - 1 && endf[j]) c = P[j];
+                    S[j] = c;
+                }
+            }
+            // straddling segments: first (local 0) and last (kL)
+            int kL;
+            {
+                int lo = 0, hi = kNoff - 2;  // largest i with offs[i] < phi
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (offs[mid] < phi) lo = mid; else hi = mid - 1;
+                }
+                kL = lo;
+            }
+            const bool sA = offs[0] < plo || offs[1] > phi;
+            const bool sB = offs[kL + 1] > phi;
+            // ---- pass 2: guarded deviations (pyx:2153-2169)
+            double qv[kK], dv[kK];
+#pragma unroll
+            for (int j = 0; j < kK; ++j) {
+                qv[j] = 0.0;
+                dv[j] = 0.0;
+                const int k = key[j];
+                if (k == kKeySentinel) continue;
+                const bool strad = (k == 0 && sA) || (k == kL && sB);
+                if (strad) {
+                    io.wf_stage[pl0 + j] = wf[j];
+                    continue;
+                }
+                const double m = S[j] / (double)nn[j];
+                double dd = wf[j] - m;
+                const double thr = eps * fabs(m);
+                if (fabs(dd) < thr) {
+                    qv[j] = thr * thr;
+                    dv[j] = 0.0;
+                } else {
+                    qv[j] = dd * dd;
+                    dv[j] = dd;
+                }
+            }
+            seg_prefix2(key, qv, dv, lane);
+            // Segment ends, one element at a time (a single copy of pow):
+            // element j sits at index 0 of the rotating windows.
+#pragma unroll 1
+            for (int j = 0; j < kK; ++j) {
+                const int k = key[0];
+                if (endf[0] && k != kKeySentinel) {
+                    const bool strad = (k == 0 && sA) || (k == kL && sB);
+                    const uint32_t h = hf + k;
+                    if (!strad) {
+                        const double n = (double)nn[0];
+                        const double Q = qv[0], D = dv[0];
+                        const double varc = (n * Q - D * D) / (n - 1.);
+                        dd_add(am, S[0] + D);
+                        dd_add(av, varc);
+                        // |var_h|; sigf_pow_kernel raises it to beta/2 (pyx:2185)
+                        if (g.adaptive_strat) io.sigf[h] = fabs(varc);
+                        if (g.train_errors) io.errv[h] = fabs(varc);
+                    }
+                    // slot A: end of the first segment; slot B: last valid point
+                    if (k == 0) {
+                        Slot sl{};
+                        sl.h = sA ? (int32_t)h : -1;
+                        sl.S = S[0];
+                        io.slots[2 * (size_t)t] = sl;
+                    }
+                    if (pl0 + j == phi - 1) {
+                        Slot sl{};
+                        sl.h = (kL != 0 && sB) ? (int32_t)h : -1;
+                        sl.S = S[0];
+                        io.slots[2 * (size_t)t + 1] = sl;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kK - 1; ++q) {
+                    key[q] = key[q + 1];
+                    endf[q] = endf[q + 1];
+                    nn[q] = nn[q + 1];
+                    S[q] = S[q + 1];
+                    qv[q] = qv[q + 1];
+                    dv[q] = dv[q + 1];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (g.hist_lds && g.train_points && ((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) {
+                const uint32_t wv = Lc[i];
+                if (wv) {
+                    if (wv & 0xffffu) unsafeAtomicAdd(&io.n_f[2 * i], (double)(wv & 0xffffu));
+                    if (wv >> 16) unsafeAtomicAdd(&io.n_f[2 * i + 1], (double)(wv >> 16));
+                    Lc[i] = 0u;
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- block epilogue ----
+    __syncthreads();
+    if (g.hist_lds && g.train_points) {
+        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+            const double s = Ls[i];
+            if (s != 0.0) unsafeAtomicAdd(&io.sum_f[i], s);
+        }
+        for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) {
+            const uint32_t wv = Lc[i];
+            if (wv & 0xffffu) unsafeAtomicAdd(&io.n_f[2 * i], (double)(wv & 0xffffu));
+            if ((wv >> 16) && 2 * i + 1 < tot) unsafeAtomicAdd(&io.n_f[2 * i + 1], (double)(wv >> 16));
+        }
+    }
+    am = wave_sum_dd(am);
+    av = wave_sum_dd(av);
+    wmax = wave_max(wmax);
+    const int anynan = __any(nan_seen);
+    if (lane == 0) {
+        Lred[wave * 6 + 0] = am.hi; Lred[wave * 6 + 1] = am.lo;
+        Lred[wave * 6 + 2] = av.hi; Lred[wave * 6 + 3] = av.lo;
+        Lred[wave * 6 + 4] = 0.0; Lred[wave * 6 + 5] = 0.0;
+        if (wmax > 0.0) atomicMax((unsigned long long*)&sc->wfn_max_bits,
+                                  (unsigned long long)__double_as_longlong(wmax));
+        if (anynan) atomicOr(&sc->status, (int32_t)VGX_ENAN);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD m{0, 0}, v{0, 0}, s{0, 0};
+        for (int k = 0; k < nwave; ++k) {
+            m = dd_sum(m, DD{Lred[k * 6 + 0], Lred[k * 6 + 1]});
+            v = dd_sum(v, DD{Lred[k * 6 + 2], Lred[k * 6 + 3]});
+            s = dd_sum(s, DD{Lred[k * 6 + 4], Lred[k * 6 + 5]});
+        }
+        double* o = io.blk_part + 6 * (size_t)blockIdx.x;
+        o[0] = m.hi; o[1] = m.lo; o[2] = v.hi; o[3] = v.lo; o[4] = s.hi; o[5] = s.lo;
+    }
+}
+
+template <int DIM, class Fcn>
+int launch_fused_dim(const IterGeom& g, const FusedPtrs& io, const Fcn& fcn, int blocks, size_t lds,
+                     hipStream_t st) {
+    auto kern = fused_kernel<DIM, Fcn>;
+    if (lds > 64 * 1024) {
+        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) != hipSuccess)
+            return VGX_EHIP;
+    }
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kFusedThreads), lds, st, g, io, fcn);
+    return check_launch();
+}
+
+}  // namespace vgx

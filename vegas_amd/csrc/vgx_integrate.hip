@@ -1,0 +1,899 @@
+// vgx_integrate.hip -- the vegas+ iteration driver (Integrator.__call__,
+// src/vegas/_vegas.pyx:2086-2227) and its non-fused kernels.
+//
+// Per iteration, all on one stream, no host synchronisation:
+//   iter_init      clear training data (AdaptiveMap.clear, pyx:596-599)
+//   alloc/scan/offsets   n_h by the damped-variance rule (pyx:1657-1706),
+//                  exclusive scan to point offsets, tile -> first hcube
+//   fused          sample, map, integrand, per-hcube sums, histogram
+//   strad0/1/2     hcubes cut by tile edges: totals, guarded deviations,
+//                  per-hcube results
+//   batch_reduce   iteration sums (compensated, fixed order)
+//   finalize       estimate, sum_sigf (pyx:2209-2215), running average
+//                  (pyx:2392-2410)
+//   adapt          AdaptiveMap.adapt(alpha) (pyx:2216-2217)
+#include <math.h>
+#include <string.h>
+
+#include "vgx_fused.h"
+
+namespace vgx {
+
+// --------------------------------------------------------------- init ----
+__global__ void iter_init_kernel(IterGeom g, double* sum_f, double* n_f, Scalars* sc, int clear_hist) {
+    const int tot = g.dim * g.nb;
+    if (clear_hist) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+            sum_f[i] = 0.0;
+            n_f[i] = kTiny;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->nh_min = g.nmin;
+        sc->nh_max = g.nmin;
+        sc->wfn_max_bits = 0;
+        for (int k = 0; k < 6; ++k) sc->acc[k] = 0.0;
+        sc->reset_sigf = 0;
+        sc->overflow = 0;
+    }
+}
+
+// ------------------------------------------------------------ allocation ----
+// n_h = min(int(sigf_h * neval_sigf) + nmin, max(nmax, nmin))   (pyx:1696-1698)
+__global__ __launch_bounds__(kAllocThreads) void alloc_kernel(IterGeom g, const double* sigf,
+                                                              const double* sum_sigf, int32_t* nh,
+                                                              uint64_t* blocksum, Scalars* sc) {
+    __shared__ uint64_t red[kAllocThreads / 64];
+    __shared__ int32_t rmin[kAllocThreads / 64], rmax[kAllocThreads / 64];
+    if (sc->status) return;
+    const double ss = *sum_sigf;
+    const double neval_sigf = (g.beta > 0 && ss > 0 && g.adaptive_sampler) ? g.neval_frac * g.neval / ss : 0.0;
+    uint64_t sum = 0;
+    int32_t mn = g.nmin, mx = g.nmin;
+    const int64_t base = (int64_t)blockIdx.x * kAllocPerBlock;
+    for (int i = 0; i < kAllocItems; ++i) {
+        const int64_t h = base + (int64_t)i * kAllocThreads + threadIdx.x;
+        if (h >= g.nhcube) break;
+        int32_t n;
+        if (g.adaptive_sampler) {
+            double v = sigf[h] * neval_sigf;
+            v = v < 2147483647.0 ? v : 2147483647.0;
+            int64_t nn = (int64_t)(int32_t)v + g.nmin;
+            if (nn > g.nmax) nn = g.nmax;
+            n = (int32_t)nn;
+            mn = n < mn ? n : mn;
+            mx = n > mx ? n : mx;
+        } else {
+            n = g.avg_nh;
+        }
+        nh[h] = n;
+        sum += (uint64_t)n;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o, kWave);
+        mn = min(mn, __shfl_xor(mn, o, kWave));
+        mx = max(mx, __shfl_xor(mx, o, kWave));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w] = sum;
+        rmin[w] = mn;
+        rmax[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s = 0;
+        for (int k = 0; k < kAllocThreads / 64; ++k) {
+            s += red[k];
+            mn = min(mn, rmin[k]);
+            mx = max(mx, rmax[k]);
+        }
+        blocksum[blockIdx.x] = s;
+        if (g.adaptive_sampler) {
+            atomicMin(&sc->nh_min, mn);
+            atomicMax(&sc->nh_max, mx);
+        }
+    }
+}
+
+// Exclusive scan of the per-block totals (one workgroup), total points and
+// the tile count of the whole-iteration batch.
+__global__ __launch_bounds__(1024) void scan_kernel(IterGeom g, uint64_t* blocksum, int nblk,
+                                                    Scalars* sc, uint64_t capacity) {
+    __shared__ uint64_t part[1024];
+    if (sc->status) return;
+    uint64_t carry = 0;
+    for (int base = 0; base < nblk; base += 1024) {
+        const int i = base + threadIdx.x;
+        const uint64_t v = i < nblk ? blocksum[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            uint64_t add = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+            __syncthreads();
+            part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < nblk) blocksum[i] = carry + part[threadIdx.x] - v;
+        const uint64_t ct = part[1023];
+        __syncthreads();
+        carry += ct;
+    }
+    if (threadIdx.x == 0) {
+        const int ok = carry <= capacity && carry < 0xFFFFFFFFull;
+        sc->overflow = ok ? 0u : 1u;
+        sc->total = ok ? (uint32_t)carry : 0u;
+        sc->b_p0 = 0;
+        sc->b_p1 = sc->total;
+        sc->b_h0 = 0;
+        sc->b_h1 = (uint32_t)g.nhcube;
+        sc->b_ntiles = (sc->total + kT - 1) / kT;
+    }
+}
+
+// Point offsets off[h] (exclusive scan of n_h) and tile -> first hcube.
+__global__ __launch_bounds__(kAllocThreads) void offsets_kernel(IterGeom g, const int32_t* nh,
+                                                                const uint64_t* blockoff,
+                                                                uint32_t* off, int32_t* tile_h,
+                                                                Scalars* sc) {
+    __shared__ int32_t ln[kAllocPerBlock];
+    __shared__ uint32_t wsum[kAllocThreads / 64];
+    if (sc->status || sc->overflow) return;
+    const int64_t base = (int64_t)blockIdx.x * kAllocPerBlock;
+    for (int i = threadIdx.x; i < kAllocPerBlock; i += kAllocThreads) {
+        const int64_t h = base + i;
+        ln[i] = h < g.nhcube ? nh[h] : 0;
+    }
+    __syncthreads();
+    uint32_t loc[kAllocItems];
+    uint32_t run = 0;
+    for (int i = 0; i < kAllocItems; ++i) {
+        loc[i] = run;
+        run += (uint32_t)ln[threadIdx.x * kAllocItems + i];
+    }
+    // exclusive scan of per-thread totals across the workgroup
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t incl = run;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t wbase = 0;
+    for (int k = 0; k < w; ++k) wbase += wsum[k];
+    const uint32_t tbase = (uint32_t)blockoff[blockIdx.x] + wbase + incl - run;
+    for (int i = 0; i < kAllocItems; ++i) {
+        const int64_t h = base + threadIdx.x * kAllocItems + i;
+        if (h >= g.nhcube) break;
+        const uint32_t lo = tbase + loc[i];
+        const uint32_t n = (uint32_t)ln[threadIdx.x * kAllocItems + i];
+        off[h] = lo;
+        const uint32_t hi = lo + n;
+        for (uint32_t t = (lo + kT - 1) / kT; t < (hi + kT - 1) / kT; ++t) tile_h[t] = (int32_t)h;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[g.nhcube] = sc->total;
+}
+
+// Staged batches: tiles of points [p0, p1) covering hcubes [h0, h1).
+__global__ void batch_tiles_kernel(const uint32_t* off, int32_t* tile_h, Scalars* sc, uint32_t h0,
+                                   uint32_t h1, uint32_t p0, uint32_t p1) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->b_p0 = p0;
+        sc->b_p1 = p1;
+        sc->b_h0 = h0;
+        sc->b_h1 = h1;
+        sc->b_ntiles = (p1 - p0 + kT - 1) / kT;
+    }
+    for (uint32_t h = h0 + blockIdx.x * blockDim.x + threadIdx.x; h < h1; h += gridDim.x * blockDim.x) {
+        const uint32_t lo = off[h] - p0, hi = off[h + 1] - p0;
+        for (uint32_t t = (lo + kT - 1) / kT; t < (hi + kT - 1) / kT; ++t) tile_h[t] = (int32_t)h;
+    }
+}
+
+// ---------------------------------------------------------- straddlers ----
+struct StradArgs {
+    double* errv;
+    const AxisC* axes;
+    const uint32_t* off;
+    Slot* slots;
+    const double* wf_stage;
+    double* sigf;
+    double* strad_part;
+    Scalars* sc;
+    double* sum_f;
+    double* n_f;
+    const double* grid;
+};
+
+__device__ inline int slot_of(const Slot* slots, uint32_t t, int32_t h) {
+    return slots[2 * (size_t)t].h == h ? (int)(2 * t) : (int)(2 * t + 1);
+}
+
+// Sum over the tiles [tf, tl] of one field of h's slot (lanes in parallel,
+// fixed butterfly).
+__device__ inline double strad_sum(const Slot* slots, uint32_t tf, uint32_t tl, int32_t h, int field) {
+    double v = 0.0;
+    for (uint32_t t = tf + lane_id(); t <= tl; t += kWave) {
+        const Slot& s = slots[slot_of(slots, t, h)];
+        v += field == 0 ? s.S : field == 1 ? s.Q : s.D;
+    }
+    return wave_sum_fixed(v);
+}
+
+// mode 0: owner writes Stot; mode 1: every part computes Q, D; mode 2:
+// owner combines and produces the hcube's results.
+template <int MODE>
+__global__ __launch_bounds__(256) void strad_kernel(IterGeom g, StradArgs a) {
+    __shared__ double red[4 * 6];
+    Scalars* sc = a.sc;
+    if (sc->status || sc->overflow) return;
+    const uint32_t ntiles = sc->b_ntiles, p0 = sc->b_p0, p1 = sc->b_p1;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t gw = blockIdx.x * 4 + wave, nw = gridDim.x * 4;
+    DD am{0, 0}, av{0, 0}, as{0, 0};
+    for (uint32_t t = gw; t < ntiles; t += nw) {
+        const uint32_t plo = p0 + t * kT;
+        const uint32_t phi = min(plo + (uint32_t)kT, p1);
+        for (int s = 0; s < 2; ++s) {
+            Slot* sl = &a.slots[2 * (size_t)t + s];
+            const int32_t h = sl->h;
+            if (h < 0) continue;
+            const uint32_t hlo = a.off[h], hhi = a.off[h + 1];
+            const uint32_t tf = (hlo - p0) / kT, tl = (hhi - 1 - p0) / kT;
+            const bool owner = hlo >= plo;
+            if (MODE == 0) {
+                if (!owner) continue;
+                const double S = strad_sum(a.slots, tf, tl, h, 0);
+                if (lane == 0) sl->Stot = S;
+            } else if (MODE == 1) {
+                const double Stot = a.slots[slot_of(a.slots, tf, h)].Stot;
+                const double n = (double)(hhi - hlo);
+                const double m = Stot / n;
+                const double thr = kEpsilon * fabs(m);
+                const uint32_t lo = max(hlo, plo), hi = min(hhi, phi);
+                double q = 0.0, d = 0.0;
+                for (uint32_t p = lo + lane; p < hi; p += kWave) {
+                    const double dd = a.wf_stage[p] - m;
+                    if (fabs(dd) < thr) {
+                        q += thr * thr;
+                    } else {
+                        q += dd * dd;
+                        d += dd;
+                    }
+                }
+                q = wave_sum_fixed(q);
+                d = wave_sum_fixed(d);
+                if (lane == 0) {
+                    sl->Q = q;
+                    sl->D = d;
+                }
+            } else {
+                if (!owner) continue;
+                const double S = sl->Stot;
+                const double Q = strad_sum(a.slots, tf, tl, h, 1);
+                const double D = strad_sum(a.slots, tf, tl, h, 2);
+                const double n = (double)(hhi - hlo);
+                const double varc = (n * Q - D * D) / (n - 1.);
+                if (lane == 0) {
+                    dd_add(am, S + D);
+                    dd_add(av, varc);
+                    if (g.adaptive_strat) a.sigf[h] = fabs(varc);
+                    if (g.train_errors) a.errv[h] = fabs(varc);
+                }
+            }
+        }
+    }
+    if (MODE == 2) {
+        if (lane == 0) {
+            red[wave * 6 + 0] = am.hi; red[wave * 6 + 1] = am.lo;
+            red[wave * 6 + 2] = av.hi; red[wave * 6 + 3] = av.lo;
+            red[wave * 6 + 4] = as.hi; red[wave * 6 + 5] = as.lo;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            DD m{0, 0}, v{0, 0}, s{0, 0};
+            for (int k = 0; k < 4; ++k) {
+                m = dd_sum(m, DD{red[k * 6 + 0], red[k * 6 + 1]});
+                v = dd_sum(v, DD{red[k * 6 + 2], red[k * 6 + 3]});
+                s = dd_sum(s, DD{red[k * 6 + 4], red[k * 6 + 5]});
+            }
+            double* o = a.strad_part + 6 * (size_t)blockIdx.x;
+            o[0] = m.hi; o[1] = m.lo; o[2] = v.hi; o[3] = v.lo; o[4] = s.hi; o[5] = s.lo;
+        }
+    }
+}
+
+// Adds the batch's partial sums (fused blocks, then straddler blocks, in a
+// fixed order) into the iteration accumulators.
+__global__ __launch_bounds__(256) void batch_reduce_kernel(const double* blk_part, int nblk,
+                                                           const double* strad_part, int nstrad,
+                                                           Scalars* sc) {
+    __shared__ double red[256 * 6];
+    if (sc->status || sc->overflow) return;
+    DD m{0, 0}, v{0, 0}, s{0, 0};
+    const int tot = nblk + nstrad;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        const double* p = i < nblk ? blk_part + 6 * (size_t)i : strad_part + 6 * (size_t)(i - nblk);
+        m = dd_sum(m, DD{p[0], p[1]});
+        v = dd_sum(v, DD{p[2], p[3]});
+        s = dd_sum(s, DD{p[4], p[5]});
+    }
+    red[threadIdx.x * 6 + 0] = m.hi; red[threadIdx.x * 6 + 1] = m.lo;
+    red[threadIdx.x * 6 + 2] = v.hi; red[threadIdx.x * 6 + 3] = v.lo;
+    red[threadIdx.x * 6 + 4] = s.hi; red[threadIdx.x * 6 + 5] = s.lo;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD M{sc->acc[0], sc->acc[1]}, V{sc->acc[2], sc->acc[3]}, Sg{sc->acc[4], sc->acc[5]};
+        for (int k = 0; k < (int)blockDim.x; ++k) {
+            M = dd_sum(M, DD{red[k * 6 + 0], red[k * 6 + 1]});
+            V = dd_sum(V, DD{red[k * 6 + 2], red[k * 6 + 3]});
+            Sg = dd_sum(Sg, DD{red[k * 6 + 4], red[k * 6 + 5]});
+        }
+        sc->acc[0] = M.hi; sc->acc[1] = M.lo;
+        sc->acc[2] = V.hi; sc->acc[3] = V.lo;
+        sc->acc[4] = Sg.hi; sc->acc[5] = Sg.lo;
+    }
+}
+
+// Iteration close: estimate, sum_sigf bookkeeping, running average (FP64).
+__global__ void finalize_kernel(IterGeom g, Scalars* sc, vgx_state st, int has_sigf) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (sc->overflow) {
+        if (st.status) *st.status = VGX_ERANGE;
+        sc->status = VGX_ERANGE;
+        return;
+    }
+    if (sc->status) {
+        if (st.status) *st.status = sc->status;
+        return;
+    }
+    const int it = (int)g.itn;
+    const double mean = sc->acc[0] + sc->acc[1];
+    const double var = sc->acc[2] + sc->acc[3];
+    const double ssig = sc->acc[4] + sc->acc[5];
+    st.itn_mean[it] = mean;
+    st.itn_var[it] = var;
+    st.itn_neval[it] = (int64_t)sc->total;
+    st.itn_nhrange[2 * it] = sc->nh_min;
+    st.itn_nhrange[2 * it + 1] = sc->nh_max;
+    // sum_sigf (pyx:2209-2215)
+    if (g.beta > 0 && g.adapt && !g.train_errors) {
+        if (ssig > 0) {
+            *st.sum_sigf = ssig;
+        } else {
+            sc->reset_sigf = 1;
+            *st.sum_sigf = has_sigf ? (double)g.nhcube : 0.0;
+        }
+    }
+    // running average over iterations 0..it (RAvg.add, pyx:2392-2410)
+    const int n = it + 1;
+    double rm, rs, chi2 = 0.0;
+    if (g.weighted) {
+        double sw = 0.0, swm = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const double s = sqrt(st.itn_var[k]);
+            const double vk = s * s;
+            const double w = 1. / (vk > kTiny ? vk : kTiny);
+            sw += w;
+            swm += w * st.itn_mean[k];
+        }
+        const double v = 1. / sw;
+        rm = swm * v;
+        rs = sqrt(v);
+        if (n > 1)
+            for (int k = 0; k < n; ++k) {
+                const double s = sqrt(st.itn_var[k]);
+                const double vk = s * s;
+                const double w = 1. / (vk > kTiny ? vk : kTiny);
+                chi2 += (rm - st.itn_mean[k]) * (rm - st.itn_mean[k]) * w;
+            }
+    } else {
+        double ms = 0.0, vs = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const double s = sqrt(st.itn_var[k]);
+            ms += st.itn_mean[k];
+            vs += s * s;
+        }
+        rm = ms / n;
+        rs = sqrt(vs / ((double)n * (double)n));
+        if (n > 1) {
+            for (int k = 0; k < n; ++k) chi2 += (st.itn_mean[k] - rm) * (st.itn_mean[k] - rm);
+            chi2 /= (vs / n);
+        }
+    }
+    st.ravg[0] = rm;
+    st.ravg[1] = rs;
+    st.ravg[2] = chi2;
+    st.ravg[3] = (double)(n - 1);
+}
+
+// sigf_h = |var_h|^(beta/2) and its sum (pyx:2183-2186): one streaming pass
+// after all hcubes of the iteration are done.
+__global__ __launch_bounds__(256) void sigf_pow_kernel(IterGeom g, double* sigf, double* part, Scalars* sc) {
+    __shared__ double red[8];
+    if (sc->status || sc->overflow) return;
+    DD s{0, 0};
+    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < g.nhcube;
+         h += (int64_t)gridDim.x * blockDim.x) {
+        const double v = pow(sigf[h], g.beta_half);
+        sigf[h] = v;
+        dd_add(s, v);
+    }
+    s = wave_sum_dd(s);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * w] = s.hi;
+        red[2 * w + 1] = s.lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD t{0, 0};
+        for (int k = 0; k < 4; ++k) t = dd_sum(t, DD{red[2 * k], red[2 * k + 1]});
+        part[2 * blockIdx.x] = t.hi;
+        part[2 * blockIdx.x + 1] = t.lo;
+    }
+}
+
+__global__ void sigf_sum_kernel(const double* part, int n, Scalars* sc) {
+    if (threadIdx.x != 0 || sc->status || sc->overflow) return;
+    DD t{0, 0};
+    for (int k = 0; k < n; ++k) t = dd_sum(t, DD{part[2 * k], part[2 * k + 1]});
+    sc->acc[4] = t.hi;
+    sc->acc[5] = t.lo;
+}
+
+// adapt_to_errors: training entry |var_h| at the last point of every hcube
+// (pyx:2187-2193).
+template <bool R32>
+__global__ __launch_bounds__(256) void errors_train_kernel(IterGeom g, const AxisC* axes, const double* grid,
+                                                           const uint32_t* off, const double* errv,
+                                                           double* sum_f, double* n_f, const Scalars* sc) {
+    if (sc->status || sc->overflow) return;
+    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < g.nhcube;
+         h += (int64_t)gridDim.x * blockDim.x) {
+        double x[VGX_MAXDIM];
+        int kb[VGX_MAXDIM];
+        double J;
+        eval_point<0, R32>(g, axes, grid, (uint32_t)h, off[h + 1] - 1, x, J, kb);
+        for (int d = 0; d < g.dim; ++d)
+            if (kb[d] >= 0) {
+                unsafeAtomicAdd(&sum_f[d * g.nb + kb[d]], errv[h]);
+                unsafeAtomicAdd(&n_f[d * g.nb + kb[d]], 1.0);
+            }
+    }
+}
+
+__global__ void sigf_reset_kernel(double* sigf, int64_t n, const Scalars* sc) {
+    if (!sc->reset_sigf || sc->status) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        sigf[i] = 1.0;
+}
+
+__global__ void grid_commit_kernel(double* grid, const double* tmp, int n, const Scalars* sc) {
+    if (sc->status) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) grid[i] = tmp[i];
+}
+
+__global__ void axes_init_kernel(IterGeom g, AxisC* axes) {
+    if (threadIdx.x < (unsigned)g.dim) axes[threadIdx.x] = make_axis(g, threadIdx.x);
+}
+
+__global__ void to_float_kernel(const double* a, float* b, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+
+// Staged path: x, y, weight and hcube of points [p0, p1) (pyx:1731-1759).
+__global__ __launch_bounds__(256) void sample_kernel(IterGeom g, const AxisC* axes, const double* grid, const uint32_t* off,
+                                                     uint32_t h0, uint32_t h1, uint32_t p0, uint32_t p1,
+                                                     double* x, double* y, double* wgt, int64_t* hcube,
+                                                     const Scalars* sc) {
+    if (sc->status) return;
+    const int dim = g.dim;
+    for (uint32_t p = p0 + blockIdx.x * blockDim.x + threadIdx.x; p < p1; p += gridDim.x * blockDim.x) {
+        uint32_t lo = h0, hi = h1 - 1;  // largest h with off[h] <= p
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        const uint32_t h = lo;
+        const uint32_t n = off[h + 1] - off[h];
+        double xx[VGX_MAXDIM];
+        int kb[VGX_MAXDIM];
+        double J;
+        if (g.rng32) eval_point<0, true>(g, axes, grid, h, p, xx, J, kb);
+        else eval_point<0, false>(g, axes, grid, h, p, xx, J, kb);
+        const int64_t i = (int64_t)(p - p0);
+        for (int d = 0; d < dim; ++d) x[i * dim + d] = xx[d];
+        if (y) {
+            // y is regenerated exactly as eval_point forms it
+            uint32_t t = h;
+            U4 w{0, 0, 0, 0};
+            for (int d = 0; d < dim; ++d) {
+                const uint32_t q = divq(t, g.dm[d]);
+                const uint32_t c = t - q * (uint32_t)g.nstrat[d];
+                t = q;
+                double u;
+                if (g.rng32) {
+                    if ((d & 3) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
+                    const int e = d & 3;
+                    u = u24(e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w);
+                } else {
+                    if ((d & 1) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
+                    u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);
+                }
+                y[i * dim + d] = g.exact_div ? ((double)c + u) / (double)g.nstrat[d]
+                                             : ((double)c + u) * g.inv_nstrat[d];
+            }
+        }
+        if (wgt) wgt[i] = J * (g.dv_y / (double)n);
+        if (hcube) hcube[i] = (int64_t)h;
+    }
+}
+
+// ------------------------------------------------------------ host side ----
+static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) {
+    memset(g, 0, sizeof(*g));
+    if (c->dim < 1 || c->dim > VGX_MAXDIM || c->nitn < 0) return VGX_EINVAL;
+    if (c->nhcube < 1 || c->nhcube >= 0x7FFFFFFFll) return VGX_ERANGE;
+    g->dim = c->dim;
+    g->ld = c->ld;
+    g->nb = c->ld - 1;
+    int64_t nh = 1;
+    for (int d = 0; d < c->dim; ++d) {
+        if (c->ninc[d] < 1 || c->ninc[d] + 1 > c->ld || c->nstrat[d] < 1) return VGX_EINVAL;
+        g->ninc[d] = c->ninc[d];
+        g->nstrat[d] = c->nstrat[d];
+        g->inv_nstrat[d] = 1.0 / (double)c->nstrat[d];
+        g->dm[d] = make_divmagic((uint32_t)c->nstrat[d]);
+        nh *= c->nstrat[d];
+    }
+    if (nh != c->nhcube) return VGX_EINVAL;
+    g->nhcube = c->nhcube;
+    g->neval = (double)c->neval;
+    g->neval_frac = c->neval_frac;
+    g->beta = c->beta;
+    g->beta_half = c->beta / 2.;
+    g->dv_y = 1. / (double)c->nhcube;
+    g->adaptive_sampler = c->beta > 0 && c->nhcube > 1 && !c->adapt_to_errors;
+    g->adaptive_strat = g->adaptive_sampler && c->adapt;
+    g->train_points = !c->adapt_to_errors && c->adapt && c->alpha > 0;
+    g->train_errors = c->adapt_to_errors && c->adapt;
+    g->rng32 = c->precision == VGX_PREC_F32;
+    g->exact_div = (c->flags & VGX_FLAG_EXACT_DIV) ? 1 : 0;
+    g->weighted = c->weighted;
+    g->adapt = c->adapt;
+    const int64_t nmin = c->min_neval_hcube;
+    const int64_t nmax = c->max_neval_hcube > nmin ? c->max_neval_hcube : nmin;
+    const int64_t avg = (int64_t)((double)c->neval / (double)c->nhcube);
+    if (nmin < 2 || nmax > 0x7FFFFFFF || (!g->adaptive_sampler && avg < 2)) return VGX_ERANGE;
+    g->nmin = (int32_t)nmin;
+    g->nmax = (int32_t)nmax;
+    g->avg_nh = (int32_t)(avg > 0x7FFFFFFF ? 0x7FFFFFFF : avg);
+    g->seed_lo = (uint32_t)(c->seed & 0xffffffffu);
+    g->seed_hi = (uint32_t)(c->seed >> 32);
+    g->nitn = (uint32_t)c->nitn;
+    // LDS plan of the fused kernel: axis table + FP64 grid always, the
+    // privatised histogram when it also fits.
+    const size_t tot = (size_t)c->dim * g->nb;
+    const size_t fixed = (size_t)VGX_MAXDIM * 24 + (size_t)16 * kNoff * sizeof(uint32_t) +
+                         16 * 6 * sizeof(double) + 64;
+    const size_t gbytes = (size_t)c->dim * c->ld * sizeof(double);
+    const size_t hbytes = tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t) + 16;
+    g->grid_lds = 1;
+    if (fixed + gbytes > info.max_lds) return VGX_ERANGE;
+    g->hist_lds = g->train_points && (fixed + gbytes + hbytes <= info.max_lds);
+    return VGX_OK;
+}
+
+static size_t fused_lds(const IterGeom& g) {
+    const size_t tot = (size_t)g.dim * g.nb;
+    size_t b = (size_t)VGX_MAXDIM * sizeof(AxisC);
+    b += (size_t)g.dim * g.ld * sizeof(double);
+    if (g.hist_lds) b += tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t);
+    b = (b + 15) & ~(size_t)15;
+    b += (size_t)16 * kNoff * sizeof(uint32_t);
+    b = (b + 15) & ~(size_t)15;
+    b += 16 * 6 * sizeof(double);
+    return b;
+}
+
+static uint64_t max_points(const vgx_config* c, const IterGeom& g) {
+    if (!g.adaptive_sampler) return (uint64_t)c->nhcube * (uint64_t)g.avg_nh;
+    const double adapt = c->neval_frac * (double)c->neval;
+    return (uint64_t)(adapt * (1.0 + 1e-9)) + 2 + (uint64_t)c->nhcube * (uint64_t)g.nmin;
+}
+
+struct Run {
+    IterGeom g;
+    FusedPtrs io;
+    StradArgs sa;
+    int fused_blocks;
+    size_t lds;
+    int nblk_alloc;
+    uint64_t cap;
+    uint32_t maxtiles;
+    Scalars* sc;
+    double* grid_tmp;
+    uint64_t* blocksum;
+    int32_t* nh;
+    float* fparams;
+    AxisC* axes;
+    double* errv;
+    double* sigf_part;
+};
+
+static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
+    int rc = build_geom(c, ctx->info, &R->g);
+    if (rc) return rc;
+    const IterGeom& g = R->g;
+    R->cap = max_points(c, g);
+    if (R->cap >= 0xFFFFFFF0ull) return VGX_ERANGE;
+    R->maxtiles = (uint32_t)((R->cap + kT - 1) / kT) + 1;
+    R->nblk_alloc = (int)((c->nhcube + kAllocPerBlock - 1) / kAllocPerBlock);
+    R->fused_blocks = ctx->info.cus;
+    R->lds = fused_lds(g);
+    if (R->lds > ctx->info.max_lds) return VGX_ERANGE;
+    if ((rc = ctx->nh.ensure((size_t)c->nhcube * 4))) return rc;
+    if ((rc = ctx->off.ensure(((size_t)c->nhcube + 1) * 4))) return rc;
+    if ((rc = ctx->blocksum.ensure((size_t)R->nblk_alloc * 8))) return rc;
+    if ((rc = ctx->tile_h.ensure((size_t)R->maxtiles * 4))) return rc;
+    if ((rc = ctx->tile_slots.ensure((size_t)R->maxtiles * 2 * sizeof(Slot)))) return rc;
+    if ((rc = ctx->blk_part.ensure((size_t)R->fused_blocks * 6 * 8))) return rc;
+    if ((rc = ctx->strad_part.ensure((size_t)kStradBlocks * 6 * 8))) return rc;
+    if ((rc = ctx->wf_stage.ensure((size_t)R->cap * 8 + 64))) return rc;
+    R->errv = nullptr;
+    if (R->g.train_errors) {
+        if ((rc = ctx->errv.ensure((size_t)c->nhcube * 8))) return rc;
+        R->errv = (double*)ctx->errv.ptr;
+    }
+    const size_t misc = 1024 + (size_t)c->dim * c->ld * 8 + VGX_MAXDIM * 32 + 4096 * 2;
+    if ((rc = ctx->misc.ensure(misc + 1024 * 1024))) return rc;
+    R->sc = (Scalars*)ctx->misc.ptr;
+    R->grid_tmp = (double*)((char*)ctx->misc.ptr + 1024);
+    R->axes = (AxisC*)((char*)R->grid_tmp + (size_t)c->dim * c->ld * 8);
+    R->sigf_part = (double*)((char*)R->axes + VGX_MAXDIM * sizeof(AxisC));
+    R->fparams = (float*)((char*)R->sigf_part + 2 * 256 * sizeof(double));
+    R->blocksum = (uint64_t*)ctx->blocksum.ptr;
+    R->nh = (int32_t*)ctx->nh.ptr;
+    FusedPtrs& io = R->io;
+    io.axes = R->axes;
+    io.grid = st->grid;
+    io.off = (const uint32_t*)ctx->off.ptr;
+    io.tile_h = (const int32_t*)ctx->tile_h.ptr;
+    io.sigf = st->sigf;
+    io.slots = (Slot*)ctx->tile_slots.ptr;
+    io.blk_part = (double*)ctx->blk_part.ptr;
+    io.wf_stage = (double*)ctx->wf_stage.ptr;
+    io.sum_f = st->sum_f;
+    io.n_f = st->n_f;
+    io.errv = R->errv;
+    io.sc = R->sc;
+    StradArgs& sa = R->sa;
+    sa.axes = R->axes;
+    sa.errv = R->errv;
+    sa.off = io.off;
+    sa.slots = io.slots;
+    sa.wf_stage = io.wf_stage;
+    sa.sigf = st->sigf;
+    sa.strad_part = (double*)ctx->strad_part.ptr;
+    sa.sc = R->sc;
+    sa.sum_f = st->sum_f;
+    sa.n_f = st->n_f;
+    sa.grid = st->grid;
+    return VGX_OK;
+}
+
+static void set_iter(Run* R, const vgx_config* c, int it) {
+    R->g.iteration = c->iteration + (uint32_t)it;
+    R->g.itn = (uint32_t)it;
+}
+
+static int init_axes(Run* R, hipStream_t s) {
+    hipLaunchKernelGGL(axes_init_kernel, dim3(1), dim3(64), 0, s, R->g, R->axes);
+    return check_launch();
+}
+
+static int iter_begin_kernels(vgx_ctx* ctx, Run* R, vgx_state* st, hipStream_t s) {
+    const IterGeom& g = R->g;
+    const int tot = g.dim * g.nb;
+    hipLaunchKernelGGL(iter_init_kernel, dim3((tot + 255) / 256 > 0 ? (tot + 255) / 256 : 1), dim3(256), 0,
+                       s, g, st->sum_f, st->n_f, R->sc, 1);
+    hipLaunchKernelGGL(alloc_kernel, dim3(R->nblk_alloc), dim3(kAllocThreads), 0, s, g,
+                       (const double*)st->sigf, (const double*)st->sum_sigf, R->nh, R->blocksum, R->sc);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, g, R->blocksum, R->nblk_alloc, R->sc, R->cap);
+    hipLaunchKernelGGL(offsets_kernel, dim3(R->nblk_alloc), dim3(kAllocThreads), 0, s, g,
+                       (const int32_t*)R->nh, (const uint64_t*)R->blocksum, (uint32_t*)ctx->off.ptr,
+                       (int32_t*)ctx->tile_h.ptr, R->sc);
+    return check_launch();
+}
+
+template <class Fcn>
+static int accumulate_kernels(vgx_ctx* ctx, Run* R, const Fcn& fcn, hipStream_t s, int timed) {
+    const IterGeom& g = R->g;
+    if (timed) (void)hipEventRecord(ctx->ev0, s);
+    int rc = launch_fused(g, R->io, fcn, R->fused_blocks, R->lds, s);
+    if (rc) return rc;
+    if (timed) {
+        (void)hipEventRecord(ctx->ev1, s);
+        ctx->have_timing = 1;
+    }
+    hipLaunchKernelGGL(strad_kernel<0>, dim3(kStradBlocks), dim3(256), 0, s, g, R->sa);
+    hipLaunchKernelGGL(strad_kernel<1>, dim3(kStradBlocks), dim3(256), 0, s, g, R->sa);
+    hipLaunchKernelGGL(strad_kernel<2>, dim3(kStradBlocks), dim3(256), 0, s, g, R->sa);
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(256), 0, s, (const double*)R->io.blk_part,
+                       R->fused_blocks, (const double*)R->sa.strad_part, kStradBlocks, R->sc);
+    return check_launch();
+}
+
+static int iter_end_kernels(vgx_ctx* ctx, Run* R, const vgx_config* c, vgx_state* st, hipStream_t s) {
+    const IterGeom& g = R->g;
+    const int has_sigf = c->beta >= 0 && c->nhcube > 1 && !c->adapt_to_errors;
+    if (g.adaptive_strat) {
+        hipLaunchKernelGGL(sigf_pow_kernel, dim3(256), dim3(256), 0, s, g, st->sigf, R->sigf_part, R->sc);
+        hipLaunchKernelGGL(sigf_sum_kernel, dim3(1), dim3(64), 0, s, (const double*)R->sigf_part, 256, R->sc);
+    }
+    if (g.train_errors) {
+        const unsigned nb = (unsigned)((c->nhcube + 255) / 256 < 4096 ? (c->nhcube + 255) / 256 : 4096);
+        if (g.rng32)
+            hipLaunchKernelGGL(errors_train_kernel<true>, dim3(nb), dim3(256), 0, s, g, (const AxisC*)R->axes,
+                               (const double*)st->grid, (const uint32_t*)ctx->off.ptr, (const double*)R->errv,
+                               st->sum_f, st->n_f, (const Scalars*)R->sc);
+        else
+            hipLaunchKernelGGL(errors_train_kernel<false>, dim3(nb), dim3(256), 0, s, g, (const AxisC*)R->axes,
+                               (const double*)st->grid, (const uint32_t*)ctx->off.ptr, (const double*)R->errv,
+                               st->sum_f, st->n_f, (const Scalars*)R->sc);
+    }
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, s, g, R->sc, *st, has_sigf);
+    hipLaunchKernelGGL(sigf_reset_kernel, dim3(256), dim3(256), 0, s, st->sigf, (int64_t)c->nhcube,
+                       (const Scalars*)R->sc);
+    int rc = check_launch();
+    if (rc) return rc;
+    if (c->alpha > 0 && c->adapt) {
+        const int has_training = g.train_points || g.train_errors;
+        rc = vgx_adapt(ctx, c->dim, c->ninc, st->grid, c->ld, has_training, st->sum_f, st->n_f, c->alpha,
+                       c->ninc, R->grid_tmp, c->ld, (c->flags & VGX_FLAG_EXACT_ADAPT) ? 1 : 0, s);
+        if (rc) return rc;
+        const int n = c->dim * c->ld;
+        hipLaunchKernelGGL(grid_commit_kernel, dim3((n + 255) / 256), dim3(256), 0, s, st->grid,
+                           (const double*)R->grid_tmp, n, (const Scalars*)R->sc);
+        rc = check_launch();
+    }
+    return rc;
+}
+
+static int make_fcn_and_run(vgx_ctx* ctx, Run* R, const vgx_config* c, const vgx_integrand* f,
+                            hipStream_t s, int timed) {
+    const bool f32 = c->precision == VGX_PREC_F32;
+    if (f->kind == VGX_FCN_BUFFER) {
+        if (f32) {
+            BufferFcn<true> b{f->fbuf, f->fbase};
+            return accumulate_kernels(ctx, R, b, s, timed);
+        }
+        BufferFcn<false> b{f->fbuf, f->fbase};
+        return accumulate_kernels(ctx, R, b, s, timed);
+    }
+    if (f->kind == VGX_FCN_GAUSSMIX || f->kind == VGX_FCN_DIAGRIDGE) {
+        const int nc = f->ncomp;
+        if (nc < 1 || !f->A || !f->B || !f->c) return VGX_EINVAL;
+        const int ncc = f->kind == VGX_FCN_GAUSSMIX ? nc * c->dim : nc;
+        if (f32) {
+            if ((size_t)(2 * nc + ncc) * 4 > 1024 * 1024) return VGX_ERANGE;
+            float* A = R->fparams;
+            float* B = A + nc;
+            float* C = B + nc;
+            hipLaunchKernelGGL(to_float_kernel, dim3(64), dim3(256), 0, s, f->A, A, nc);
+            hipLaunchKernelGGL(to_float_kernel, dim3(64), dim3(256), 0, s, f->B, B, nc);
+            hipLaunchKernelGGL(to_float_kernel, dim3(64), dim3(256), 0, s, f->c, C, ncc);
+            if (f->kind == VGX_FCN_GAUSSMIX) {
+                GaussMix32 fn{nc, A, B, C};
+                return accumulate_kernels(ctx, R, fn, s, timed);
+            }
+            DiagRidge32 fn{nc, A, B, C};
+            return accumulate_kernels(ctx, R, fn, s, timed);
+        }
+        if (f->kind == VGX_FCN_GAUSSMIX) {
+            GaussMix64 fn{nc, f->A, f->B, f->c};
+            return accumulate_kernels(ctx, R, fn, s, timed);
+        }
+        DiagRidge64 fn{nc, f->A, f->B, f->c};
+        return accumulate_kernels(ctx, R, fn, s, timed);
+    }
+    return VGX_EINVAL;
+}
+
+}  // namespace vgx
+
+using namespace vgx;
+
+extern "C" int vgx_integrate(vgx_ctx* ctx, const vgx_config* c, const vgx_integrand* f, vgx_state* st,
+                             void* stream) {
+    if (!ctx || !c || !f || !st) return VGX_EINVAL;
+    if (f->kind == VGX_FCN_BUFFER) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    Run R;
+    int rc = prepare(ctx, c, st, &R);
+    if (rc) return rc;
+    if ((rc = init_axes(&R, s))) return rc;
+    for (int it = 0; it < c->nitn; ++it) {
+        set_iter(&R, c, it);
+        if ((rc = iter_begin_kernels(ctx, &R, st, s))) return rc;
+        if ((rc = make_fcn_and_run(ctx, &R, c, f, s, it == c->nitn - 1))) return rc;
+        if ((rc = iter_end_kernels(ctx, &R, c, st, s))) return rc;
+    }
+    return VGX_OK;
+}
+
+extern "C" int vgx_iter_begin(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, int it, int32_t* n_h,
+                              int64_t* total_host, void* stream) {
+    if (!ctx || !c || !st) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    Run R;
+    int rc = prepare(ctx, c, st, &R);
+    if (rc) return rc;
+    if ((rc = init_axes(&R, s))) return rc;
+    set_iter(&R, c, it);
+    if ((rc = iter_begin_kernels(ctx, &R, st, s))) return rc;
+    if (n_h) VGX_HIP(hipMemcpyAsync(n_h, R.nh, (size_t)c->nhcube * 4, hipMemcpyDeviceToDevice, s));
+    if (total_host) {
+        Scalars tmp;
+        VGX_HIP(hipMemcpyAsync(&tmp, R.sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
+        VGX_HIP(hipStreamSynchronize(s));
+        if (tmp.overflow) return VGX_ERANGE;
+        *total_host = (int64_t)tmp.total;
+    }
+    return VGX_OK;
+}
+
+extern "C" int vgx_iter_sample(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, int it, int64_t h0,
+                               int64_t h1, int64_t p0, int64_t p1, double* x, double* y, double* wgt,
+                               int64_t* hcube, void* stream) {
+    if (!ctx || !c || !st || !x) return VGX_EINVAL;
+    if (h0 < 0 || h1 > c->nhcube || h0 >= h1 || p0 < 0 || p1 <= p0) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    Run R;
+    int rc = prepare(ctx, c, st, &R);
+    if (rc) return rc;
+    if ((rc = init_axes(&R, s))) return rc;
+    set_iter(&R, c, it);
+    const int64_t n = p1 - p0;
+    int64_t b = (n + 255) / 256;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(sample_kernel, dim3((unsigned)b), dim3(256), 0, s, R.g, (const AxisC*)R.axes, (const double*)st->grid,
+                       (const uint32_t*)ctx->off.ptr, (uint32_t)h0, (uint32_t)h1, (uint32_t)p0, (uint32_t)p1,
+                       x, y, wgt, hcube, (const Scalars*)R.sc);
+    return check_launch();
+}
+
+extern "C" int vgx_iter_accumulate(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, int it, int64_t h0,
+                                   int64_t h1, int64_t p0, int64_t p1, const double* fv, void* stream) {
+    if (!ctx || !c || !st || !fv) return VGX_EINVAL;
+    if (h0 < 0 || h1 > c->nhcube || h0 >= h1 || p0 < 0 || p1 <= p0) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    Run R;
+    int rc = prepare(ctx, c, st, &R);
+    if (rc) return rc;
+    if ((rc = init_axes(&R, s))) return rc;
+    set_iter(&R, c, it);
+    const uint32_t nhb = (uint32_t)(h1 - h0);
+    hipLaunchKernelGGL(batch_tiles_kernel, dim3((nhb + 255) / 256 < 1024 ? (nhb + 255) / 256 : 1024),
+                       dim3(256), 0, s, (const uint32_t*)ctx->off.ptr, (int32_t*)ctx->tile_h.ptr, R.sc,
+                       (uint32_t)h0, (uint32_t)h1, (uint32_t)p0, (uint32_t)p1);
+    if ((rc = check_launch())) return rc;
+    vgx_integrand f{};
+    f.kind = VGX_FCN_BUFFER;
+    f.fbuf = fv;
+    f.fbase = p0;
+    return make_fcn_and_run(ctx, &R, c, &f, s, 0);
+}
+
+extern "C" int vgx_iter_end(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, int it, void* stream) {
+    if (!ctx || !c || !st) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    Run R;
+    int rc = prepare(ctx, c, st, &R);
+    if (rc) return rc;
+    if ((rc = init_axes(&R, s))) return rc;
+    set_iter(&R, c, it);
+    return iter_end_kernels(ctx, &R, c, st, s);
+}

@@ -1,0 +1,236 @@
+// vgx_iter.h -- shared definitions of the vegas+ iteration kernels.
+#pragma once
+#include "vgx_internal.h"
+
+namespace vgx {
+
+constexpr int kK = 4;               // points per lane in a wave tile
+constexpr int kT = 64 * kK;         // points per wave tile
+constexpr int kNoff = kT / 2 + 4;   // offsets slice: hcubes touching a tile (+1 end, +pad)
+constexpr int kFusedThreads = 1024; // 16 waves per workgroup, one workgroup per CU
+constexpr int kAllocThreads = 256;
+constexpr int kAllocItems = 8;
+constexpr int kAllocPerBlock = kAllocThreads * kAllocItems;
+constexpr int kCntFlushRounds = 15; // u16 LDS counts: 15 rounds x 16 tiles x 256 < 65536
+constexpr int kStradBlocks = 256;   // fixed grid of the straddler kernels
+constexpr int kKeySentinel = 0x7fff;
+
+// Kernel-argument copy of the iteration geometry (Integrator.set, pyx:1331-1446).
+struct IterGeom {
+    int dim, ld, nb;
+    int adaptive_sampler;  // beta > 0 && nhcube > 1 && !adapt_to_errors  (pyx:1675)
+    int adaptive_strat;    // ... && adapt                              (pyx:2052-2055)
+    int train_points;      // !adapt_to_errors && adapt && alpha > 0      (pyx:2196)
+    int train_errors;      // adapt_to_errors && adapt                    (pyx:2187)
+    int rng32;             // 24-bit uniforms (FP32 integrand) else 53-bit
+    int exact_div;         // y = (c+u)/s by IEEE division
+    int hist_lds, grid_lds;
+    int weighted;
+    int adapt;
+    int pad_a;
+    int32_t ninc[VGX_MAXDIM];
+    int32_t nstrat[VGX_MAXDIM];
+    double inv_nstrat[VGX_MAXDIM];
+    DivMagic dm[VGX_MAXDIM];
+    int64_t nhcube;
+    double neval;
+    double neval_frac;
+    double beta_half;
+    double beta;
+    double dv_y;
+    int32_t nmin, nmax, avg_nh, pad2;
+    uint32_t seed_lo, seed_hi;
+    uint32_t iteration;  // RNG stream of this iteration
+    uint32_t itn;        // index within the run
+    uint32_t nitn;
+    uint32_t pad3;
+};
+
+// Device-resident scalars of one run.
+struct Scalars {
+    uint32_t total;       // points this iteration (off[nhcube])
+    uint32_t overflow;
+    int32_t nh_min, nh_max;
+    // current batch (whole iteration for device integrands)
+    uint32_t b_p0, b_p1, b_h0, b_h1, b_ntiles, pad0;
+    uint64_t wfn_max_bits;  // max |w f n| seen (non-negative double bits)
+    double acc[6];          // iteration sums (DD): mean, var, sum_sigf
+    int32_t status;         // 0 or VGX_ENAN
+    int32_t reset_sigf;
+    double pad1[2];
+};
+
+// Per-tile record of an hcube cut by a tile boundary ("straddler").
+struct Slot {
+    int32_t h;      // hcube, or -1
+    int32_t pad;
+    double S;       // sum of w f over the tile's part of h
+    double Q;       // sum of guarded (w f - mean)^2 over that part
+    double D;       // sum of guarded (w f - mean) over that part
+    double Stot;    // owner's slot only: S over the whole hcube
+    double pad2;
+};
+
+struct AxisC;
+struct FusedPtrs {
+    const AxisC* axes;  // dim per-axis sampler constants (device)
+    const double* grid;
+    const uint32_t* off;
+    const int32_t* tile_h;
+    double* sigf;
+    Slot* slots;
+    double* blk_part;   // [gridDim.x][6]
+    double* wf_stage;   // [points] w f of straddler points
+    double* sum_f;
+    double* n_f;
+    double* errv;       // adapt_to_errors: |variance| per hcube
+    Scalars* sc;
+};
+
+// ---------------------------------------------------------------------------
+// Integrands (device functors).  eval() receives x in FP64 and returns f in
+// FP64; the arithmetic precision is the functor's.
+
+// f(x) = sum_k A_k exp(-B_k |x - c_k|^2)
+struct GaussMix64 {
+    static constexpr bool kRng32 = false;
+    int ncomp;
+    const double* A;
+    const double* B;
+    const double* c;
+    template <int DIM>
+    __device__ double eval(const double* x, int dim, uint32_t) const {
+        const int nd = DIM > 0 ? DIM : dim;
+        double acc = 0.0;
+        for (int k = 0; k < ncomp; ++k) {
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
+                if (DIM == 0 && d >= nd) break;
+                double t = x[d] - c[k * nd + d];
+                r2 += t * t;
+            }
+            acc += A[k] * exp(-B[k] * r2);
+        }
+        return acc;
+    }
+};
+
+struct GaussMix32 {
+    static constexpr bool kRng32 = true;
+    int ncomp;
+    const float* A;
+    const float* B;
+    const float* c;
+    template <int DIM>
+    __device__ double eval(const double* x, int dim, uint32_t) const {
+        const int nd = DIM > 0 ? DIM : dim;
+        float xf[DIM > 0 ? DIM : VGX_MAXDIM];
+#pragma unroll
+        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
+            if (DIM == 0 && d >= nd) break;
+            xf[d] = (float)x[d];
+        }
+        float acc = 0.0f;
+        for (int k = 0; k < ncomp; ++k) {
+            float r2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
+                if (DIM == 0 && d >= nd) break;
+                float t = xf[d] - c[k * nd + d];
+                r2 = __builtin_fmaf(t, t, r2);
+            }
+            acc = __builtin_fmaf(A[k], __expf(-B[k] * r2), acc);
+        }
+        return (double)acc;
+    }
+};
+
+// Mixture with centres t_k (1,...,1): |x - c_k|^2 = P + dim (xbar - t_k)^2,
+// P = sum_d (x_d - xbar)^2 (exact algebra, no cancellation).
+struct DiagRidge32 {
+    static constexpr bool kRng32 = true;
+    int ncomp;
+    const float* A;
+    const float* B;
+    const float* c;  // t_k
+    template <int DIM>
+    __device__ double eval(const double* x, int dim, uint32_t) const {
+        const int nd = DIM > 0 ? DIM : dim;
+        float xf[DIM > 0 ? DIM : VGX_MAXDIM];
+        float s = 0.0f;
+#pragma unroll
+        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
+            if (DIM == 0 && d >= nd) break;
+            xf[d] = (float)x[d];
+            s += xf[d];
+        }
+        const float xbar = s / (float)nd;
+        float P = 0.0f;
+#pragma unroll
+        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
+            if (DIM == 0 && d >= nd) break;
+            float t = xf[d] - xbar;
+            P = __builtin_fmaf(t, t, P);
+        }
+        const float fd = (float)nd;
+        float acc = 0.0f;
+        for (int k = 0; k < ncomp; ++k) {
+            float t = xbar - c[k];
+            float r2 = __builtin_fmaf(fd * t, t, P);
+            acc = __builtin_fmaf(A[k], __expf(-B[k] * r2), acc);
+        }
+        return (double)acc;
+    }
+};
+
+struct DiagRidge64 {
+    static constexpr bool kRng32 = false;
+    int ncomp;
+    const double* A;
+    const double* B;
+    const double* c;
+    template <int DIM>
+    __device__ double eval(const double* x, int dim, uint32_t) const {
+        const int nd = DIM > 0 ? DIM : dim;
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
+            if (DIM == 0 && d >= nd) break;
+            s += x[d];
+        }
+        const double xbar = s / (double)nd;
+        double P = 0.0;
+#pragma unroll
+        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
+            if (DIM == 0 && d >= nd) break;
+            double t = x[d] - xbar;
+            P += t * t;
+        }
+        double acc = 0.0;
+        for (int k = 0; k < ncomp; ++k) {
+            double t = xbar - c[k];
+            acc += A[k] * exp(-B[k] * (P + (double)nd * t * t));
+        }
+        return acc;
+    }
+};
+
+// f supplied by the caller for global point p (host-driven integrands).
+template <bool R32>
+struct BufferFcn {
+    static constexpr bool kRng32 = R32;
+    const double* f;
+    int64_t base;
+    template <int DIM>
+    __device__ double eval(const double*, int, uint32_t p) const {
+        return f[(int64_t)p - base];
+    }
+};
+
+// Launchers, one per (integrand, dim) instantiation (vgx_fused_*.hip).
+template <class Fcn>
+int launch_fused(const IterGeom& g, const FusedPtrs& io, const Fcn& fcn, int grid_blocks,
+                 size_t lds, hipStream_t st);
+
+}  // namespace vgx

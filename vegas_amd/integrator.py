@@ -1,0 +1,520 @@
+"""Integrator -- the reference's ``vegas.Integrator`` (src/vegas/_vegas.pyx:
+834-2230) with every iteration executed on the GPU.
+
+Two execution paths behind the same ``__call__``:
+
+* device integrands (:mod:`vegas_amd.integrands`): one ``vgx_integrate`` call
+  enqueues all ``nitn`` iterations -- allocation, sampling, map, integrand,
+  per-hcube sums, training histogram, sigf update, adapt and the running
+  average -- with no host synchronisation until the results are read;
+* Python batch integrands (the reference's ``@lbatchintegrand`` /
+  ``@rbatchintegrand`` convention, numpy or torch): the iteration is staged
+  in batches cut at hcube boundaries exactly as ``_random_batch`` cuts them
+  (pyx:1692-1710); the device samples each batch, the integrand is called on
+  it, and the device accumulates it.
+
+The random numbers come from Philox4x32-10 keyed by ``seed``, with counter
+(point index within the iteration, iteration, axis block): the sample does
+not depend on batching or launch configuration.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import pickle
+import warnings
+
+import numpy as np
+
+from . import _geometry, _lib
+from ._ravg import RAvg, GVar
+from .adaptive_map import AdaptiveMap
+from .integrands import DeviceIntegrand
+
+HUGE = 10 ** (308 - 50)
+
+
+def _region_map(region):
+    """AdaptiveMap for an integration region given as limits or node lists
+    (pyx:1209-1253; dictionary regions are not supported)."""
+    if isinstance(region, AdaptiveMap):
+        return AdaptiveMap(region)
+    if isinstance(region, Integrator):
+        return AdaptiveMap(region.map)
+    if hasattr(region, 'keys'):
+        raise NotImplementedError('dictionary integration regions are not supported')
+    arr = np.array(region, dtype=object)
+    if np.shape(arr.flat[0]) == ():
+        grid = np.array(region, float).reshape(-1, 2).tolist()
+    else:
+        grid = [list(np.asarray(g, float).reshape(-1)) for g in arr.reshape(-1)]
+    return AdaptiveMap(grid)
+
+
+class Integrator:
+    """Adaptive multidimensional Monte Carlo integration (vegas+)."""
+
+    # pyx:1115-1140, plus device options (seed, precision, exact)
+    defaults = dict(
+        map=None, neval=1000, maxinc_axis=1000, min_neval_batch=100000, max_neval_hcube=50000,
+        gpu_pad=False, neval_frac=0.75, max_mem=1e9, nitn=10, alpha=0.5, beta=0.75, adapt=True,
+        correlate_integrals=True, minimize_mem=False, adapt_to_errors=False,
+        uniform_nstrat=False, rtol=0, atol=0, analyzer=None, ran_array_generator=None,
+        sync_ran=True, mpi=False, uses_jac=False, nproc=1,
+        seed=None, precision=None, exact=False,
+    )
+
+    def __init__(self, map, **kargs):
+        self.neval_hcube_range = None
+        self.last_neval = 0
+        self._rng_iter = 0
+        self._sigf = None
+        self.sum_sigf = HUGE
+        if isinstance(map, Integrator):
+            self.map = AdaptiveMap(map.map)
+            args = {k: getattr(map, k) for k in Integrator.defaults if k != 'map'}
+            self.nstrat = np.array(map.nstrat)
+            self._sigf = None if map._sigf is None else map._sigf.clone()
+            self.sum_sigf = map.sum_sigf
+            self._rng_iter = map._rng_iter
+            if 'nstrat' not in kargs:
+                kargs['nstrat'] = map.nstrat
+                if 'neval' not in kargs:
+                    kargs['neval'] = map.neval
+        else:
+            args = dict(Integrator.defaults)
+            del args['map']
+            self.map = _region_map(map)
+            self.nstrat = np.full(self.map.dim, 0, dtype=np.intp)
+        for k, v in args.items():
+            setattr(self, k, v)
+        args.update(kargs)
+        if 'nstrat' in kargs and 'neval' not in kargs and 'neval' in args:
+            del args['neval']
+        if 'neval' in kargs and 'nstrat' not in kargs and 'nstrat' in args:
+            del args['nstrat']
+        self._old_neval_given = 'neval' in args
+        self.set(args)
+        if self.seed is None:
+            self.seed = int(np.random.SeedSequence().generate_state(1, np.uint64)[0])
+
+    # ------------------------------------------------------------ settings
+    @property
+    def sigf(self):
+        return np.array([]) if self._sigf is None else self._sigf.cpu().numpy()
+
+    @property
+    def dim(self):
+        return self.map.dim
+
+    def __reduce__(self):
+        odict = {k: getattr(self, k) for k in Integrator.defaults if k not in ('map',)}
+        odict['nstrat'] = np.asarray(self.nstrat)
+        odict['sigf'] = self.sigf
+        odict['_rng_iter'] = self._rng_iter
+        return (Integrator, (self.map,), odict)
+
+    def __setstate__(self, odict):
+        rng_iter = odict.pop('_rng_iter', 0)
+        self.set(odict)
+        self._rng_iter = rng_iter
+
+    def set(self, ka={}, **kargs):
+        """Reset parameters; returns the old values (pyx:1256-1446)."""
+        torch = _lib.torch_cuda()
+        if kargs:
+            kargs.update(ka)
+        else:
+            kargs = dict(ka)
+        old_val = {}
+        nstrat = None
+        for k in kargs:
+            if k == 'map':
+                old_val['map'] = self.map
+                self.map = _region_map(kargs['map'])
+            elif k == 'nstrat':
+                if kargs['nstrat'] is None:
+                    continue
+                old_val['nstrat'] = self.nstrat
+                nstrat = np.array(kargs['nstrat'], dtype=np.intp)
+            elif k == 'sigf':
+                old_val['sigf'] = self.sigf
+                s = np.fabs(np.asarray(kargs['sigf'], float))
+                self._sigf = torch.as_tensor(s, dtype=torch.float64).cuda()
+                self.sum_sigf = float(np.sum(s))
+            elif k in Integrator.defaults:
+                old_val[k] = getattr(self, k, None)
+                v = kargs[k]
+                if k in ('neval', 'nitn', 'min_neval_batch', 'max_neval_hcube', 'maxinc_axis', 'nproc'):
+                    v = int(v) if v is not None else v
+                setattr(self, k, v)
+            elif k not in ('nhcube_batch', 'max_nhcube'):
+                raise AttributeError('no parameter named "%s"' % str(k))
+        if self.nproc not in (1, None) or self.mpi:
+            warnings.warn('nproc/mpi are ignored: the iteration runs on one GPU')
+        if self.ran_array_generator is not None:
+            raise NotImplementedError('ran_array_generator: samples come from the device Philox stream')
+        # 2) sanity checks
+        if nstrat is not None:
+            if len(nstrat) != self.map.dim:
+                raise ValueError('nstrat[d] has wrong length: %d not %d' % (len(nstrat), self.map.dim))
+            if np.any(nstrat < 1):
+                raise ValueError('bad nstrat: ' + str(np.asarray(self.nstrat)))
+        if self.neval_frac < 0 or self.neval_frac >= 1:
+            raise ValueError('neval_frac = {} but require 0 <= neval_frac < 1'.format(self.neval_frac))
+        if 'neval' in old_val and self.neval < 2:
+            raise ValueError('neval>2 required, not ' + str(self.neval))
+        nf = _geometry.effective_neval_frac(self.neval_frac, self.beta, self.adapt_to_errors)
+        # 3) strata per axis
+        if nstrat is not None:
+            nhcube = int(np.prod(nstrat))
+            if 'neval' not in old_val:
+                old_val['neval'] = self.neval
+                self.neval = int(2. * nhcube / (1. - nf))
+            elif self.neval < 2. * nhcube / (1. - nf):
+                raise ValueError('neval too small: {} < {}'.format(self.neval, 2. * nhcube / (1. - nf)))
+        elif 'neval' in old_val or 'neval_frac' in old_val:
+            nstrat = _geometry.auto_nstrat(self.dim, self.neval, nf, self.uniform_nstrat,
+                                           self.max_mem, self.minimize_mem)
+        else:
+            nstrat = np.array(self.nstrat)
+        # 4) increments aligned with strata
+        nstrat, ninc = _geometry.increments(nstrat, self.neval, self.maxinc_axis, self.adapt_to_errors)
+        if not np.all(np.equal(self.map.ninc, ninc)):
+            self.map.adapt(ninc=ninc)
+        if not np.all(np.equal(self.nstrat, nstrat)):
+            if 'sigf' not in old_val:
+                old_val['sigf'] = self.sigf
+                self._sigf = None
+                self.sum_sigf = HUGE
+            self.nstrat = nstrat
+        # 5) min_neval_hcube
+        self.nhcube = int(np.prod(self.nstrat))
+        self.min_neval_hcube = _geometry.min_neval_hcube(self.neval, self.nhcube, nf)
+        # 6) stratification weights (sigf) live on the device
+        if self.beta >= 0 and self.nhcube > 1 and not self.adapt_to_errors and \
+                (self._sigf is None or self._sigf.numel() != self.nhcube):
+            self._sigf = torch.ones(self.nhcube, dtype=torch.float64, device='cuda')
+            self.sum_sigf = float(self.nhcube)
+        _geometry.workspace(self.neval, self.dim, self.nhcube, self.min_neval_batch,
+                            self.max_neval_hcube, self.max_mem, self.minimize_mem)
+        return old_val
+
+    def settings(self, ngrid=0):
+        """Summary of the integrator settings (pyx:1448-1590)."""
+        nhcube = int(np.prod(self.nstrat))
+        neval = nhcube * self.min_neval_hcube if self.beta <= 0 else self.neval
+        ans = "Integrator Settings:\n"
+        if self.beta > 0 and not self.adapt_to_errors:
+            ans += "    %.6g (approx) integrand evaluations in each of %d iterations\n" % (self.neval, self.nitn)
+        else:
+            ans += "    %.6g integrand evaluations in each of %d iterations\n" % (neval, self.nitn)
+        ans += "    number of: strata/axis = %s\n" % np.array2string(
+            np.asarray(self.nstrat), max_line_width=80, prefix=29 * ' ')
+        ans += "               increments/axis = %s\n" % np.array2string(
+            np.asarray(self.map.ninc), max_line_width=80, prefix=33 * ' ')
+        ans += "               h-cubes = %.6g  processors = %d\n" % (nhcube, 1)
+        max_neval_hcube = max(self.max_neval_hcube, self.min_neval_hcube)
+        ans += "               evaluations/batch >= %.2g\n" % (float(self.min_neval_batch),)
+        ans += "               %d <= evaluations/h-cube <= %.2g\n" % (int(self.min_neval_hcube),
+                                                                     float(max_neval_hcube))
+        ans += "    minimize_mem = %s  adapt_to_errors = %s  adapt = %s\n" % (
+            str(self.minimize_mem), str(self.adapt_to_errors), str(self.adapt))
+        ans += "    accuracy: relative = %g  absolute = %g\n" % (self.rtol, self.atol)
+        if not self.adapt:
+            ans += "    damping: alpha = %g  beta= %g\n\n" % (0., 0.)
+        elif self.adapt_to_errors:
+            ans += "    damping: alpha = %g  beta= %g\n\n" % (self.alpha, 0.)
+        else:
+            ans += "    damping: alpha = %g  beta= %g\n\n" % (self.alpha, self.beta)
+        limits = ['({:.5}, {:.5})'.format(*lim) for lim in self.map.region()]
+        entries = [(str(a), lim) for a, lim in enumerate(limits)]
+        w1 = max(max(len(e[0]) for e in entries), len('axis'))
+        w2 = max(max(len(e[1]) for e in entries), len('integration limits'))
+        lines = ['{0:>{1}}    {2:>{3}}'.format('axis', w1, 'integration limits', w2)]
+        lines.append(len(lines[0]) * '-')
+        lines += ['{0:>{1}}    {2:>{3}}'.format(e[0], w1, e[1], w2) for e in entries]
+        ans += 4 * ' ' + ('\n' + 4 * ' ').join(lines) + '\n'
+        if ngrid > 0:
+            ans += '\n' + self.map.settings(ngrid=ngrid)
+        return ans
+
+    # --------------------------------------------------------- device plumbing
+    def _precision(self, device_fcn):
+        p = self.precision
+        if p is None:
+            p = 'f32' if device_fcn else 'f64'
+        if p not in ('f32', 'f64'):
+            raise ValueError("precision must be 'f32' or 'f64'")
+        return _lib.PREC_F32 if p == 'f32' else _lib.PREC_F64
+
+    def _config(self, nitn, iteration, prec):
+        c = _lib.VgxConfig()
+        c.dim = self.dim
+        c.ld = self.map.ld
+        for d in range(self.dim):
+            c.ninc[d] = int(self.map.ninc[d])
+            c.nstrat[d] = int(self.nstrat[d])
+        c.nhcube = self.nhcube
+        c.neval = int(self.neval)
+        c.neval_frac = float(self.neval_frac)
+        c.alpha = float(self.alpha)
+        c.beta = float(self.beta)
+        c.adapt = int(bool(self.adapt))
+        c.adapt_to_errors = int(bool(self.adapt_to_errors))
+        c.min_neval_hcube = int(self.min_neval_hcube)
+        c.max_neval_hcube = int(self.max_neval_hcube)
+        c.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
+        c.iteration = int(iteration) & 0xFFFFFFFF
+        c.nitn = int(nitn)
+        c.precision = prec
+        c.flags = (_lib.FLAG_EXACT_ADAPT | _lib.FLAG_EXACT_DIV) if self.exact else 0
+        c.weighted = int(bool(self.adapt))
+        return c
+
+    def _state(self, torch, nitn):
+        dev = 'cuda'
+        f64 = torch.float64
+        if self._sigf is None:
+            self._sigf = torch.ones(max(self.nhcube, 1), dtype=f64, device=dev)
+        nb = (self.dim, self.map.ld - 1)
+        bufs = dict(
+            sum_sigf=torch.tensor([self.sum_sigf], dtype=f64, device=dev),
+            sum_f=torch.zeros(nb, dtype=f64, device=dev),
+            n_f=torch.zeros(nb, dtype=f64, device=dev),
+            itn_mean=torch.zeros(nitn, dtype=f64, device=dev),
+            itn_var=torch.zeros(nitn, dtype=f64, device=dev),
+            ravg=torch.zeros(4, dtype=f64, device=dev),
+            itn_neval=torch.zeros(nitn, dtype=torch.int64, device=dev),
+            itn_nhrange=torch.zeros(2 * nitn, dtype=torch.int64, device=dev),
+            status=torch.zeros(1, dtype=torch.int32, device=dev),
+        )
+        st = _lib.VgxState()
+        st.grid = self.map._grid.data_ptr()
+        st.sigf = self._sigf.data_ptr()
+        for k, v in bufs.items():
+            setattr(st, k, v.data_ptr())
+        return st, bufs
+
+    def _read_back(self, bufs):
+        # one synchronising copy of everything the host needs
+        status = int(bufs['status'].item())
+        res = {k: bufs[k].cpu().numpy() for k in ('itn_mean', 'itn_var', 'itn_neval',
+                                                   'itn_nhrange', 'ravg', 'sum_sigf')}
+        return status, res
+
+    # ---------------------------------------------------------- integration
+    def __call__(self, fcn, save=None, saveall=None, **kargs):
+        """Integrate ``fcn`` (pyx:1912-2230); returns an :class:`RAvg`."""
+        if kargs:
+            self.set(kargs)
+        torch = _lib.torch_cuda()
+        result = RAvg(weighted=self.adapt)
+        device_fcn = isinstance(fcn, DeviceIntegrand)
+        if device_fcn and fcn.dim != self.dim:
+            raise ValueError('integrand dimension %d != integrator dimension %d' % (fcn.dim, self.dim))
+        prec = self._precision(device_fcn)
+        per_itn = bool(self.rtol or self.atol or self.analyzer is not None or save or saveall)
+        if device_fcn and not per_itn:
+            self._run_device(torch, fcn, self.nitn, prec, result)
+        else:
+            std = None if device_fcn else self._std_integrand(fcn)
+            for itn in range(self.nitn):
+                if self.analyzer is not None:
+                    self.analyzer.begin(itn, self)
+                if device_fcn:
+                    self._run_device(torch, fcn, 1, prec, result)
+                else:
+                    self._run_staged(torch, std, prec, result)
+                if self.analyzer is not None:
+                    self.analyzer.end(result.itn_results[-1], result)
+                if save is not None:
+                    _dump(save, result)
+                if saveall is not None:
+                    _dump(saveall, (result, self))
+                if result.converged(self.rtol, self.atol):
+                    break
+        self.map.clear()
+        return result
+
+    def _finish(self, status, res, n, result):
+        if status == _lib.VGX_ENAN:
+            raise ValueError('integrand evaluates to nan')
+        if status:
+            _lib.check(status, 'integrate')
+        for k in range(n):
+            result.add(GVar(res['itn_mean'][k], math.sqrt(max(res['itn_var'][k], 0.0))
+                            if res['itn_var'][k] >= 0 else float('nan')))
+            result.sum_neval += int(res['itn_neval'][k])
+        self.last_neval = int(res['itn_neval'][n - 1])
+        self.neval_hcube_range = np.array(res['itn_nhrange'][2 * (n - 1):2 * n])
+        if self.beta > 0 and not self.adapt_to_errors and self.adapt:
+            self.sum_sigf = float(res['sum_sigf'][0])
+        self._last_device_ravg = res['ravg']
+
+    def _run_device(self, torch, fcn, nitn, prec, result):
+        st, bufs = self._state(torch, nitn)
+        f, keep = fcn.vgx_integrand(torch, 'cuda')
+        cfg = self._config(nitn, self._rng_iter, prec)
+        _lib.check(_lib.lib().vgx_integrate(_lib.ctx(), C.byref(cfg), C.byref(f), C.byref(st),
+                                            _lib.stream_ptr()), 'vgx_integrate')
+        status, res = self._read_back(bufs)
+        self._rng_iter += nitn
+        self._finish(status, res, nitn, result)
+
+    def _std_integrand(self, fcn):
+        """Normalise fcn to f(x[n, dim] device float64) -> f[n] (pyx:2959-3383
+        for scalar integrands)."""
+        torch = _lib.torch_cuda()
+        kind = getattr(fcn, 'fcntype', 'scalar')
+        on_gpu = bool(getattr(fcn, 'vgx_torch', False))
+        uses_jac = self.uses_jac
+
+        def call(x, jac):
+            if kind == 'rbatch':
+                xa, ja = x.T, (None if jac is None else jac.T)
+            elif kind == 'lbatch':
+                xa, ja = x, jac
+            else:
+                xa, ja = x, jac
+            if on_gpu:
+                fx = fcn(xa) if ja is None else fcn(xa, jac=ja)
+                fx = torch.as_tensor(fx, dtype=torch.float64, device='cuda')
+            else:
+                xn = xa.cpu().numpy()
+                jn = None if ja is None else ja.cpu().numpy()
+                if kind in ('lbatch', 'rbatch'):
+                    fx = fcn(xn) if jn is None else fcn(xn, jac=jn)
+                else:
+                    fx = np.array([fcn(xi) if jn is None else fcn(xi, jac=jn[i])
+                                   for i, xi in enumerate(xn)], dtype=float)
+                fx = np.asarray(fx, dtype=float)
+                fx = torch.as_tensor(fx, dtype=torch.float64).cuda()
+            n = x.shape[0]
+            if kind == 'rbatch':
+                fx = fx.reshape(-1, n).T
+            else:
+                fx = fx.reshape(n, -1)
+            if fx.shape[1] != 1:
+                raise NotImplementedError('multi-component integrands are not supported yet')
+            return fx[:, 0].contiguous()
+
+        call.uses_jac = uses_jac
+        return call
+
+    def _run_staged(self, torch, std, prec, result):
+        L = _lib.lib()
+        st, bufs = self._state(torch, 1)
+        cfg = self._config(1, self._rng_iter, prec)
+        nh = torch.empty(self.nhcube, dtype=torch.int32, device='cuda')
+        total = C.c_int64()
+        cx, sp = _lib.ctx(), _lib.stream_ptr()
+        _lib.check(L.vgx_iter_begin(cx, C.byref(cfg), C.byref(st), 0, _lib.ptr(nh), C.byref(total),
+                                    sp), 'iter_begin')
+        for h0, h1, p0, p1 in _geometry.batches(nh.cpu().numpy(), self.min_neval_batch):
+            n = p1 - p0
+            x = torch.empty((n, self.dim), dtype=torch.float64, device='cuda')
+            y = torch.empty((n, self.dim), dtype=torch.float64, device='cuda') if std.uses_jac else None
+            _lib.check(L.vgx_iter_sample(cx, C.byref(cfg), C.byref(st), 0, h0, h1, p0, p1, _lib.ptr(x),
+                                         _lib.ptr(y), None, None, sp), 'iter_sample')
+            jac = self.map.jac1d(y) if y is not None else None
+            f = std(x, jac)
+            if bool(torch.isnan(f).any()):
+                raise ValueError('integrand evaluates to nan')
+            _lib.check(L.vgx_iter_accumulate(cx, C.byref(cfg), C.byref(st), 0, h0, h1, p0, p1,
+                                             _lib.ptr(f), sp), 'iter_accumulate')
+        _lib.check(L.vgx_iter_end(cx, C.byref(cfg), C.byref(st), 0, sp), 'iter_end')
+        status, res = self._read_back(bufs)
+        self._rng_iter += 1
+        self._finish(status, res, 1, result)
+
+    # -------------------------------------------------------- sampler APIs
+    def random_batch(self, yield_hcube=False, yield_y=False):
+        """Batches of integration points and weights for one iteration
+        (pyx:1601-1634, 1636-1765): yields (x, [y,] wgt[, hcube])."""
+        torch = _lib.torch_cuda()
+        L = _lib.lib()
+        st, bufs = self._state(torch, 1)
+        cfg = self._config(1, self._rng_iter, self._precision(False))
+        nh = torch.empty(self.nhcube, dtype=torch.int32, device='cuda')
+        total = C.c_int64()
+        cx, sp = _lib.ctx(), _lib.stream_ptr()
+        _lib.check(L.vgx_iter_begin(cx, C.byref(cfg), C.byref(st), 0, _lib.ptr(nh), C.byref(total), sp),
+                   'iter_begin')
+        self._rng_iter += 1
+        nhh = nh.cpu().numpy()
+        self.last_neval = int(total.value)
+        for h0, h1, p0, p1 in _geometry.batches(nhh, self.min_neval_batch):
+            n = p1 - p0
+            x = torch.empty((n, self.dim), dtype=torch.float64, device='cuda')
+            y = torch.empty_like(x) if yield_y else None
+            w = torch.empty(n, dtype=torch.float64, device='cuda')
+            hc = torch.empty(n, dtype=torch.int64, device='cuda') if yield_hcube else None
+            _lib.check(L.vgx_iter_sample(cx, C.byref(cfg), C.byref(st), 0, h0, h1, p0, p1, _lib.ptr(x),
+                                         _lib.ptr(y), _lib.ptr(w), _lib.ptr(hc), sp), 'iter_sample')
+            ans = (x.cpu().numpy(),)
+            if yield_y:
+                ans += (y.cpu().numpy(),)
+            ans += (w.cpu().numpy(),)
+            if yield_hcube:
+                ans += (hc.cpu().numpy(),)
+            yield ans
+
+    random_vec = random_batch
+
+    def random(self, yield_hcube=False, yield_y=False):
+        """Single points and weights (pyx:1770-1814)."""
+        for t in self.random_batch(yield_hcube=yield_hcube, yield_y=yield_y):
+            for i in range(t[0].shape[0]):
+                yield tuple(ti[i] for ti in t)
+
+    def sample(self, nbatch=None, mode='rbatch'):
+        """Sample of weights and points covering whole iterations (pyx:1816-1863)."""
+        neval = self.last_neval if self.last_neval > 0 else self.neval
+        nbatch = neval if nbatch is None else int(nbatch)
+        nit = nbatch // neval
+        if nit * neval < nbatch:
+            nit += 1
+        samples, wgts = [], []
+        for _ in range(nit):
+            for x, w in self.random_batch():
+                samples.append(x)
+                wgts.append(w)
+        samples = np.concatenate(samples, axis=0)
+        wgts = np.concatenate(wgts) / nit
+        if mode == 'rbatch':
+            samples = samples.T
+        return wgts, samples
+
+
+def _dump(target, obj):
+    if isinstance(target, str):
+        with open(target, 'wb') as f:
+            pickle.dump(obj, f)
+    else:
+        pickle.dump(obj, target)
+
+
+class reporter:
+    """Analyzer printing a per-iteration report (pyx:2232-2269)."""
+
+    def __init__(self, ngrid=0):
+        import time
+        self.ngrid = ngrid
+        self.clock = time.perf_counter
+
+    def begin(self, itn, integrator):
+        self.integrator = integrator
+        self.itn = itn
+        self.t0 = self.clock()
+        if itn == 0:
+            print(integrator.settings())
+
+    def end(self, itn_ans, ans):
+        print("    itn %2d: %s\n all itn's: %s" % (self.itn + 1, itn_ans, ans))
+        print('    neval = %s  neval/h-cube = %s\n    chi2/dof = %.2f  Q = %.2f  time = %.2f' % (
+            format(self.integrator.last_neval, '.6g'), tuple(self.integrator.neval_hcube_range),
+            ans.chi2 / ans.dof if ans.dof > 0 else 0, ans.Q if ans.dof > 0 else 1.,
+            self.clock() - self.t0))
+        print(self.integrator.map.settings(ngrid=self.ngrid))
+        print('')
