@@ -1,0 +1,126 @@
+"""Benchmark: vegas+ iterations of the 8-D double-peak workload on one MI355X.
+
+A step is one vegas+ iteration (allocation, sampling, map, integrand,
+per-hcube sums, training histogram, adapt, running average) over
+neval = 1e7 evaluations of the 8-D double Gaussian, all on the device.
+`value` is integrand evaluations per second over the K timed steps (points
+already resident: the whole iteration is generated on the device).
+
+Also reported:
+  roofline      the fused sampler kernel's algorithmic FP32 work per launch
+                (evaluations x flops/eval) over its average HIP-event time
+  cpu_baseline  the FP64 CPU oracle (reference-structured C restatement,
+                1 thread) on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "integrand evals/s; GPU wall-time to 1e-4 rel err (vegas grid vs learned map)"
+PEAK_FP32_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=15)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-iters", type=int, default=2)
+    return p.parse_args()
+
+
+def cpu_baseline(wl, iters):
+    """Reference-structured FP64 oracle on `iters` iterations of the workload
+    (single thread, integrand inlined in C)."""
+    from oracle import oracle as orc
+    f = wl["f"]
+    g = orc.GaussMix(f.A, f.B, f.centers)
+    I = orc.OracleIntegrator([(0, 1)] * wl["dim"], neval=wl["neval"], seed=1)
+    t0 = time.perf_counter()
+    _, _, nev = I.run(g, iters)
+    dt = time.perf_counter() - t0
+    return dict(value=float(nev.sum()) / dt, unit="evals/s", cores=1, kind="port",
+                sample="%d iteration(s) of the 8-D double peak, neval 1e7 (%d evals) through "
+                       "the FP64 C oracle, 1 thread, %.1f s" % (iters, int(nev.sum()), dt))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import vegas_amd as va
+    from vegas_amd import _lib, workloads
+
+    wl = workloads.double_peak_8d()
+    integ = va.Integrator([[0, 1]] * wl["dim"], neval=wl["neval"], seed=2026,
+                          precision=wl["precision"])
+    f = wl["f"]
+    if a.warmup > 0:
+        integ(f, nitn=a.warmup)
+    torch.cuda.synchronize()
+    L, cx = _lib.lib(), _lib.ctx()
+    L.vgx_set_timing(cx, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = integ(f, nitn=a.steps)  # reads back once, at the end
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    nt = max(a.steps, 1)
+    ms = (C.c_float * nt)()
+    got = L.vgx_fused_times(cx, ms, nt)
+    L.vgx_set_timing(cx, 0)
+    evals = [int(g) for g in getattr(integ, "_last_itn_neval", [])] or [integ.last_neval] * a.steps
+    total = float(r.sum_neval)
+    value = total / dt
+    fpe = workloads.flops_per_eval(wl["dim"], wl["ncomp"])
+    kern_ms = [ms[i] for i in range(got)]
+    avg_ms = sum(kern_ms) / len(kern_ms) if kern_ms else float("nan")
+    avg_evals = total / a.steps
+    achieved = avg_evals * fpe / (avg_ms * 1e-3) / 1e12 if kern_ms else None
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "fused_double_peak_8d_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC, "value": value, "unit": "evals/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": dt * 1e3 / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (device Philox samples)",
+        "config": {"workload": "vegas+ 8-D double peak, sigma 0.02, [0,1]^8, neval 1e7/iteration",
+                   "dim": 8, "neval": int(wl["neval"]), "nstrat": [int(s) for s in integ.nstrat],
+                   "nhcube": int(integ.nhcube), "ninc": [int(n) for n in integ.map.ninc],
+                   "precision": "integrand/RNG f32; map, weights, sums f64",
+                   "parallelism": "single GPU"},
+        "result": {"mean": r.mean, "sdev": r.sdev, "chi2_dof": r.chi2 / max(r.dof, 1), "Q": r.Q,
+                   "exact": wl["exact"]},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
+                     "traffic": traffic, "kernel": "fused_kernel<8, GaussMix32>",
+                     "kernel_ms_avg": avg_ms, "flops_per_eval": fpe,
+                     "note": "VALU-bound (no MFMA in this kernel); peak is the FP32 vector peak"},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl, a.cpu_iters)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -179,8 +179,12 @@ int vgx_iter_accumulate(vgx_ctx* ctx, const vgx_config* cfg, vgx_state* st, int 
 int vgx_iter_end(vgx_ctx* ctx, const vgx_config* cfg, vgx_state* st, int it,
                  void* stream);
 
-/* Last-kernel timing of the fused sampler (HIP events on `stream`), ms. */
-int vgx_last_fused_ms(vgx_ctx* ctx, float* ms);
+/* Timing of the fused sampler kernel: vgx_set_timing(ctx, 1) clears the
+ * record and brackets every following fused launch (up to 256) with HIP
+ * events on its stream; vgx_fused_times writes the durations in ms and
+ * returns how many were recorded (synchronises on the last event).      */
+int vgx_set_timing(vgx_ctx* ctx, int on);
+int vgx_fused_times(vgx_ctx* ctx, float* ms, int cap);
 
 #ifdef __cplusplus
 }

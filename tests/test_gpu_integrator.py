@@ -33,7 +33,7 @@ def double_peak(va, dim=8, sigma=0.02):
     return va.GaussianMixture.isotropic([[0.33] * dim, [0.67] * dim], sigma)
 
 
-@pytest.mark.parametrize("neval,nstrat", [(20000, None), (3000, [5, 5, 4, 4]), (2e5, None)])
+@pytest.mark.parametrize("neval,nstrat", [(20000, None), (3300, [5, 5, 4, 4]), (2e5, None)])
 def test_first_iteration_parity(va, orc, neval, nstrat):
     f, fo = gauss4(va, orc)
     integ = va.Integrator([[0, 1]] * 4, neval=neval, nstrat=nstrat, seed=11, precision='f64',
@@ -47,6 +47,29 @@ def test_first_iteration_parity(va, orc, neval, nstrat):
     assert abs(m.var - out.var) <= 1e-10 * out.var
     np.testing.assert_allclose(integ.sigf, o.sigf, rtol=1e-11)
     assert abs(integ.sum_sigf - o.sigf.sum()) <= 1e-10 * o.sigf.sum()
+
+
+@pytest.mark.parametrize("neval,scale", [(30000, 3.0), (2e5, 6.0)])
+def test_parity_variable_nh(va, orc, neval, scale):
+    """Heavy-tailed sigf: n_h from 2 to thousands, hcubes spanning many wave
+    tiles and cut at every lane offset."""
+    f, fo = gauss4(va, orc)
+    integ = va.Integrator([[0, 1]] * 4, neval=neval, seed=17, precision='f64', exact=True,
+                          alpha=0.0)
+    rng = np.random.default_rng(int(neval))
+    sigf = np.exp(scale * rng.standard_normal(integ.nhcube))
+    integ.set(sigf=sigf)
+    r = integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * 4, neval=neval, seed=17, alpha=0.0)
+    o.sigf = sigf.copy()
+    o.sum_sigf = float(np.sum(sigf))
+    out, sf, nf = o.iterate(fo)
+    assert integ.last_neval == out.last_neval
+    assert integ.neval_hcube_range[1] == out.nh_max and out.nh_max > 600
+    m = r.itn_results[0]
+    assert abs(m.mean - out.mean) <= 1e-12 * abs(out.mean)
+    assert abs(m.var - out.var) <= 1e-10 * out.var
+    np.testing.assert_allclose(integ.sigf, o.sigf, rtol=1e-11)
 
 
 def test_training_and_adapt_parity(va, orc):

@@ -29,16 +29,38 @@ struct AdaptArgs {
     const double* n_f;
     double* new_grid;
     double* buf;  // exact mode scratch: avg and tmp, 2 x nb per axis (chain: one pair)
+    // in-place use by the integrator (all null for vgx_adapt):
+    double* old_rows;      // dim x ld copy of the old grid rows
+    double* clear_sum_f;   // training data to clear once read
+    double* clear_n_f;
+    const int32_t* status; // skip everything when nonzero (NaN seen)
 };
+
+// In-place mode: copy the axis' old row aside before it is overwritten and
+// return the row to read from.
+__device__ inline const double* source_row(const AdaptArgs& a, int d, int lanes) {
+    const double* g = a.grid + (int64_t)d * a.ld;
+    if (!a.old_rows) return g;
+    double* o = a.old_rows + (int64_t)d * a.ld;
+    for (int i = threadIdx.x; i < a.ld; i += lanes) o[i] = g[i];
+    return o;
+}
+
+__device__ inline void clear_row(const AdaptArgs& a, int d, int lanes) {
+    if (!a.clear_sum_f) return;
+    for (int i = threadIdx.x; i < a.nb; i += lanes) {
+        a.clear_sum_f[(int64_t)d * a.nb + i] = 0.0;
+        a.clear_n_f[(int64_t)d * a.nb + i] = kTiny;
+    }
+}
 
 // Fill the padding of one output row and nothing else.
 __device__ inline void pad_row(double* ng, int nn, int new_ld) {
     for (int i = nn + 1; i < new_ld; ++i) ng[i] = ng[nn];
 }
 
-__device__ void adapt_axis_exact(const AdaptArgs& a, int d, double* avg, double* tmp) {
+__device__ void adapt_axis_exact(const AdaptArgs& a, int d, double* avg, double* tmp, const double* g) {
     const int old = a.ninc[d], nn = a.new_ninc[d];
-    const double* g = a.grid + (int64_t)d * a.ld;
     double* ng = a.new_grid + (int64_t)d * a.new_ld;
     if (a.alpha != 0 && old > 1) {
         if (a.has_training) {
@@ -90,18 +112,32 @@ __device__ void adapt_axis_exact(const AdaptArgs& a, int d, double* avg, double*
 }
 
 __global__ void adapt_exact_kernel(AdaptArgs a) {
-    if (threadIdx.x != 0) return;
+    if (a.status && *a.status) return;
     if (a.chain) {
-        double* avg = a.buf;
-        double* tmp = a.buf + a.nb;
-        for (int i = 0; i < a.nb; ++i) avg[i] = 1.0;
-        for (int d = 0; d < a.dim; ++d) adapt_axis_exact(a, d, avg, tmp);
+        for (int d = 0; d < a.dim; ++d) source_row(a, d, blockDim.x);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double* avg = a.buf;
+            double* tmp = a.buf + a.nb;
+            for (int i = 0; i < a.nb; ++i) avg[i] = 1.0;
+            for (int d = 0; d < a.dim; ++d)
+                adapt_axis_exact(a, d, avg, tmp, a.old_rows ? a.old_rows + (int64_t)d * a.ld
+                                                            : a.grid + (int64_t)d * a.ld);
+        }
+        __syncthreads();
+        for (int d = 0; d < a.dim; ++d) clear_row(a, d, blockDim.x);
     } else {
         const int d = blockIdx.x;
-        double* avg = a.buf + (int64_t)d * 2 * a.nb;
-        double* tmp = avg + a.nb;
-        for (int i = 0; i < a.nb; ++i) avg[i] = 1.0;
-        adapt_axis_exact(a, d, avg, tmp);
+        const double* g = source_row(a, d, blockDim.x);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double* avg = a.buf + (int64_t)d * 2 * a.nb;
+            double* tmp = avg + a.nb;
+            for (int i = 0; i < a.nb; ++i) avg[i] = 1.0;
+            adapt_axis_exact(a, d, avg, tmp, g);
+        }
+        __syncthreads();
+        clear_row(a, d, blockDim.x);
     }
 }
 
@@ -122,17 +158,18 @@ __device__ double block_sum(double v, double* red) {
     return t;
 }
 
-constexpr int kAdaptLdsMax = 4096;  // old increments handled in LDS (2 x 32 KB)
+constexpr int kAdaptLdsMax = 4000;  // old increments handled in LDS (2 x 31.25 KB)
 
 __global__ __launch_bounds__(1024) void adapt_parallel_kernel(AdaptArgs a) {
     __shared__ double av[kAdaptLdsMax];
     __shared__ double P[kAdaptLdsMax];
     __shared__ double red[40];
+    if (a.status && *a.status) return;
     const int d = blockIdx.x;
     const int old = a.ninc[d], nn = a.new_ninc[d];
-    const double* g = a.grid + (int64_t)d * a.ld;
     double* ng = a.new_grid + (int64_t)d * a.new_ld;
     const int T = blockDim.x;
+    const double* g = source_row(a, d, T);
 
     for (int i = threadIdx.x; i < old; i += T) {
         double v = 1.0;
@@ -143,6 +180,7 @@ __global__ __launch_bounds__(1024) void adapt_parallel_kernel(AdaptArgs a) {
         av[i] = v;
     }
     __syncthreads();
+    clear_row(a, d, T);
     if (a.alpha > 0 && old > 1) {
         double part = 0.0;
         for (int i = threadIdx.x; i < old; i += T) {
@@ -205,6 +243,46 @@ __global__ __launch_bounds__(1024) void adapt_parallel_kernel(AdaptArgs a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) pad_row(ng, nn, a.new_ld);
+}
+
+// The integrator's adapt: in place on `grid`, skipped when *status != 0,
+// clearing the training data after reading it.
+int adapt_in_place(vgx_ctx* ctx, int dim, const int32_t* ninc, double* grid, int ld, double* sum_f,
+                   double* n_f, int has_training, double alpha, int exact, const int32_t* status,
+                   hipStream_t st) {
+    AdaptArgs a{};
+    a.dim = dim;
+    a.ld = ld;
+    a.nb = ld - 1;
+    a.new_ld = ld;
+    a.has_training = has_training;
+    a.alpha = alpha;
+    a.grid = grid;
+    a.sum_f = sum_f;
+    a.n_f = n_f;
+    a.new_grid = grid;
+    a.status = status;
+    a.clear_sum_f = sum_f;
+    a.clear_n_f = n_f;
+    int max_old = 0;
+    for (int d = 0; d < dim; ++d) {
+        a.ninc[d] = ninc[d];
+        a.new_ninc[d] = ninc[d];
+        max_old = ninc[d] > max_old ? ninc[d] : max_old;
+    }
+    a.chain = (!has_training && alpha > 0) ? 1 : 0;
+    if (!exact && max_old > kAdaptLdsMax) exact = 1;
+    size_t rows = (size_t)dim * ld;
+    size_t bufs = (size_t)(a.chain ? 1 : dim) * 2 * (size_t)a.nb;
+    int rc = ctx->adapt_tmp.ensure((rows + bufs) * sizeof(double));
+    if (rc) return rc;
+    a.old_rows = (double*)ctx->adapt_tmp.ptr;
+    a.buf = a.old_rows + rows;
+    if (exact || a.chain)
+        hipLaunchKernelGGL(adapt_exact_kernel, dim3(a.chain ? 1 : dim), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL(adapt_parallel_kernel, dim3(dim), dim3(1024), 0, st, a);
+    return check_launch();
 }
 
 }  // namespace vgx

@@ -46,9 +46,11 @@ extern "C" int vgx_ctx_create(vgx_ctx** out) {
         return rc;
     }
     c->device = c->info.device;
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
-        return VGX_EHIP;
+    for (int i = 0; i < 2 * vgx_ctx::kMaxTimed; ++i) {
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+            delete c;
+            return VGX_EHIP;
+        }
     }
     *out = c;
     return VGX_OK;
@@ -61,16 +63,23 @@ extern "C" int vgx_ctx_destroy(vgx_ctx* c) {
                            &c->tile_h, &c->blocksum, &c->tile_slots, &c->blk_part,
                            &c->strad_part, &c->wf_stage, &c->misc, &c->adapt_tmp, &c->errv};
     for (auto* s : all) s->release();
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (int i = 0; i < 2 * vgx_ctx::kMaxTimed; ++i)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     delete c;
     return VGX_OK;
 }
 
-extern "C" int vgx_last_fused_ms(vgx_ctx* c, float* ms) {
-    if (!c || !ms) return VGX_EINVAL;
-    if (!c->have_timing) return VGX_EINVAL;
-    VGX_HIP(hipEventSynchronize(c->ev1));
-    VGX_HIP(hipEventElapsedTime(ms, c->ev0, c->ev1));
+extern "C" int vgx_set_timing(vgx_ctx* c, int on) {
+    if (!c) return VGX_EINVAL;
+    c->timing_on = on ? 1 : 0;
+    c->ntimed = 0;
     return VGX_OK;
+}
+
+extern "C" int vgx_fused_times(vgx_ctx* c, float* ms, int cap) {
+    if (!c || (!ms && cap > 0)) return VGX_EINVAL;
+    const int n = c->ntimed < cap ? c->ntimed : cap;
+    if (c->ntimed > 0) VGX_HIP(hipEventSynchronize(c->ev[2 * (c->ntimed - 1) + 1]));
+    for (int i = 0; i < n; ++i) VGX_HIP(hipEventElapsedTime(&ms[i], c->ev[2 * i], c->ev[2 * i + 1]));
+    return n;
 }

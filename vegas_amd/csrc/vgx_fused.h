@@ -18,10 +18,18 @@ namespace vgx {
 struct AxisC {
     uint32_t m, l, nstrat, ninc;  // DivMagic of nstrat, strata and increments
     double inv_s;                 // 1 / nstrat
+    uint32_t per;                 // ninc / nstrat when ninc is a multiple of nstrat, else 0
+    uint32_t pad;
 };
 
-// Generates uniforms, y, x and J for point p of hcube h (R32: 24-bit
-// uniforms, one Philox block per 4 axes; else 53-bit, one per 2 axes).
+// Generates uniforms, y, x and J for point p of hcube h.
+//   R32 (24-bit uniforms): when ninc = per * nstrat the increment index and
+//       offset follow exactly from t = (c + u) * per with 24-bit integer
+//       multiplies: k = c*per + floor(u24*per / 2^24), offset = the low 24
+//       bits * 2^-24 (the map of the exact y = (c+u)/nstrat);
+//   otherwise (53-bit uniforms, or unaligned axes, or exact_div) y is formed
+//       in FP64 as the reference does, y = (c+u)/nstrat (pyx:1741), and the
+//       map computes t = y*ninc (pyx:351).
 template <int DIM, bool R32>
 __device__ __forceinline__ void eval_point(const IterGeom& g, const AxisC* __restrict__ ax,
                                            const double* __restrict__ G, uint32_t h, uint32_t p,
@@ -37,20 +45,37 @@ __device__ __forceinline__ void eval_point(const IterGeom& g, const AxisC* __res
         const uint32_t q = (uint32_t)(((uint64_t)__umulhi(t, a.m) + t) >> a.l);
         const uint32_t c = t - q * a.nstrat;
         t = q;
-        double u;
+        const int ni = (int)a.ninc;
+        int k, kv;
+        double dl;
         if (R32) {
             if ((d & 3) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
             const int e = d & 3;
-            u = u24(e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w);
+            const uint32_t wd = e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w;
+            const uint32_t ui = wd >> 8;
+            if (a.per && !g.exact_div) {
+                // both factors < 2^24: the backend selects the 24-bit multiplies
+                const uint64_t prod = (uint64_t)ui * (uint64_t)(a.per & 0xFFFFFu);
+                k = (int)(c * a.per + (uint32_t)(prod >> 24));
+                dl = (double)((uint32_t)prod & 0xFFFFFFu) * (1.0 / 16777216.0);
+                kv = (c == 0 && ui == 0) ? -1 : k;
+            } else {
+                const double yv = g.exact_div ? ((double)c + u24(wd)) / (double)a.nstrat
+                                              : ((double)c + u24(wd)) * a.inv_s;
+                const double tt = fmin(yv * (double)ni, (double)ni);
+                k = (int)tt;
+                dl = tt - (double)k;
+                kv = (yv > 0 && yv < 1) ? (k < ni ? k : ni - 1) : -1;
+            }
         } else {
             if ((d & 1) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
-            u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);
+            const double u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);
+            const double yv = g.exact_div ? ((double)c + u) / (double)a.nstrat : ((double)c + u) * a.inv_s;
+            const double tt = fmin(yv * (double)ni, (double)ni);
+            k = (int)tt;
+            dl = tt - (double)k;
+            kv = (yv > 0 && yv < 1) ? (k < ni ? k : ni - 1) : -1;
         }
-        const double yv = g.exact_div ? ((double)c + u) / (double)a.nstrat : ((double)c + u) * a.inv_s;
-        const int ni = (int)a.ninc;
-        const double tt = fmin(yv * (double)ni, (double)ni);
-        const int k = (int)tt;
-        const double dl = tt - (double)k;
         const double* gd = G + d * g.ld;
         if (k < ni) {
             const double inc = gd[k + 1] - gd[k];
@@ -60,12 +85,38 @@ __device__ __forceinline__ void eval_point(const IterGeom& g, const AxisC* __res
             x[d] = gd[ni];
             J *= (gd[ni] - gd[ni - 1]) * (double)ni;
         }
-        kbin[d] = (yv > 0 && yv < 1) ? (k < ni ? k : ni - 1) : -1;
+        kbin[d] = kv;
+    }
+}
+
+// y of point p of hcube h, formed as eval_point forms it (staged sampler).
+template <bool R32>
+__device__ inline void point_y(const IterGeom& g, const AxisC* __restrict__ ax, uint32_t h,
+                               uint32_t p, double* y) {
+    uint32_t t = h;
+    U4 w{0, 0, 0, 0};
+    for (int d = 0; d < g.dim; ++d) {
+        const AxisC a = ax[d];
+        const uint32_t q = (uint32_t)(((uint64_t)__umulhi(t, a.m) + t) >> a.l);
+        const uint32_t c = t - q * a.nstrat;
+        t = q;
+        double u;
+        if (R32) {
+            if ((d & 3) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
+            const int e = d & 3;
+            u = u24(e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w);
+        } else {
+            if ((d & 1) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
+            u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);
+        }
+        y[d] = g.exact_div ? ((double)c + u) / (double)a.nstrat : ((double)c + u) * a.inv_s;
     }
 }
 
 __host__ __device__ inline AxisC make_axis(const IterGeom& g, int d) {
-    return AxisC{g.dm[d].m, g.dm[d].l, (uint32_t)g.nstrat[d], (uint32_t)g.ninc[d], g.inv_nstrat[d]};
+    const uint32_t s = (uint32_t)g.nstrat[d], n = (uint32_t)g.ninc[d];
+    const uint32_t per = (n % s == 0 && n / s < (1u << 20)) ? n / s : 0u;
+    return AxisC{g.dm[d].m, g.dm[d].l, s, n, g.inv_nstrat[d], per, 0u};
 }
 
 // Two-value version of seg_prefix (shares the segment flags).
@@ -268,17 +319,27 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
 #pragma unroll
             for (int j = 0; j < kK - 1; ++j) endf[j] = key[j + 1] != key[j];
             endf[kK - 1] = (lane == kWave - 1) || (next_first != key[kK - 1]);
-            double vlast = P[kK - 1];
-            int fl = endf[kK - 1];
+            // Total of the lane's last segment.  If it ends in this lane it
+            // is P[kK-1]; otherwise it ends inside the first run of the
+            // nearest later lane whose first run ends there: copy that run's
+            // closing prefix backwards across the lanes that only continue.
+            double fv = P[0];
+#pragma unroll
+            for (int j = 1; j < kK; ++j)
+                if (key[j] == key[0]) fv = P[j];
+            double rv = fv;
+            int rf = (key[0] != key[kK - 1]) || endf[kK - 1];
 #pragma unroll
             for (int o = 1; o < kWave; o <<= 1) {
-                double nv = __shfl_down(vlast, o, kWave);
-                int nf = __shfl_down(fl, o, kWave);
-                if (!fl && lane + o < kWave) {
-                    vlast = nv;
-                    fl = nf;
+                const double nv = __shfl_down(rv, o, kWave);
+                const int nf = __shfl_down(rf, o, kWave);
+                if (!rf && lane + o < kWave) {
+                    rv = nv;
+                    rf = nf;
                 }
             }
+            const double next_rv = __shfl_down(rv, 1, kWave);
+            const double vlast = endf[kK - 1] ? P[kK - 1] : next_rv;
             double S[kK];
             {
                 double c = vlast;

@@ -2,16 +2,18 @@
 // src/vegas/_vegas.pyx:2086-2227) and its non-fused kernels.
 //
 // Per iteration, all on one stream, no host synchronisation:
-//   iter_init      clear training data (AdaptiveMap.clear, pyx:596-599)
-//   alloc/scan/offsets   n_h by the damped-variance rule (pyx:1657-1706),
-//                  exclusive scan to point offsets, tile -> first hcube
-//   fused          sample, map, integrand, per-hcube sums, histogram
-//   strad0/1/2     hcubes cut by tile edges: totals, guarded deviations,
-//                  per-hcube results
-//   batch_reduce   iteration sums (compensated, fixed order)
-//   finalize       estimate, sum_sigf (pyx:2209-2215), running average
-//                  (pyx:2392-2410)
-//   adapt          AdaptiveMap.adapt(alpha) (pyx:2216-2217)
+//   alloc, scan, offsets  n_h by the damped-variance rule (pyx:1657-1706),
+//                         exclusive scan to point offsets, tile -> hcube
+//   fused                 sample, map, integrand, per-hcube sums, histogram
+//   strad0, 1, 2          hcubes cut by tile edges: totals, guarded
+//                         deviations per part, then per-hcube results
+//   batch_reduce          iteration sums (compensated, fixed order)
+//   sigf_pow              sigf = |var|^(beta/2) and its sum (pyx:2183-2186)
+//   finalize              estimate, sum_sigf (pyx:2209-2215), running average
+//                         (pyx:2392-2410); resets the per-iteration scalars
+//   adapt                 AdaptiveMap.adapt(alpha) in place (pyx:2216-2217);
+//                         clears the training data for the next iteration
+// The run starts with iter_init and ends with sigf_reset.
 #include <math.h>
 #include <string.h>
 
@@ -19,33 +21,35 @@
 
 namespace vgx {
 
-// --------------------------------------------------------------- init ----
-__global__ void iter_init_kernel(IterGeom g, double* sum_f, double* n_f, Scalars* sc, int clear_hist) {
+// Resets the per-iteration scalars (the NaN flag too on a new run) and
+// clears the training data (AdaptiveMap.clear, pyx:596-599).
+__global__ void iter_init_kernel(IterGeom g, double* sum_f, double* n_f, Scalars* sc, int new_run) {
     const int tot = g.dim * g.nb;
-    if (clear_hist) {
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
-            sum_f[i] = 0.0;
-            n_f[i] = kTiny;
-        }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+        sum_f[i] = 0.0;
+        n_f[i] = kTiny;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        sc->nh_min = g.nmin;
-        sc->nh_max = g.nmin;
+        if (new_run) {
+            sc->status = 0;  // the NaN flag is sticky within one run only
+            sc->reset_sigf = 0;
+        }
         sc->wfn_max_bits = 0;
         for (int k = 0; k < 6; ++k) sc->acc[k] = 0.0;
-        sc->reset_sigf = 0;
         sc->overflow = 0;
     }
 }
 
 // ------------------------------------------------------------ allocation ----
 // n_h = min(int(sigf_h * neval_sigf) + nmin, max(nmax, nmin))   (pyx:1696-1698)
-__global__ __launch_bounds__(kAllocThreads) void alloc_kernel(IterGeom g, const double* sigf,
+// Per-workgroup total, min and max go to stat[3 * blockIdx.x + {0,1,2}].
+__global__ __launch_bounds__(kAllocThreads) void alloc_kernel(IterGeom g, double* sigf,
                                                               const double* sum_sigf, int32_t* nh,
-                                                              uint64_t* blocksum, Scalars* sc) {
+                                                              uint64_t* stat, const Scalars* sc) {
     __shared__ uint64_t red[kAllocThreads / 64];
     __shared__ int32_t rmin[kAllocThreads / 64], rmax[kAllocThreads / 64];
     if (sc->status) return;
+    const int reset = sc->reset_sigf;  // previous iteration found sum(sigf) == 0
     const double ss = *sum_sigf;
     const double neval_sigf = (g.beta > 0 && ss > 0 && g.adaptive_sampler) ? g.neval_frac * g.neval / ss : 0.0;
     uint64_t sum = 0;
@@ -56,7 +60,14 @@ __global__ __launch_bounds__(kAllocThreads) void alloc_kernel(IterGeom g, const 
         if (h >= g.nhcube) break;
         int32_t n;
         if (g.adaptive_sampler) {
-            double v = sigf[h] * neval_sigf;
+            double sv;
+            if (reset) {
+                sv = 1.0;
+                sigf[h] = 1.0;
+            } else {
+                sv = sigf[h];
+            }
+            double v = sv * neval_sigf;
             v = v < 2147483647.0 ? v : 2147483647.0;
             int64_t nn = (int64_t)(int32_t)v + g.nmin;
             if (nn > g.nmax) nn = g.nmax;
@@ -88,24 +99,28 @@ __global__ __launch_bounds__(kAllocThreads) void alloc_kernel(IterGeom g, const 
             mn = min(mn, rmin[k]);
             mx = max(mx, rmax[k]);
         }
-        blocksum[blockIdx.x] = s;
-        if (g.adaptive_sampler) {
-            atomicMin(&sc->nh_min, mn);
-            atomicMax(&sc->nh_max, mx);
-        }
+        stat[3 * blockIdx.x] = s;
+        stat[3 * blockIdx.x + 1] = (uint64_t)(uint32_t)mn;
+        stat[3 * blockIdx.x + 2] = (uint64_t)(uint32_t)mx;
     }
 }
 
-// Exclusive scan of the per-block totals (one workgroup), total points and
-// the tile count of the whole-iteration batch.
-__global__ __launch_bounds__(1024) void scan_kernel(IterGeom g, uint64_t* blocksum, int nblk,
-                                                    Scalars* sc, uint64_t capacity) {
+// Exclusive scan of the per-workgroup totals (one workgroup), total points,
+// neval_hcube_range and the tile count of the whole-iteration batch.
+__global__ __launch_bounds__(1024) void scan_kernel(IterGeom g, uint64_t* stat, int nblk, Scalars* sc,
+                                                    uint64_t capacity) {
     __shared__ uint64_t part[1024];
+    __shared__ int32_t pmin[1024], pmax[1024];
     if (sc->status) return;
     uint64_t carry = 0;
+    int32_t mn = g.nmin, mx = g.nmin;
     for (int base = 0; base < nblk; base += 1024) {
         const int i = base + threadIdx.x;
-        const uint64_t v = i < nblk ? blocksum[i] : 0;
+        const uint64_t v = i < nblk ? stat[3 * i] : 0;
+        if (i < nblk) {
+            mn = min(mn, (int32_t)stat[3 * i + 1]);
+            mx = max(mx, (int32_t)stat[3 * i + 2]);
+        }
         part[threadIdx.x] = v;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
@@ -114,15 +129,27 @@ __global__ __launch_bounds__(1024) void scan_kernel(IterGeom g, uint64_t* blocks
             part[threadIdx.x] += add;
             __syncthreads();
         }
-        if (i < nblk) blocksum[i] = carry + part[threadIdx.x] - v;
+        if (i < nblk) stat[3 * i] = carry + part[threadIdx.x] - v;
         const uint64_t ct = part[1023];
         __syncthreads();
         carry += ct;
+    }
+    pmin[threadIdx.x] = mn;
+    pmax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) {
+            pmin[threadIdx.x] = min(pmin[threadIdx.x], pmin[threadIdx.x + o]);
+            pmax[threadIdx.x] = max(pmax[threadIdx.x], pmax[threadIdx.x + o]);
+        }
+        __syncthreads();
     }
     if (threadIdx.x == 0) {
         const int ok = carry <= capacity && carry < 0xFFFFFFFFull;
         sc->overflow = ok ? 0u : 1u;
         sc->total = ok ? (uint32_t)carry : 0u;
+        sc->nh_min = pmin[0];
+        sc->nh_max = pmax[0];
         sc->b_p0 = 0;
         sc->b_p1 = sc->total;
         sc->b_h0 = 0;
@@ -131,27 +158,26 @@ __global__ __launch_bounds__(1024) void scan_kernel(IterGeom g, uint64_t* blocks
     }
 }
 
-// Point offsets off[h] (exclusive scan of n_h) and tile -> first hcube.
+// Point offsets off[h] (exclusive scan of n_h, written coalesced through LDS)
+// and tile -> first hcube.
 __global__ __launch_bounds__(kAllocThreads) void offsets_kernel(IterGeom g, const int32_t* nh,
-                                                                const uint64_t* blockoff,
-                                                                uint32_t* off, int32_t* tile_h,
-                                                                Scalars* sc) {
-    __shared__ int32_t ln[kAllocPerBlock];
+                                                                const uint64_t* stat, uint32_t* off,
+                                                                int32_t* tile_h, const Scalars* sc) {
+    __shared__ uint32_t ln[kAllocPerBlock];
     __shared__ uint32_t wsum[kAllocThreads / 64];
     if (sc->status || sc->overflow) return;
     const int64_t base = (int64_t)blockIdx.x * kAllocPerBlock;
     for (int i = threadIdx.x; i < kAllocPerBlock; i += kAllocThreads) {
         const int64_t h = base + i;
-        ln[i] = h < g.nhcube ? nh[h] : 0;
+        ln[i] = h < g.nhcube ? (uint32_t)nh[h] : 0u;
     }
     __syncthreads();
     uint32_t loc[kAllocItems];
     uint32_t run = 0;
     for (int i = 0; i < kAllocItems; ++i) {
         loc[i] = run;
-        run += (uint32_t)ln[threadIdx.x * kAllocItems + i];
+        run += ln[threadIdx.x * kAllocItems + i];
     }
-    // exclusive scan of per-thread totals across the workgroup
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t incl = run;
     for (int o = 1; o < 64; o <<= 1) {
@@ -162,15 +188,23 @@ __global__ __launch_bounds__(kAllocThreads) void offsets_kernel(IterGeom g, cons
     __syncthreads();
     uint32_t wbase = 0;
     for (int k = 0; k < w; ++k) wbase += wsum[k];
-    const uint32_t tbase = (uint32_t)blockoff[blockIdx.x] + wbase + incl - run;
+    const uint32_t tbase = (uint32_t)stat[3 * blockIdx.x] + wbase + incl - run;
     for (int i = 0; i < kAllocItems; ++i) {
         const int64_t h = base + threadIdx.x * kAllocItems + i;
-        if (h >= g.nhcube) break;
         const uint32_t lo = tbase + loc[i];
-        const uint32_t n = (uint32_t)ln[threadIdx.x * kAllocItems + i];
-        off[h] = lo;
-        const uint32_t hi = lo + n;
-        for (uint32_t t = (lo + kT - 1) / kT; t < (hi + kT - 1) / kT; ++t) tile_h[t] = (int32_t)h;
+        const uint32_t n = ln[threadIdx.x * kAllocItems + i];
+        if (h < g.nhcube) {
+            const uint32_t hi = lo + n;
+            for (uint32_t t = (lo + kT - 1) / kT; t < (hi + kT - 1) / kT; ++t) tile_h[t] = (int32_t)h;
+        }
+        loc[i] = lo;
+    }
+    __syncthreads();
+    for (int i = 0; i < kAllocItems; ++i) ln[threadIdx.x * kAllocItems + i] = loc[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < kAllocPerBlock; i += kAllocThreads) {
+        const int64_t h = base + i;
+        if (h < g.nhcube) off[h] = ln[i];
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[g.nhcube] = sc->total;
 }
@@ -192,153 +226,231 @@ __global__ void batch_tiles_kernel(const uint32_t* off, int32_t* tile_h, Scalars
 }
 
 // ---------------------------------------------------------- straddlers ----
+// One thread per tile slot.  A slot (tile t, part 0 = first segment, part 1 =
+// last segment) holds the tile's share of an hcube cut by a tile edge.
 struct StradArgs {
     double* errv;
-    const AxisC* axes;
     const uint32_t* off;
     Slot* slots;
     const double* wf_stage;
     double* sigf;
     double* strad_part;
     Scalars* sc;
-    double* sum_f;
-    double* n_f;
-    const double* grid;
 };
 
-__device__ inline int slot_of(const Slot* slots, uint32_t t, int32_t h) {
-    return slots[2 * (size_t)t].h == h ? (int)(2 * t) : (int)(2 * t + 1);
+__device__ inline const Slot& slot_of(const Slot* slots, uint32_t t, int32_t h) {
+    return slots[2 * (size_t)t].h == h ? slots[2 * (size_t)t] : slots[2 * (size_t)t + 1];
 }
 
-// Sum over the tiles [tf, tl] of one field of h's slot (lanes in parallel,
-// fixed butterfly).
-__device__ inline double strad_sum(const Slot* slots, uint32_t tf, uint32_t tl, int32_t h, int field) {
+// One wave per slot; the slot's hcube h, its point range and tile range.
+struct SlotCtx {
+    int32_t h;
+    uint32_t t, hlo, hhi, tf, tl;
+    bool owner;
+};
+
+__device__ inline bool load_slot(const StradArgs& a, uint32_t s, SlotCtx& c) {
+    const Scalars* sc = a.sc;
+    const uint32_t ntiles = sc->b_ntiles, p0 = sc->b_p0;
+    if (s >= 2 * ntiles) return false;
+    c.h = a.slots[s].h;
+    if (c.h < 0) return false;
+    c.t = s >> 1;
+    c.hlo = a.off[c.h];
+    c.hhi = a.off[c.h + 1];
+    c.tf = (c.hlo - p0) / kT;
+    c.tl = (c.hhi - 1 - p0) / kT;
+    c.owner = c.hlo >= p0 + c.t * kT;
+    return true;
+}
+
+// Sum over the hcube's tiles of one slot field, lanes in parallel then a
+// fixed butterfly: the same order for every caller.
+__device__ inline double parts_sum(const Slot* slots, const SlotCtx& c, int field) {
     double v = 0.0;
-    for (uint32_t t = tf + lane_id(); t <= tl; t += kWave) {
-        const Slot& s = slots[slot_of(slots, t, h)];
-        v += field == 0 ? s.S : field == 1 ? s.Q : s.D;
+    for (uint32_t u = c.tf + lane_id(); u <= c.tl; u += kWave) {
+        const Slot& o = slot_of(slots, u, c.h);
+        v += field == 0 ? o.S : field == 1 ? o.Q : o.D;
     }
     return wave_sum_fixed(v);
 }
 
-// mode 0: owner writes Stot; mode 1: every part computes Q, D; mode 2:
-// owner combines and produces the hcube's results.
-template <int MODE>
-__global__ __launch_bounds__(256) void strad_kernel(IterGeom g, StradArgs a) {
-    __shared__ double red[4 * 6];
-    Scalars* sc = a.sc;
+// Owner slots: S of the whole cut hcube.
+__global__ __launch_bounds__(256) void strad0_kernel(StradArgs a) {
+    if (a.sc->status || a.sc->overflow) return;
+    const uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    SlotCtx c;
+    if (!load_slot(a, s, c) || !c.owner) return;
+    const double S = parts_sum(a.slots, c, 0);
+    if (lane_id() == 0) a.slots[s].Stot = S;
+}
+
+// Every slot: guarded deviations of its part from the hcube mean
+// (pyx:2153-2169).
+__global__ __launch_bounds__(256) void strad1_kernel(StradArgs a) {
+    const Scalars* sc = a.sc;
     if (sc->status || sc->overflow) return;
-    const uint32_t ntiles = sc->b_ntiles, p0 = sc->b_p0, p1 = sc->b_p1;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t gw = blockIdx.x * 4 + wave, nw = gridDim.x * 4;
-    DD am{0, 0}, av{0, 0}, as{0, 0};
-    for (uint32_t t = gw; t < ntiles; t += nw) {
-        const uint32_t plo = p0 + t * kT;
-        const uint32_t phi = min(plo + (uint32_t)kT, p1);
-        for (int s = 0; s < 2; ++s) {
-            Slot* sl = &a.slots[2 * (size_t)t + s];
-            const int32_t h = sl->h;
-            if (h < 0) continue;
-            const uint32_t hlo = a.off[h], hhi = a.off[h + 1];
-            const uint32_t tf = (hlo - p0) / kT, tl = (hhi - 1 - p0) / kT;
-            const bool owner = hlo >= plo;
-            if (MODE == 0) {
-                if (!owner) continue;
-                const double S = strad_sum(a.slots, tf, tl, h, 0);
-                if (lane == 0) sl->Stot = S;
-            } else if (MODE == 1) {
-                const double Stot = a.slots[slot_of(a.slots, tf, h)].Stot;
-                const double n = (double)(hhi - hlo);
-                const double m = Stot / n;
-                const double thr = kEpsilon * fabs(m);
-                const uint32_t lo = max(hlo, plo), hi = min(hhi, phi);
-                double q = 0.0, d = 0.0;
-                for (uint32_t p = lo + lane; p < hi; p += kWave) {
-                    const double dd = a.wf_stage[p] - m;
-                    if (fabs(dd) < thr) {
-                        q += thr * thr;
-                    } else {
-                        q += dd * dd;
-                        d += dd;
-                    }
-                }
-                q = wave_sum_fixed(q);
-                d = wave_sum_fixed(d);
-                if (lane == 0) {
-                    sl->Q = q;
-                    sl->D = d;
-                }
-            } else {
-                if (!owner) continue;
-                const double S = sl->Stot;
-                const double Q = strad_sum(a.slots, tf, tl, h, 1);
-                const double D = strad_sum(a.slots, tf, tl, h, 2);
-                const double n = (double)(hhi - hlo);
-                const double varc = (n * Q - D * D) / (n - 1.);
-                if (lane == 0) {
-                    dd_add(am, S + D);
-                    dd_add(av, varc);
-                    if (g.adaptive_strat) a.sigf[h] = fabs(varc);
-                    if (g.train_errors) a.errv[h] = fabs(varc);
-                }
-            }
+    const uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    SlotCtx c;
+    if (!load_slot(a, s, c)) return;
+    const uint32_t p0 = sc->b_p0, p1 = sc->b_p1;
+    const double S = slot_of(a.slots, c.tf, c.h).Stot;
+    const double n = (double)(c.hhi - c.hlo);
+    const double m = S / n;
+    const double thr = kEpsilon * fabs(m);
+    const uint32_t plo = p0 + c.t * kT;
+    const uint32_t lo = max(c.hlo, plo), hi = min(c.hhi, min(plo + (uint32_t)kT, p1));
+    double q = 0.0, d = 0.0;
+    for (uint32_t p = lo + lane_id(); p < hi; p += kWave) {
+        const double dd = a.wf_stage[p] - m;
+        if (fabs(dd) < thr) {
+            q += thr * thr;
+        } else {
+            q += dd * dd;
+            d += dd;
         }
     }
-    if (MODE == 2) {
-        if (lane == 0) {
-            red[wave * 6 + 0] = am.hi; red[wave * 6 + 1] = am.lo;
-            red[wave * 6 + 2] = av.hi; red[wave * 6 + 3] = av.lo;
-            red[wave * 6 + 4] = as.hi; red[wave * 6 + 5] = as.lo;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            DD m{0, 0}, v{0, 0}, s{0, 0};
-            for (int k = 0; k < 4; ++k) {
-                m = dd_sum(m, DD{red[k * 6 + 0], red[k * 6 + 1]});
-                v = dd_sum(v, DD{red[k * 6 + 2], red[k * 6 + 3]});
-                s = dd_sum(s, DD{red[k * 6 + 4], red[k * 6 + 5]});
-            }
-            double* o = a.strad_part + 6 * (size_t)blockIdx.x;
-            o[0] = m.hi; o[1] = m.lo; o[2] = v.hi; o[3] = v.lo; o[4] = s.hi; o[5] = s.lo;
-        }
+    q = wave_sum_fixed(q);
+    d = wave_sum_fixed(d);
+    if (lane_id() == 0) {
+        a.slots[s].Q = q;
+        a.slots[s].D = d;
     }
 }
 
-// Adds the batch's partial sums (fused blocks, then straddler blocks, in a
-// fixed order) into the iteration accumulators.
-__global__ __launch_bounds__(256) void batch_reduce_kernel(const double* blk_part, int nblk,
+// Owner slots combine the parts: mean += S + D, var += (nQ - D^2)/(n-1),
+// |var| for sigf (pyx:2177-2185); per-workgroup compensated partials.
+__global__ __launch_bounds__(1024) void strad2_kernel(IterGeom g, StradArgs a) {
+    __shared__ double red[16 * 4];
+    const int w = threadIdx.x >> 6;
+    DD m{0, 0}, v{0, 0};
+    if (!(a.sc->status || a.sc->overflow)) {
+        const uint32_t s = blockIdx.x * 16 + w;
+        SlotCtx c;
+        if (load_slot(a, s, c) && c.owner) {
+            const double Q = parts_sum(a.slots, c, 1);
+            const double D = parts_sum(a.slots, c, 2);
+            if (lane_id() == 0) {
+                const double n = (double)(c.hhi - c.hlo);
+                const double varc = (n * Q - D * D) / (n - 1.);
+                dd_add(m, a.slots[s].Stot + D);
+                dd_add(v, varc);
+                if (g.adaptive_strat) a.sigf[c.h] = fabs(varc);
+                if (g.train_errors) a.errv[c.h] = fabs(varc);
+            }
+        }
+    }
+    if (lane_id() == 0) {
+        red[4 * w] = m.hi; red[4 * w + 1] = m.lo; red[4 * w + 2] = v.hi; red[4 * w + 3] = v.lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD M{0, 0}, V{0, 0};
+        for (int k = 0; k < 16; ++k) {
+            M = dd_sum(M, DD{red[4 * k], red[4 * k + 1]});
+            V = dd_sum(V, DD{red[4 * k + 2], red[4 * k + 3]});
+        }
+        double* o = a.strad_part + 6 * (size_t)blockIdx.x;
+        o[0] = M.hi; o[1] = M.lo; o[2] = V.hi; o[3] = V.lo; o[4] = 0.0; o[5] = 0.0;
+    }
+}
+
+// Fixed-order tree over DD triples: red[k*6 .. k*6+5] for k < blockDim.x.
+__device__ void tree_dd3(double* red) {
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) {
+            const int i = threadIdx.x, j = threadIdx.x + o;
+            for (int q = 0; q < 3; ++q) {
+                DD r = dd_sum(DD{red[6 * i + 2 * q], red[6 * i + 2 * q + 1]},
+                              DD{red[6 * j + 2 * q], red[6 * j + 2 * q + 1]});
+                red[6 * i + 2 * q] = r.hi;
+                red[6 * i + 2 * q + 1] = r.lo;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Adds the batch's partial sums (fused workgroups, then straddler
+// workgroups, fixed order) into the iteration accumulators.
+__global__ __launch_bounds__(512) void batch_reduce_kernel(const double* blk_part, int nblk,
                                                            const double* strad_part, int nstrad,
                                                            Scalars* sc) {
-    __shared__ double red[256 * 6];
+    __shared__ double red[512 * 6];
     if (sc->status || sc->overflow) return;
-    DD m{0, 0}, v{0, 0}, s{0, 0};
+    DD m{0, 0}, v{0, 0};
     const int tot = nblk + nstrad;
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         const double* p = i < nblk ? blk_part + 6 * (size_t)i : strad_part + 6 * (size_t)(i - nblk);
         m = dd_sum(m, DD{p[0], p[1]});
         v = dd_sum(v, DD{p[2], p[3]});
-        s = dd_sum(s, DD{p[4], p[5]});
     }
-    red[threadIdx.x * 6 + 0] = m.hi; red[threadIdx.x * 6 + 1] = m.lo;
-    red[threadIdx.x * 6 + 2] = v.hi; red[threadIdx.x * 6 + 3] = v.lo;
-    red[threadIdx.x * 6 + 4] = s.hi; red[threadIdx.x * 6 + 5] = s.lo;
+    red[6 * threadIdx.x + 0] = m.hi; red[6 * threadIdx.x + 1] = m.lo;
+    red[6 * threadIdx.x + 2] = v.hi; red[6 * threadIdx.x + 3] = v.lo;
+    red[6 * threadIdx.x + 4] = 0.0; red[6 * threadIdx.x + 5] = 0.0;
     __syncthreads();
+    tree_dd3(red);
     if (threadIdx.x == 0) {
-        DD M{sc->acc[0], sc->acc[1]}, V{sc->acc[2], sc->acc[3]}, Sg{sc->acc[4], sc->acc[5]};
-        for (int k = 0; k < (int)blockDim.x; ++k) {
-            M = dd_sum(M, DD{red[k * 6 + 0], red[k * 6 + 1]});
-            V = dd_sum(V, DD{red[k * 6 + 2], red[k * 6 + 3]});
-            Sg = dd_sum(Sg, DD{red[k * 6 + 4], red[k * 6 + 5]});
-        }
+        const DD M = dd_sum(DD{sc->acc[0], sc->acc[1]}, DD{red[0], red[1]});
+        const DD V = dd_sum(DD{sc->acc[2], sc->acc[3]}, DD{red[2], red[3]});
         sc->acc[0] = M.hi; sc->acc[1] = M.lo;
         sc->acc[2] = V.hi; sc->acc[3] = V.lo;
-        sc->acc[4] = Sg.hi; sc->acc[5] = Sg.lo;
     }
 }
 
-// Iteration close: estimate, sum_sigf bookkeeping, running average (FP64).
-__global__ void finalize_kernel(IterGeom g, Scalars* sc, vgx_state st, int has_sigf) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// sigf_h = |var_h|^(beta/2) (pyx:2185) and per-workgroup sums, after every
+// hcube of the iteration is done.
+__global__ __launch_bounds__(256) void sigf_pow_kernel(IterGeom g, double* sigf, double* part, const Scalars* sc) {
+    __shared__ double red[8];
+    if (sc->status || sc->overflow) return;
+    DD s{0, 0};
+    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < g.nhcube;
+         h += (int64_t)gridDim.x * blockDim.x) {
+        // exact mode: pow as the reference; otherwise exp2(b log2 x), within
+        // ~1e-14 relative (sigf only steers the allocation)
+        const double x = sigf[h];
+        const double v = g.exact_div ? pow(x, g.beta_half) : (x > 0 ? exp2(g.beta_half * log2(x)) : 0.0);
+        sigf[h] = v;
+        dd_add(s, v);
+    }
+    s = wave_sum_dd(s);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * w] = s.hi;
+        red[2 * w + 1] = s.lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD t{0, 0};
+        for (int k = 0; k < 4; ++k) t = dd_sum(t, DD{red[2 * k], red[2 * k + 1]});
+        part[2 * blockIdx.x] = t.hi;
+        part[2 * blockIdx.x + 1] = t.lo;
+    }
+}
+
+// Iteration close (one workgroup): sum of sigf from the sigf_pow partials,
+// the estimate, sum_sigf bookkeeping and the FP64 running average; resets the
+// per-iteration scalars for the next iteration of the run.
+__global__ __launch_bounds__(256) void finalize_kernel(IterGeom g, Scalars* sc, vgx_state st, int has_sigf,
+                                                       const double* sigf_part, int nsigf) {
+    __shared__ double red[256 * 2];
+    DD t{0, 0};
+    if (g.adaptive_strat)
+        for (int k = threadIdx.x; k < nsigf; k += blockDim.x)
+            t = dd_sum(t, DD{sigf_part[2 * k], sigf_part[2 * k + 1]});
+    red[2 * threadIdx.x] = t.hi;
+    red[2 * threadIdx.x + 1] = t.lo;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) {
+            DD r = dd_sum(DD{red[2 * threadIdx.x], red[2 * threadIdx.x + 1]},
+                          DD{red[2 * (threadIdx.x + o)], red[2 * (threadIdx.x + o) + 1]});
+            red[2 * threadIdx.x] = r.hi;
+            red[2 * threadIdx.x + 1] = r.lo;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     if (sc->overflow) {
         if (st.status) *st.status = VGX_ERANGE;
         sc->status = VGX_ERANGE;
@@ -351,18 +463,20 @@ __global__ void finalize_kernel(IterGeom g, Scalars* sc, vgx_state st, int has_s
     const int it = (int)g.itn;
     const double mean = sc->acc[0] + sc->acc[1];
     const double var = sc->acc[2] + sc->acc[3];
-    const double ssig = sc->acc[4] + sc->acc[5];
+    const double ssig = red[0] + red[1];
     st.itn_mean[it] = mean;
     st.itn_var[it] = var;
     st.itn_neval[it] = (int64_t)sc->total;
-    st.itn_nhrange[2 * it] = sc->nh_min;
-    st.itn_nhrange[2 * it + 1] = sc->nh_max;
-    // sum_sigf (pyx:2209-2215)
+    st.itn_nhrange[2 * it] = g.adaptive_sampler ? sc->nh_min : g.nmin;
+    st.itn_nhrange[2 * it + 1] = g.adaptive_sampler ? sc->nh_max : g.nmin;
+    sc->reset_sigf = 0;
+    // sum_sigf (pyx:2209-2215); sigf[:] = 1 is applied lazily by the next
+    // allocation, and by sigf_reset_kernel at the end of the run
     if (g.beta > 0 && g.adapt && !g.train_errors) {
         if (ssig > 0) {
             *st.sum_sigf = ssig;
         } else {
-            sc->reset_sigf = 1;
+            sc->reset_sigf = has_sigf;
             *st.sum_sigf = has_sigf ? (double)g.nhcube : 0.0;
         }
     }
@@ -406,41 +520,15 @@ __global__ void finalize_kernel(IterGeom g, Scalars* sc, vgx_state st, int has_s
     st.ravg[1] = rs;
     st.ravg[2] = chi2;
     st.ravg[3] = (double)(n - 1);
+    // next iteration of this run
+    sc->wfn_max_bits = 0;
+    for (int k = 0; k < 6; ++k) sc->acc[k] = 0.0;
 }
 
-// sigf_h = |var_h|^(beta/2) and its sum (pyx:2183-2186): one streaming pass
-// after all hcubes of the iteration are done.
-__global__ __launch_bounds__(256) void sigf_pow_kernel(IterGeom g, double* sigf, double* part, Scalars* sc) {
-    __shared__ double red[8];
-    if (sc->status || sc->overflow) return;
-    DD s{0, 0};
-    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < g.nhcube;
-         h += (int64_t)gridDim.x * blockDim.x) {
-        const double v = pow(sigf[h], g.beta_half);
-        sigf[h] = v;
-        dd_add(s, v);
-    }
-    s = wave_sum_dd(s);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red[2 * w] = s.hi;
-        red[2 * w + 1] = s.lo;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        DD t{0, 0};
-        for (int k = 0; k < 4; ++k) t = dd_sum(t, DD{red[2 * k], red[2 * k + 1]});
-        part[2 * blockIdx.x] = t.hi;
-        part[2 * blockIdx.x + 1] = t.lo;
-    }
-}
-
-__global__ void sigf_sum_kernel(const double* part, int n, Scalars* sc) {
-    if (threadIdx.x != 0 || sc->status || sc->overflow) return;
-    DD t{0, 0};
-    for (int k = 0; k < n; ++k) t = dd_sum(t, DD{part[2 * k], part[2 * k + 1]});
-    sc->acc[4] = t.hi;
-    sc->acc[5] = t.lo;
+__global__ void sigf_reset_kernel(double* sigf, int64_t n, Scalars* sc) {
+    if (!sc->reset_sigf || sc->status) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        sigf[i] = 1.0;
 }
 
 // adapt_to_errors: training entry |var_h| at the last point of every hcube
@@ -464,17 +552,6 @@ __global__ __launch_bounds__(256) void errors_train_kernel(IterGeom g, const Axi
     }
 }
 
-__global__ void sigf_reset_kernel(double* sigf, int64_t n, const Scalars* sc) {
-    if (!sc->reset_sigf || sc->status) return;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        sigf[i] = 1.0;
-}
-
-__global__ void grid_commit_kernel(double* grid, const double* tmp, int n, const Scalars* sc) {
-    if (sc->status) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) grid[i] = tmp[i];
-}
-
 __global__ void axes_init_kernel(IterGeom g, AxisC* axes) {
     if (threadIdx.x < (unsigned)g.dim) axes[threadIdx.x] = make_axis(g, threadIdx.x);
 }
@@ -484,10 +561,11 @@ __global__ void to_float_kernel(const double* a, float* b, int n) {
 }
 
 // Staged path: x, y, weight and hcube of points [p0, p1) (pyx:1731-1759).
-__global__ __launch_bounds__(256) void sample_kernel(IterGeom g, const AxisC* axes, const double* grid, const uint32_t* off,
-                                                     uint32_t h0, uint32_t h1, uint32_t p0, uint32_t p1,
-                                                     double* x, double* y, double* wgt, int64_t* hcube,
-                                                     const Scalars* sc) {
+template <bool R32>
+__global__ __launch_bounds__(256) void sample_kernel(IterGeom g, const AxisC* axes, const double* grid,
+                                                     const uint32_t* off, uint32_t h0, uint32_t h1,
+                                                     uint32_t p0, uint32_t p1, double* x, double* y,
+                                                     double* wgt, int64_t* hcube, const Scalars* sc) {
     if (sc->status) return;
     const int dim = g.dim;
     for (uint32_t p = p0 + blockIdx.x * blockDim.x + threadIdx.x; p < p1; p += gridDim.x * blockDim.x) {
@@ -501,31 +579,10 @@ __global__ __launch_bounds__(256) void sample_kernel(IterGeom g, const AxisC* ax
         double xx[VGX_MAXDIM];
         int kb[VGX_MAXDIM];
         double J;
-        if (g.rng32) eval_point<0, true>(g, axes, grid, h, p, xx, J, kb);
-        else eval_point<0, false>(g, axes, grid, h, p, xx, J, kb);
+        eval_point<0, R32>(g, axes, grid, h, p, xx, J, kb);
         const int64_t i = (int64_t)(p - p0);
         for (int d = 0; d < dim; ++d) x[i * dim + d] = xx[d];
-        if (y) {
-            // y is regenerated exactly as eval_point forms it
-            uint32_t t = h;
-            U4 w{0, 0, 0, 0};
-            for (int d = 0; d < dim; ++d) {
-                const uint32_t q = divq(t, g.dm[d]);
-                const uint32_t c = t - q * (uint32_t)g.nstrat[d];
-                t = q;
-                double u;
-                if (g.rng32) {
-                    if ((d & 3) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
-                    const int e = d & 3;
-                    u = u24(e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w);
-                } else {
-                    if ((d & 1) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
-                    u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);
-                }
-                y[i * dim + d] = g.exact_div ? ((double)c + u) / (double)g.nstrat[d]
-                                             : ((double)c + u) * g.inv_nstrat[d];
-            }
-        }
+        if (y) point_y<R32>(g, axes, h, p, y + i * dim);
         if (wgt) wgt[i] = J * (g.dv_y / (double)n);
         if (hcube) hcube[i] = (int64_t)h;
     }
@@ -576,7 +633,7 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
     // LDS plan of the fused kernel: axis table + FP64 grid always, the
     // privatised histogram when it also fits.
     const size_t tot = (size_t)c->dim * g->nb;
-    const size_t fixed = (size_t)VGX_MAXDIM * 24 + (size_t)16 * kNoff * sizeof(uint32_t) +
+    const size_t fixed = (size_t)VGX_MAXDIM * sizeof(AxisC) + (size_t)16 * kNoff * sizeof(uint32_t) +
                          16 * 6 * sizeof(double) + 64;
     const size_t gbytes = (size_t)c->dim * c->ld * sizeof(double);
     const size_t hbytes = tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t) + 16;
@@ -609,13 +666,13 @@ struct Run {
     FusedPtrs io;
     StradArgs sa;
     int fused_blocks;
+    int strad_blocks;
     size_t lds;
     int nblk_alloc;
     uint64_t cap;
     uint32_t maxtiles;
     Scalars* sc;
-    double* grid_tmp;
-    uint64_t* blocksum;
+    uint64_t* stat;
     int32_t* nh;
     float* fparams;
     AxisC* axes;
@@ -632,29 +689,30 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     R->maxtiles = (uint32_t)((R->cap + kT - 1) / kT) + 1;
     R->nblk_alloc = (int)((c->nhcube + kAllocPerBlock - 1) / kAllocPerBlock);
     R->fused_blocks = ctx->info.cus;
+    R->strad_blocks = (int)((2ull * R->maxtiles + 15) / 16);
     R->lds = fused_lds(g);
     if (R->lds > ctx->info.max_lds) return VGX_ERANGE;
     if ((rc = ctx->nh.ensure((size_t)c->nhcube * 4))) return rc;
     if ((rc = ctx->off.ensure(((size_t)c->nhcube + 1) * 4))) return rc;
-    if ((rc = ctx->blocksum.ensure((size_t)R->nblk_alloc * 8))) return rc;
+    if ((rc = ctx->blocksum.ensure((size_t)R->nblk_alloc * 3 * 8))) return rc;
     if ((rc = ctx->tile_h.ensure((size_t)R->maxtiles * 4))) return rc;
     if ((rc = ctx->tile_slots.ensure((size_t)R->maxtiles * 2 * sizeof(Slot)))) return rc;
     if ((rc = ctx->blk_part.ensure((size_t)R->fused_blocks * 6 * 8))) return rc;
-    if ((rc = ctx->strad_part.ensure((size_t)kStradBlocks * 6 * 8))) return rc;
+    if ((rc = ctx->strad_part.ensure((size_t)R->strad_blocks * 6 * 8))) return rc;
     if ((rc = ctx->wf_stage.ensure((size_t)R->cap * 8 + 64))) return rc;
     R->errv = nullptr;
-    if (R->g.train_errors) {
+    if (g.train_errors) {
         if ((rc = ctx->errv.ensure((size_t)c->nhcube * 8))) return rc;
         R->errv = (double*)ctx->errv.ptr;
     }
-    const size_t misc = 1024 + (size_t)c->dim * c->ld * 8 + VGX_MAXDIM * 32 + 4096 * 2;
-    if ((rc = ctx->misc.ensure(misc + 1024 * 1024))) return rc;
+    // misc: Scalars | axes | sigf partials | float integrand parameters
+    const size_t misc = 1024 + VGX_MAXDIM * sizeof(AxisC) + 2 * 256 * sizeof(double) + 1024 * 1024;
+    if ((rc = ctx->misc.ensure(misc))) return rc;
     R->sc = (Scalars*)ctx->misc.ptr;
-    R->grid_tmp = (double*)((char*)ctx->misc.ptr + 1024);
-    R->axes = (AxisC*)((char*)R->grid_tmp + (size_t)c->dim * c->ld * 8);
+    R->axes = (AxisC*)((char*)ctx->misc.ptr + 1024);
     R->sigf_part = (double*)((char*)R->axes + VGX_MAXDIM * sizeof(AxisC));
     R->fparams = (float*)((char*)R->sigf_part + 2 * 256 * sizeof(double));
-    R->blocksum = (uint64_t*)ctx->blocksum.ptr;
+    R->stat = (uint64_t*)ctx->blocksum.ptr;
     R->nh = (int32_t*)ctx->nh.ptr;
     FusedPtrs& io = R->io;
     io.axes = R->axes;
@@ -670,7 +728,6 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     io.errv = R->errv;
     io.sc = R->sc;
     StradArgs& sa = R->sa;
-    sa.axes = R->axes;
     sa.errv = R->errv;
     sa.off = io.off;
     sa.slots = io.slots;
@@ -678,9 +735,6 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     sa.sigf = st->sigf;
     sa.strad_part = (double*)ctx->strad_part.ptr;
     sa.sc = R->sc;
-    sa.sum_f = st->sum_f;
-    sa.n_f = st->n_f;
-    sa.grid = st->grid;
     return VGX_OK;
 }
 
@@ -694,45 +748,52 @@ static int init_axes(Run* R, hipStream_t s) {
     return check_launch();
 }
 
-static int iter_begin_kernels(vgx_ctx* ctx, Run* R, vgx_state* st, hipStream_t s) {
+static int run_init(Run* R, vgx_state* st, hipStream_t s, int new_run) {
     const IterGeom& g = R->g;
     const int tot = g.dim * g.nb;
-    hipLaunchKernelGGL(iter_init_kernel, dim3((tot + 255) / 256 > 0 ? (tot + 255) / 256 : 1), dim3(256), 0,
-                       s, g, st->sum_f, st->n_f, R->sc, 1);
-    hipLaunchKernelGGL(alloc_kernel, dim3(R->nblk_alloc), dim3(kAllocThreads), 0, s, g,
-                       (const double*)st->sigf, (const double*)st->sum_sigf, R->nh, R->blocksum, R->sc);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, g, R->blocksum, R->nblk_alloc, R->sc, R->cap);
+    const int nb = (tot + 255) / 256 < 64 ? (tot + 255) / 256 : 64;
+    hipLaunchKernelGGL(iter_init_kernel, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, g, st->sum_f, st->n_f,
+                       R->sc, new_run);
+    return check_launch();
+}
+
+static int alloc_kernels(vgx_ctx* ctx, Run* R, vgx_state* st, hipStream_t s) {
+    const IterGeom& g = R->g;
+    hipLaunchKernelGGL(alloc_kernel, dim3(R->nblk_alloc), dim3(kAllocThreads), 0, s, g, st->sigf,
+                       (const double*)st->sum_sigf, R->nh, R->stat, (const Scalars*)R->sc);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, g, R->stat, R->nblk_alloc, R->sc, R->cap);
     hipLaunchKernelGGL(offsets_kernel, dim3(R->nblk_alloc), dim3(kAllocThreads), 0, s, g,
-                       (const int32_t*)R->nh, (const uint64_t*)R->blocksum, (uint32_t*)ctx->off.ptr,
-                       (int32_t*)ctx->tile_h.ptr, R->sc);
+                       (const int32_t*)R->nh, (const uint64_t*)R->stat, (uint32_t*)ctx->off.ptr,
+                       (int32_t*)ctx->tile_h.ptr, (const Scalars*)R->sc);
     return check_launch();
 }
 
 template <class Fcn>
 static int accumulate_kernels(vgx_ctx* ctx, Run* R, const Fcn& fcn, hipStream_t s, int timed) {
     const IterGeom& g = R->g;
-    if (timed) (void)hipEventRecord(ctx->ev0, s);
+    const int rec = timed && ctx->timing_on && ctx->ntimed < vgx_ctx::kMaxTimed;
+    if (rec) VGX_HIP(hipEventRecord(ctx->ev[2 * ctx->ntimed], s));
     int rc = launch_fused(g, R->io, fcn, R->fused_blocks, R->lds, s);
     if (rc) return rc;
-    if (timed) {
-        (void)hipEventRecord(ctx->ev1, s);
-        ctx->have_timing = 1;
+    if (rec) {
+        VGX_HIP(hipEventRecord(ctx->ev[2 * ctx->ntimed + 1], s));
+        ctx->ntimed++;
     }
-    hipLaunchKernelGGL(strad_kernel<0>, dim3(kStradBlocks), dim3(256), 0, s, g, R->sa);
-    hipLaunchKernelGGL(strad_kernel<1>, dim3(kStradBlocks), dim3(256), 0, s, g, R->sa);
-    hipLaunchKernelGGL(strad_kernel<2>, dim3(kStradBlocks), dim3(256), 0, s, g, R->sa);
-    hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(256), 0, s, (const double*)R->io.blk_part,
-                       R->fused_blocks, (const double*)R->sa.strad_part, kStradBlocks, R->sc);
+    const unsigned nslot4 = (unsigned)((2ull * R->maxtiles + 3) / 4);
+    hipLaunchKernelGGL(strad0_kernel, dim3(nslot4), dim3(256), 0, s, R->sa);
+    hipLaunchKernelGGL(strad1_kernel, dim3(nslot4), dim3(256), 0, s, R->sa);
+    hipLaunchKernelGGL(strad2_kernel, dim3(R->strad_blocks), dim3(1024), 0, s, g, R->sa);
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(512), 0, s, (const double*)R->io.blk_part,
+                       R->fused_blocks, (const double*)R->sa.strad_part, R->strad_blocks, R->sc);
     return check_launch();
 }
 
 static int iter_end_kernels(vgx_ctx* ctx, Run* R, const vgx_config* c, vgx_state* st, hipStream_t s) {
     const IterGeom& g = R->g;
     const int has_sigf = c->beta >= 0 && c->nhcube > 1 && !c->adapt_to_errors;
-    if (g.adaptive_strat) {
-        hipLaunchKernelGGL(sigf_pow_kernel, dim3(256), dim3(256), 0, s, g, st->sigf, R->sigf_part, R->sc);
-        hipLaunchKernelGGL(sigf_sum_kernel, dim3(1), dim3(64), 0, s, (const double*)R->sigf_part, 256, R->sc);
-    }
+    if (g.adaptive_strat)
+        hipLaunchKernelGGL(sigf_pow_kernel, dim3(256), dim3(256), 0, s, g, st->sigf, R->sigf_part,
+                           (const Scalars*)R->sc);
     if (g.train_errors) {
         const unsigned nb = (unsigned)((c->nhcube + 255) / 256 < 4096 ? (c->nhcube + 255) / 256 : 4096);
         if (g.rng32)
@@ -744,26 +805,25 @@ static int iter_end_kernels(vgx_ctx* ctx, Run* R, const vgx_config* c, vgx_state
                                (const double*)st->grid, (const uint32_t*)ctx->off.ptr, (const double*)R->errv,
                                st->sum_f, st->n_f, (const Scalars*)R->sc);
     }
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(64), 0, s, g, R->sc, *st, has_sigf);
-    hipLaunchKernelGGL(sigf_reset_kernel, dim3(256), dim3(256), 0, s, st->sigf, (int64_t)c->nhcube,
-                       (const Scalars*)R->sc);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, g, R->sc, *st, has_sigf,
+                       (const double*)R->sigf_part, 256);
     int rc = check_launch();
     if (rc) return rc;
     if (c->alpha > 0 && c->adapt) {
         const int has_training = g.train_points || g.train_errors;
-        rc = vgx_adapt(ctx, c->dim, c->ninc, st->grid, c->ld, has_training, st->sum_f, st->n_f, c->alpha,
-                       c->ninc, R->grid_tmp, c->ld, (c->flags & VGX_FLAG_EXACT_ADAPT) ? 1 : 0, s);
-        if (rc) return rc;
-        const int n = c->dim * c->ld;
-        hipLaunchKernelGGL(grid_commit_kernel, dim3((n + 255) / 256), dim3(256), 0, s, st->grid,
-                           (const double*)R->grid_tmp, n, (const Scalars*)R->sc);
-        rc = check_launch();
+        rc = adapt_in_place(ctx, c->dim, c->ninc, st->grid, c->ld, st->sum_f, st->n_f, has_training,
+                            c->alpha, (c->flags & VGX_FLAG_EXACT_ADAPT) ? 1 : 0, &R->sc->status, s);
     }
     return rc;
 }
 
+static int run_end(Run* R, const vgx_config* c, vgx_state* st, hipStream_t s) {
+    hipLaunchKernelGGL(sigf_reset_kernel, dim3(256), dim3(256), 0, s, st->sigf, (int64_t)c->nhcube, R->sc);
+    return check_launch();
+}
+
 static int make_fcn_and_run(vgx_ctx* ctx, Run* R, const vgx_config* c, const vgx_integrand* f,
-                            hipStream_t s, int timed) {
+                            hipStream_t s, int timed, int convert) {
     const bool f32 = c->precision == VGX_PREC_F32;
     if (f->kind == VGX_FCN_BUFFER) {
         if (f32) {
@@ -782,9 +842,11 @@ static int make_fcn_and_run(vgx_ctx* ctx, Run* R, const vgx_config* c, const vgx
             float* A = R->fparams;
             float* B = A + nc;
             float* C = B + nc;
-            hipLaunchKernelGGL(to_float_kernel, dim3(64), dim3(256), 0, s, f->A, A, nc);
-            hipLaunchKernelGGL(to_float_kernel, dim3(64), dim3(256), 0, s, f->B, B, nc);
-            hipLaunchKernelGGL(to_float_kernel, dim3(64), dim3(256), 0, s, f->c, C, ncc);
+            if (convert) {
+                hipLaunchKernelGGL(to_float_kernel, dim3(64), dim3(256), 0, s, f->A, A, nc);
+                hipLaunchKernelGGL(to_float_kernel, dim3(64), dim3(256), 0, s, f->B, B, nc);
+                hipLaunchKernelGGL(to_float_kernel, dim3(64), dim3(256), 0, s, f->c, C, ncc);
+            }
             if (f->kind == VGX_FCN_GAUSSMIX) {
                 GaussMix32 fn{nc, A, B, C};
                 return accumulate_kernels(ctx, R, fn, s, timed);
@@ -815,13 +877,14 @@ extern "C" int vgx_integrate(vgx_ctx* ctx, const vgx_config* c, const vgx_integr
     int rc = prepare(ctx, c, st, &R);
     if (rc) return rc;
     if ((rc = init_axes(&R, s))) return rc;
+    if ((rc = run_init(&R, st, s, 1))) return rc;
     for (int it = 0; it < c->nitn; ++it) {
         set_iter(&R, c, it);
-        if ((rc = iter_begin_kernels(ctx, &R, st, s))) return rc;
-        if ((rc = make_fcn_and_run(ctx, &R, c, f, s, it == c->nitn - 1))) return rc;
+        if ((rc = alloc_kernels(ctx, &R, st, s))) return rc;
+        if ((rc = make_fcn_and_run(ctx, &R, c, f, s, 1, it == 0))) return rc;
         if ((rc = iter_end_kernels(ctx, &R, c, st, s))) return rc;
     }
-    return VGX_OK;
+    return run_end(&R, c, st, s);
 }
 
 extern "C" int vgx_iter_begin(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, int it, int32_t* n_h,
@@ -833,7 +896,8 @@ extern "C" int vgx_iter_begin(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, 
     if (rc) return rc;
     if ((rc = init_axes(&R, s))) return rc;
     set_iter(&R, c, it);
-    if ((rc = iter_begin_kernels(ctx, &R, st, s))) return rc;
+    if ((rc = run_init(&R, st, s, 1))) return rc;
+    if ((rc = alloc_kernels(ctx, &R, st, s))) return rc;
     if (n_h) VGX_HIP(hipMemcpyAsync(n_h, R.nh, (size_t)c->nhcube * 4, hipMemcpyDeviceToDevice, s));
     if (total_host) {
         Scalars tmp;
@@ -859,9 +923,14 @@ extern "C" int vgx_iter_sample(vgx_ctx* ctx, const vgx_config* c, vgx_state* st,
     const int64_t n = p1 - p0;
     int64_t b = (n + 255) / 256;
     if (b > 4096) b = 4096;
-    hipLaunchKernelGGL(sample_kernel, dim3((unsigned)b), dim3(256), 0, s, R.g, (const AxisC*)R.axes, (const double*)st->grid,
-                       (const uint32_t*)ctx->off.ptr, (uint32_t)h0, (uint32_t)h1, (uint32_t)p0, (uint32_t)p1,
-                       x, y, wgt, hcube, (const Scalars*)R.sc);
+    if (R.g.rng32)
+        hipLaunchKernelGGL(sample_kernel<true>, dim3((unsigned)b), dim3(256), 0, s, R.g, (const AxisC*)R.axes,
+                           (const double*)st->grid, (const uint32_t*)ctx->off.ptr, (uint32_t)h0, (uint32_t)h1,
+                           (uint32_t)p0, (uint32_t)p1, x, y, wgt, hcube, (const Scalars*)R.sc);
+    else
+        hipLaunchKernelGGL(sample_kernel<false>, dim3((unsigned)b), dim3(256), 0, s, R.g, (const AxisC*)R.axes,
+                           (const double*)st->grid, (const uint32_t*)ctx->off.ptr, (uint32_t)h0, (uint32_t)h1,
+                           (uint32_t)p0, (uint32_t)p1, x, y, wgt, hcube, (const Scalars*)R.sc);
     return check_launch();
 }
 
@@ -884,7 +953,7 @@ extern "C" int vgx_iter_accumulate(vgx_ctx* ctx, const vgx_config* c, vgx_state*
     f.kind = VGX_FCN_BUFFER;
     f.fbuf = fv;
     f.fbase = p0;
-    return make_fcn_and_run(ctx, &R, c, &f, s, 0);
+    return make_fcn_and_run(ctx, &R, c, &f, s, 0, 0);
 }
 
 extern "C" int vgx_iter_end(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, int it, void* stream) {
@@ -895,5 +964,6 @@ extern "C" int vgx_iter_end(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, in
     if (rc) return rc;
     if ((rc = init_axes(&R, s))) return rc;
     set_iter(&R, c, it);
-    return iter_end_kernels(ctx, &R, c, st, s);
+    if ((rc = iter_end_kernels(ctx, &R, c, st, s))) return rc;
+    return run_end(&R, c, st, s);
 }

@@ -62,6 +62,8 @@ struct vgx_ctx {
     vgx::Scratch nh, off, tile_h, blocksum, tile_slots, blk_part, strad_part, wf_stage, misc, errv;
     // adapt
     vgx::Scratch adapt_tmp;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int have_timing = 0;
+    static constexpr int kMaxTimed = 256;
+    hipEvent_t ev[2 * kMaxTimed] = {};
+    int timing_on = 0;
+    int ntimed = 0;
 };

@@ -228,6 +228,12 @@ struct BufferFcn {
     }
 };
 
+// AdaptiveMap.adapt for the integrator loop (vgx_adapt.hip): in place, a
+// no-op once *status != 0, clears sum_f/n_f after reading them.
+int adapt_in_place(vgx_ctx* ctx, int dim, const int32_t* ninc, double* grid, int ld, double* sum_f,
+                   double* n_f, int has_training, double alpha, int exact, const int32_t* status,
+                   hipStream_t st);
+
 // Launchers, one per (integrand, dim) instantiation (vgx_fused_*.hip).
 template <class Fcn>
 int launch_fused(const IterGeom& g, const FusedPtrs& io, const Fcn& fcn, int grid_blocks,
