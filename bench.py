@@ -35,8 +35,23 @@ def parse():
     p.add_argument("--steps", type=int, default=15)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-iters", type=int, default=2)
+    p.add_argument("--cpu-iters", type=int, default=5)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only copy 512 MiB device to device 4 times (HBM counter calibration)")
     return p.parse_args()
+
+
+def copy_calibration():
+    """Plain device copy of a known size: FETCH_SIZE / WRITE_SIZE per dispatch
+    of this copy, against 512 MiB, scale the fused kernel's counters to bytes."""
+    import torch
+    n = 512 << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "dispatches": 4}))
 
 
 def cpu_baseline(wl, iters):
@@ -60,7 +75,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
-    import numpy as np
+    if a.copy_calibration:
+        copy_calibration()
+        return
     import torch
 
     import vegas_amd as va

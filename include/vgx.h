@@ -122,7 +122,8 @@ typedef struct {
 /* Device integrands.  GAUSSMIX: f(x) = sum_k A[k] exp(-B[k] |x - c[k]|^2)
  * with c double[ncomp][dim].  DIAGRIDGE: the same mixture with centres on
  * the main diagonal, c[k] = t[k]*(1,...,1) (c holds t[ncomp]); evaluated
- * with |x-c|^2 = |x|^2 - 2 t sum(x) + dim t^2.  BUFFER: f values supplied
+ * with |x-c|^2 = P + dim (xbar - t)^2, xbar the mean coordinate and
+ * P = sum_d (x_d - xbar)^2 computed once per point.  BUFFER: f values supplied
  * by the caller (device double, indexed by global point index - fbase).  */
 typedef struct {
     int32_t kind;
