@@ -7,7 +7,7 @@
 // J*(dv_y/n_h) (pyx:1749) -> integrand -> w f.  Per hcube: the reference's
 // two-pass sums with the EPSILON guard (pyx:2146-2183) as wave-level
 // segmented scans over the tile; hcubes cut by a tile edge are finished by
-// the straddler kernels.  Training data (w f n)^2 (pyx:2173) goes into an
+// the straddler kernel from per-tile part sums.  Training data (w f n)^2 (pyx:2173) goes into an
 // LDS-privatised histogram (FP64 sums, packed u16 counts).
 #pragma once
 #include "vgx_iter.h"
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     const uint32_t te = min(tb + chunk, ntiles);
     const uint32_t nrounds = (chunk + nwave - 1) / nwave;
 
-    DD am{0.0, 0.0}, av{0.0, 0.0};
+    DD am{0.0, 0.0}, av{0.0, 0.0}, as{0.0, 0.0};
     double wmax = 0.0;
     int nan_seen = 0;
     const double eps = kEpsilon;
@@ -361,7 +361,8 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
             }
             const bool sA = offs[0] < plo || offs[1] > phi;
             const bool sB = offs[kL + 1] > phi;
-            // ---- pass 2: guarded deviations (pyx:2153-2169)
+            // ---- pass 2: guarded deviations (pyx:2153-2169).  A cut hcube's
+            // part uses the part's own mean; the straddler kernel combines.
             double qv[kK], dv[kK];
 #pragma unroll
             for (int j = 0; j < kK; ++j) {
@@ -370,11 +371,8 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
                 const int k = key[j];
                 if (k == kKeySentinel) continue;
                 const bool strad = (k == 0 && sA) || (k == kL && sB);
-                if (strad) {
-                    io.wf_stage[pl0 + j] = wf[j];
-                    continue;
-                }
-                const double m = S[j] / (double)nn[j];
+                const uint32_t cnt = strad ? min(offs[k + 1], phi) - max(offs[k], plo) : nn[j];
+                const double m = S[j] / (double)cnt;
                 double dd = wf[j] - m;
                 const double thr = eps * fabs(m);
                 if (fabs(dd) < thr) {
@@ -386,8 +384,8 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
                 }
             }
             seg_prefix2(key, qv, dv, lane);
-            // Segment ends, one element at a time (a single copy of pow):
-            // element j sits at index 0 of the rotating windows.
+            // Segment ends, one element at a time: element j sits at index 0
+            // of the rotating windows.
 #pragma unroll 1
             for (int j = 0; j < kK; ++j) {
                 const int k = key[0];
@@ -400,22 +398,32 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
                         const double varc = (n * Q - D * D) / (n - 1.);
                         dd_add(am, S[0] + D);
                         dd_add(av, varc);
-                        // |var_h|; sigf_pow_kernel raises it to beta/2 (pyx:2185)
-                        if (g.adaptive_strat) io.sigf[h] = fabs(varc);
+                        // sigf_h = |var_h|^(beta/2) (pyx:2183-2186); in exact
+                        // mode a separate FP64 pass raises it to the power
+                        if (g.adaptive_strat) {
+                            if (g.fast_pow) {
+                                const double sv = pow_fast(fabs(varc), g.beta_half);
+                                io.sigf[h] = sv;
+                                dd_add(as, sv);
+                            } else {
+                                io.sigf[h] = fabs(varc);
+                            }
+                        }
                         if (g.train_errors) io.errv[h] = fabs(varc);
                     }
-                    // slot A: end of the first segment; slot B: last valid point
-                    if (k == 0) {
-                        Slot sl{};
-                        sl.h = sA ? (int32_t)h : -1;
-                        sl.S = S[0];
-                        io.slots[2 * (size_t)t] = sl;
-                    }
-                    if (pl0 + j == phi - 1) {
-                        Slot sl{};
-                        sl.h = (kL != 0 && sB) ? (int32_t)h : -1;
-                        sl.S = S[0];
-                        io.slots[2 * (size_t)t + 1] = sl;
+                    // cut hcubes: the tile's first segment and its last one
+                    const bool slA = k == 0, slB = pl0 + j == phi - 1;
+                    if (slA || slB) {
+                        Slot sl{-1, 0u, 0.0, 0.0, 0.0};
+                        if (strad && (slA ? sA : kL != 0)) {
+                            sl.h = (int32_t)h;
+                            sl.n = min(offs[k + 1], phi) - max(offs[k], plo);
+                            sl.S = S[0];
+                            sl.Q = qv[0];
+                            sl.D = dv[0];
+                        }
+                        io.slots[2 * (size_t)t + (slA ? 0 : 1)] = sl;
+                        if (slA && slB) io.slots[2 * (size_t)t + 1].h = -1;
                     }
                 }
 #pragma unroll
@@ -459,12 +467,13 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     }
     am = wave_sum_dd(am);
     av = wave_sum_dd(av);
+    as = wave_sum_dd(as);
     wmax = wave_max(wmax);
     const int anynan = __any(nan_seen);
     if (lane == 0) {
         Lred[wave * 6 + 0] = am.hi; Lred[wave * 6 + 1] = am.lo;
         Lred[wave * 6 + 2] = av.hi; Lred[wave * 6 + 3] = av.lo;
-        Lred[wave * 6 + 4] = 0.0; Lred[wave * 6 + 5] = 0.0;
+        Lred[wave * 6 + 4] = as.hi; Lred[wave * 6 + 5] = as.lo;
         if (wmax > 0.0) atomicMax((unsigned long long*)&sc->wfn_max_bits,
                                   (unsigned long long)__double_as_longlong(wmax));
         if (anynan) atomicOr(&sc->status, (int32_t)VGX_ENAN);

@@ -2,14 +2,14 @@
 // src/vegas/_vegas.pyx:2086-2227) and its non-fused kernels.
 //
 // Per iteration, all on one stream, no host synchronisation:
-//   alloc, scan, offsets  n_h by the damped-variance rule (pyx:1657-1706),
+//   alloc, offsets        n_h by the damped-variance rule (pyx:1657-1706),
 //                         exclusive scan to point offsets, tile -> hcube
-//   fused                 sample, map, integrand, per-hcube sums, histogram
-//   strad0, 1, 2          hcubes cut by tile edges: totals, guarded
-//                         deviations per part, then per-hcube results
-//   batch_reduce          iteration sums (compensated, fixed order)
-//   sigf_pow              sigf = |var|^(beta/2) and its sum (pyx:2183-2186)
-//   finalize              estimate, sum_sigf (pyx:2209-2215), running average
+//   fused                 sample, map, integrand, per-hcube sums and
+//                         sigf = |var|^(beta/2) (pyx:2183-2186), histogram
+//   strad                 the same per-hcube sums for hcubes cut by tile edges
+//   sigf_pow              exact mode only: sigf by FP64 pow, and its sum
+//   finalize              iteration sums (compensated, fixed order), estimate,
+//                         sum_sigf (pyx:2209-2215), running average
 //                         (pyx:2392-2410); resets the per-iteration scalars
 //   adapt                 AdaptiveMap.adapt(alpha) in place (pyx:2216-2217);
 //                         clears the training data for the next iteration
@@ -105,80 +105,86 @@ __global__ __launch_bounds__(kAllocThreads) void alloc_kernel(IterGeom g, double
     }
 }
 
-// Exclusive scan of the per-workgroup totals (one workgroup), total points,
-// neval_hcube_range and the tile count of the whole-iteration batch.
-__global__ __launch_bounds__(1024) void scan_kernel(IterGeom g, uint64_t* stat, int nblk, Scalars* sc,
-                                                    uint64_t capacity) {
-    __shared__ uint64_t part[1024];
-    __shared__ int32_t pmin[1024], pmax[1024];
+// Point offsets off[h]: exclusive scan of n_h.  Each workgroup takes its
+// base from the totals of the workgroups before it (alloc_kernel's stat),
+// scans its 2048 hcubes through LDS and writes them coalesced, with
+// tile -> first hcube.  The last workgroup also writes the iteration's point
+// count, neval_hcube_range and the tile count (or flags an overflow of the
+// sample buffer).  LDS items are padded (item j at j + j/8) so the per-thread
+// runs of 8 are read without bank conflicts.
+__device__ inline int lds_pad(int j) { return j + (j >> 3); }
+
+__global__ __launch_bounds__(kAllocThreads) void offsets_kernel(IterGeom g, const int32_t* nh,
+                                                                const uint64_t* stat, int nblk, uint32_t* off,
+                                                                int32_t* tile_h, Scalars* sc,
+                                                                uint64_t capacity) {
+    __shared__ uint32_t ln[kAllocPerBlock + kAllocPerBlock / 8];
+    __shared__ uint64_t wred[2 * (kAllocThreads / 64)];
+    __shared__ int32_t wmm[2 * (kAllocThreads / 64)];
+    __shared__ uint32_t wsum[kAllocThreads / 64];
     if (sc->status) return;
-    uint64_t carry = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int nw = kAllocThreads / 64;
+    // totals before this workgroup and over all workgroups
+    uint64_t pre = 0, tot = 0;
     int32_t mn = g.nmin, mx = g.nmin;
-    for (int base = 0; base < nblk; base += 1024) {
-        const int i = base + threadIdx.x;
-        const uint64_t v = i < nblk ? stat[3 * i] : 0;
-        if (i < nblk) {
+    const bool last = blockIdx.x == gridDim.x - 1;
+    for (int i = threadIdx.x; i < nblk; i += kAllocThreads) {
+        const uint64_t v = stat[3 * i];
+        tot += v;
+        if (i < (int)blockIdx.x) pre += v;
+        if (last) {
             mn = min(mn, (int32_t)stat[3 * i + 1]);
             mx = max(mx, (int32_t)stat[3 * i + 2]);
         }
-        part[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            uint64_t add = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
-            __syncthreads();
-            part[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (i < nblk) stat[3 * i] = carry + part[threadIdx.x] - v;
-        const uint64_t ct = part[1023];
-        __syncthreads();
-        carry += ct;
     }
-    pmin[threadIdx.x] = mn;
-    pmax[threadIdx.x] = mx;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        pre += __shfl_xor(pre, o, kWave);
+        tot += __shfl_xor(tot, o, kWave);
+        mn = min(mn, __shfl_xor(mn, o, kWave));
+        mx = max(mx, __shfl_xor(mx, o, kWave));
+    }
+    if (lane == 0) {
+        wred[2 * w] = pre;
+        wred[2 * w + 1] = tot;
+        wmm[2 * w] = mn;
+        wmm[2 * w + 1] = mx;
+    }
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-        if (threadIdx.x < (unsigned)o) {
-            pmin[threadIdx.x] = min(pmin[threadIdx.x], pmin[threadIdx.x + o]);
-            pmax[threadIdx.x] = max(pmax[threadIdx.x], pmax[threadIdx.x + o]);
-        }
-        __syncthreads();
+    pre = 0;
+    tot = 0;
+    for (int k = 0; k < nw; ++k) {
+        pre += wred[2 * k];
+        tot += wred[2 * k + 1];
+        mn = min(mn, wmm[2 * k]);
+        mx = max(mx, wmm[2 * k + 1]);
     }
-    if (threadIdx.x == 0) {
-        const int ok = carry <= capacity && carry < 0xFFFFFFFFull;
+    const bool ok = tot <= capacity && tot < 0xFFFFFFFFull;
+    if (last && threadIdx.x == 0) {
         sc->overflow = ok ? 0u : 1u;
-        sc->total = ok ? (uint32_t)carry : 0u;
-        sc->nh_min = pmin[0];
-        sc->nh_max = pmax[0];
+        sc->total = ok ? (uint32_t)tot : 0u;
+        sc->nh_min = mn;
+        sc->nh_max = mx;
         sc->b_p0 = 0;
         sc->b_p1 = sc->total;
         sc->b_h0 = 0;
         sc->b_h1 = (uint32_t)g.nhcube;
         sc->b_ntiles = (sc->total + kT - 1) / kT;
     }
-}
-
-// Point offsets off[h] (exclusive scan of n_h, written coalesced through LDS)
-// and tile -> first hcube.
-__global__ __launch_bounds__(kAllocThreads) void offsets_kernel(IterGeom g, const int32_t* nh,
-                                                                const uint64_t* stat, uint32_t* off,
-                                                                int32_t* tile_h, const Scalars* sc) {
-    __shared__ uint32_t ln[kAllocPerBlock];
-    __shared__ uint32_t wsum[kAllocThreads / 64];
-    if (sc->status || sc->overflow) return;
+    if (!ok) return;
     const int64_t base = (int64_t)blockIdx.x * kAllocPerBlock;
     for (int i = threadIdx.x; i < kAllocPerBlock; i += kAllocThreads) {
         const int64_t h = base + i;
-        ln[i] = h < g.nhcube ? (uint32_t)nh[h] : 0u;
+        ln[lds_pad(i)] = h < g.nhcube ? (uint32_t)nh[h] : 0u;
     }
     __syncthreads();
     uint32_t loc[kAllocItems];
     uint32_t run = 0;
     for (int i = 0; i < kAllocItems; ++i) {
         loc[i] = run;
-        run += ln[threadIdx.x * kAllocItems + i];
+        run += ln[lds_pad(threadIdx.x * kAllocItems + i)];
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t incl = run;
     for (int o = 1; o < 64; o <<= 1) {
         uint32_t t = __shfl_up(incl, o, kWave);
@@ -188,11 +194,11 @@ __global__ __launch_bounds__(kAllocThreads) void offsets_kernel(IterGeom g, cons
     __syncthreads();
     uint32_t wbase = 0;
     for (int k = 0; k < w; ++k) wbase += wsum[k];
-    const uint32_t tbase = (uint32_t)stat[3 * blockIdx.x] + wbase + incl - run;
+    const uint32_t tbase = (uint32_t)pre + wbase + incl - run;
     for (int i = 0; i < kAllocItems; ++i) {
         const int64_t h = base + threadIdx.x * kAllocItems + i;
         const uint32_t lo = tbase + loc[i];
-        const uint32_t n = ln[threadIdx.x * kAllocItems + i];
+        const uint32_t n = ln[lds_pad(threadIdx.x * kAllocItems + i)];
         if (h < g.nhcube) {
             const uint32_t hi = lo + n;
             for (uint32_t t = (lo + kT - 1) / kT; t < (hi + kT - 1) / kT; ++t) tile_h[t] = (int32_t)h;
@@ -200,13 +206,13 @@ __global__ __launch_bounds__(kAllocThreads) void offsets_kernel(IterGeom g, cons
         loc[i] = lo;
     }
     __syncthreads();
-    for (int i = 0; i < kAllocItems; ++i) ln[threadIdx.x * kAllocItems + i] = loc[i];
+    for (int i = 0; i < kAllocItems; ++i) ln[lds_pad(threadIdx.x * kAllocItems + i)] = loc[i];
     __syncthreads();
     for (int i = threadIdx.x; i < kAllocPerBlock; i += kAllocThreads) {
         const int64_t h = base + i;
-        if (h < g.nhcube) off[h] = ln[i];
+        if (h < g.nhcube) off[h] = ln[lds_pad(i)];
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[g.nhcube] = sc->total;
+    if (last && threadIdx.x == 0) off[g.nhcube] = (uint32_t)tot;
 }
 
 // Staged batches: tiles of points [p0, p1) covering hcubes [h0, h1).
@@ -226,132 +232,123 @@ __global__ void batch_tiles_kernel(const uint32_t* off, int32_t* tile_h, Scalars
 }
 
 // ---------------------------------------------------------- straddlers ----
-// One thread per tile slot.  A slot (tile t, part 0 = first segment, part 1 =
-// last segment) holds the tile's share of an hcube cut by a tile edge.
+// Hcubes cut by a tile edge.  For each tile part the fused kernel left
+// (n_u, S_u, Q_u, D_u): count, sum and guarded two-pass sums about the part
+// mean m_u = S_u/n_u.  The slot of the tile where the hcube starts owns it
+// and combines the parts exactly: with m = S/n and d_u = m_u - m,
+//   sum (w f - m)^2 = sum_u [Q_u + 2 d_u D_u + n_u d_u^2],
+//   sum (w f - m)   = sum_u [D_u + n_u d_u],
+// where a shift below the reference's guard EPSILON |m| counts as zero as a
+// deviation does (pyx:2157-2163).  Then the per-hcube results
+// (pyx:2177-2193).  One lane per slot; owners spanning more than kLaneParts
+// tiles are combined by the whole wave.  For u > first tile the part is
+// always the tile's first segment (slot 2u).
 struct StradArgs {
     double* errv;
     const uint32_t* off;
-    Slot* slots;
-    const double* wf_stage;
+    const Slot* slots;
     double* sigf;
-    double* strad_part;
+    double* strad_part;   // [gridDim.x][6]: DD mean, var, sigf sums
     Scalars* sc;
 };
 
-__device__ inline const Slot& slot_of(const Slot* slots, uint32_t t, int32_t h) {
-    return slots[2 * (size_t)t].h == h ? slots[2 * (size_t)t] : slots[2 * (size_t)t + 1];
+__device__ inline const Slot& part_of(const StradArgs& a, uint32_t s, uint32_t tf, uint32_t u) {
+    return u == tf ? a.slots[s] : a.slots[2 * (size_t)u];
 }
 
-// One wave per slot; the slot's hcube h, its point range and tile range.
-struct SlotCtx {
-    int32_t h;
-    uint32_t t, hlo, hhi, tf, tl;
-    bool owner;
-};
-
-__device__ inline bool load_slot(const StradArgs& a, uint32_t s, SlotCtx& c) {
-    const Scalars* sc = a.sc;
-    const uint32_t ntiles = sc->b_ntiles, p0 = sc->b_p0;
-    if (s >= 2 * ntiles) return false;
-    c.h = a.slots[s].h;
-    if (c.h < 0) return false;
-    c.t = s >> 1;
-    c.hlo = a.off[c.h];
-    c.hhi = a.off[c.h + 1];
-    c.tf = (c.hlo - p0) / kT;
-    c.tl = (c.hhi - 1 - p0) / kT;
-    c.owner = c.hlo >= p0 + c.t * kT;
-    return true;
+__device__ inline void part_dev(const Slot& p, double mu, double thr, double& Q, double& D) {
+    double d = p.S / (double)p.n - mu;
+    if (fabs(d) < thr) d = 0.0;
+    const double nd = (double)p.n * d;
+    Q += p.Q + 2.0 * d * p.D + nd * d;
+    D += p.D + nd;
 }
 
-// Sum over the hcube's tiles of one slot field, lanes in parallel then a
-// fixed butterfly: the same order for every caller.
-__device__ inline double parts_sum(const Slot* slots, const SlotCtx& c, int field) {
-    double v = 0.0;
-    for (uint32_t u = c.tf + lane_id(); u <= c.tl; u += kWave) {
-        const Slot& o = slot_of(slots, u, c.h);
-        v += field == 0 ? o.S : field == 1 ? o.Q : o.D;
-    }
-    return wave_sum_fixed(v);
-}
-
-// Owner slots: S of the whole cut hcube.
-__global__ __launch_bounds__(256) void strad0_kernel(StradArgs a) {
-    if (a.sc->status || a.sc->overflow) return;
-    const uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6);
-    SlotCtx c;
-    if (!load_slot(a, s, c) || !c.owner) return;
-    const double S = parts_sum(a.slots, c, 0);
-    if (lane_id() == 0) a.slots[s].Stot = S;
-}
-
-// Every slot: guarded deviations of its part from the hcube mean
-// (pyx:2153-2169).
-__global__ __launch_bounds__(256) void strad1_kernel(StradArgs a) {
-    const Scalars* sc = a.sc;
-    if (sc->status || sc->overflow) return;
-    const uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6);
-    SlotCtx c;
-    if (!load_slot(a, s, c)) return;
-    const uint32_t p0 = sc->b_p0, p1 = sc->b_p1;
-    const double S = slot_of(a.slots, c.tf, c.h).Stot;
-    const double n = (double)(c.hhi - c.hlo);
-    const double m = S / n;
-    const double thr = kEpsilon * fabs(m);
-    const uint32_t plo = p0 + c.t * kT;
-    const uint32_t lo = max(c.hlo, plo), hi = min(c.hhi, min(plo + (uint32_t)kT, p1));
-    double q = 0.0, d = 0.0;
-    for (uint32_t p = lo + lane_id(); p < hi; p += kWave) {
-        const double dd = a.wf_stage[p] - m;
-        if (fabs(dd) < thr) {
-            q += thr * thr;
+// Per-hcube results of a cut hcube (pyx:2177-2186).
+__device__ inline void strad_close(const IterGeom& g, const StradArgs& a, int32_t h, double n, double S, double Q,
+                                   double D, DD& m, DD& v, DD& ss) {
+    const double varc = (n * Q - D * D) / (n - 1.);
+    dd_add(m, S + D);
+    dd_add(v, varc);
+    if (g.adaptive_strat) {
+        if (g.fast_pow) {
+            const double sv = pow_fast(fabs(varc), g.beta_half);
+            a.sigf[h] = sv;
+            dd_add(ss, sv);
         } else {
-            q += dd * dd;
-            d += dd;
+            a.sigf[h] = fabs(varc);
         }
     }
-    q = wave_sum_fixed(q);
-    d = wave_sum_fixed(d);
-    if (lane_id() == 0) {
-        a.slots[s].Q = q;
-        a.slots[s].D = d;
-    }
+    if (g.train_errors) a.errv[h] = fabs(varc);
 }
 
-// Owner slots combine the parts: mean += S + D, var += (nQ - D^2)/(n-1),
-// |var| for sigf (pyx:2177-2185); per-workgroup compensated partials.
-__global__ __launch_bounds__(1024) void strad2_kernel(IterGeom g, StradArgs a) {
-    __shared__ double red[16 * 4];
-    const int w = threadIdx.x >> 6;
-    DD m{0, 0}, v{0, 0};
-    if (!(a.sc->status || a.sc->overflow)) {
-        const uint32_t s = blockIdx.x * 16 + w;
-        SlotCtx c;
-        if (load_slot(a, s, c) && c.owner) {
-            const double Q = parts_sum(a.slots, c, 1);
-            const double D = parts_sum(a.slots, c, 2);
-            if (lane_id() == 0) {
-                const double n = (double)(c.hhi - c.hlo);
-                const double varc = (n * Q - D * D) / (n - 1.);
-                dd_add(m, a.slots[s].Stot + D);
-                dd_add(v, varc);
-                if (g.adaptive_strat) a.sigf[c.h] = fabs(varc);
-                if (g.train_errors) a.errv[c.h] = fabs(varc);
+__global__ __launch_bounds__(kStradThreads) void strad_kernel(IterGeom g, StradArgs a) {
+    __shared__ double acc_l[(kStradThreads / 64) * 6];
+    const Scalars* sc = a.sc;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    DD m{0, 0}, v{0, 0}, ss{0, 0};
+    if (!(sc->status || sc->overflow)) {
+        const uint32_t ntiles = sc->b_ntiles, p0 = sc->b_p0;
+        const uint32_t s = blockIdx.x * kStradThreads + threadIdx.x;
+        int32_t h = -1;
+        uint32_t tf = 0, tl = 0, nh = 0;
+        bool own = false;
+        if (s < 2 * ntiles) {
+            h = a.slots[s].h;
+            if (h >= 0) {
+                const uint32_t lo = a.off[h], hi = a.off[h + 1];
+                tf = s >> 1;
+                own = lo >= p0 + tf * kT;  // the hcube starts in this tile
+                tl = (hi - 1 - p0) / kT;
+                nh = hi - lo;
             }
         }
+        const bool in_lane = own && tl - tf < (uint32_t)kLaneParts;
+        if (in_lane) {
+            double S = 0.0;
+            for (uint32_t u = tf; u <= tl; ++u) S += part_of(a, s, tf, u).S;
+            const double n = (double)nh, mu = S / n, thr = kEpsilon * fabs(mu);
+            double Q = 0.0, D = 0.0;
+            for (uint32_t u = tf; u <= tl; ++u) part_dev(part_of(a, s, tf, u), mu, thr, Q, D);
+            strad_close(g, a, h, n, S, Q, D, m, v, ss);
+        }
+        // owners of many parts: one at a time across the wave, fixed order
+        uint64_t wide = __ballot(own && !in_lane);
+        while (wide) {
+            const int src = __ffsll((unsigned long long)wide) - 1;
+            wide &= wide - 1;
+            const int32_t hh = __shfl(h, src, kWave);
+            const uint32_t ss_ = __shfl(s, src, kWave), tf_ = __shfl(tf, src, kWave);
+            const uint32_t tl_ = __shfl(tl, src, kWave), nh_ = __shfl(nh, src, kWave);
+            double S = 0.0;
+            for (uint32_t u = tf_ + lane; u <= tl_; u += kWave) S += part_of(a, ss_, tf_, u).S;
+            S = wave_sum_fixed(S);
+            const double n = (double)nh_, mu = S / n, thr = kEpsilon * fabs(mu);
+            double Q = 0.0, D = 0.0;
+            for (uint32_t u = tf_ + lane; u <= tl_; u += kWave) part_dev(part_of(a, ss_, tf_, u), mu, thr, Q, D);
+            Q = wave_sum_fixed(Q);
+            D = wave_sum_fixed(D);
+            if (lane == src) strad_close(g, a, hh, n, S, Q, D, m, v, ss);
+        }
     }
-    if (lane_id() == 0) {
-        red[4 * w] = m.hi; red[4 * w + 1] = m.lo; red[4 * w + 2] = v.hi; red[4 * w + 3] = v.lo;
+    m = wave_sum_dd(m);
+    v = wave_sum_dd(v);
+    ss = wave_sum_dd(ss);
+    if (lane == 0) {
+        acc_l[6 * w + 0] = m.hi; acc_l[6 * w + 1] = m.lo;
+        acc_l[6 * w + 2] = v.hi; acc_l[6 * w + 3] = v.lo;
+        acc_l[6 * w + 4] = ss.hi; acc_l[6 * w + 5] = ss.lo;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        DD M{0, 0}, V{0, 0};
-        for (int k = 0; k < 16; ++k) {
-            M = dd_sum(M, DD{red[4 * k], red[4 * k + 1]});
-            V = dd_sum(V, DD{red[4 * k + 2], red[4 * k + 3]});
+        DD M{0, 0}, V{0, 0}, T{0, 0};
+        for (int k = 0; k < kStradThreads / 64; ++k) {
+            M = dd_sum(M, DD{acc_l[6 * k], acc_l[6 * k + 1]});
+            V = dd_sum(V, DD{acc_l[6 * k + 2], acc_l[6 * k + 3]});
+            T = dd_sum(T, DD{acc_l[6 * k + 4], acc_l[6 * k + 5]});
         }
         double* o = a.strad_part + 6 * (size_t)blockIdx.x;
-        o[0] = M.hi; o[1] = M.lo; o[2] = V.hi; o[3] = V.lo; o[4] = 0.0; o[5] = 0.0;
+        o[0] = M.hi; o[1] = M.lo; o[2] = V.hi; o[3] = V.lo; o[4] = T.hi; o[5] = T.lo;
     }
 }
 
@@ -371,31 +368,41 @@ __device__ void tree_dd3(double* red) {
     }
 }
 
-// Adds the batch's partial sums (fused workgroups, then straddler
-// workgroups, fixed order) into the iteration accumulators.
-__global__ __launch_bounds__(512) void batch_reduce_kernel(const double* blk_part, int nblk,
-                                                           const double* strad_part, int nstrad,
-                                                           Scalars* sc) {
-    __shared__ double red[512 * 6];
-    if (sc->status || sc->overflow) return;
-    DD m{0, 0}, v{0, 0};
+// Adds partial sums (fused workgroups, then straddler workgroups, fixed
+// order) into the iteration accumulators sc->acc (DD mean, var, sigf sum).
+// One workgroup of kReduceThreads.
+constexpr int kReduceThreads = 256;
+__device__ void reduce_parts(const double* blk_part, int nblk, const double* strad_part, int nstrad,
+                             Scalars* sc, double* red) {
+    DD m{0, 0}, v{0, 0}, t{0, 0};
     const int tot = nblk + nstrad;
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         const double* p = i < nblk ? blk_part + 6 * (size_t)i : strad_part + 6 * (size_t)(i - nblk);
         m = dd_sum(m, DD{p[0], p[1]});
         v = dd_sum(v, DD{p[2], p[3]});
+        t = dd_sum(t, DD{p[4], p[5]});
     }
     red[6 * threadIdx.x + 0] = m.hi; red[6 * threadIdx.x + 1] = m.lo;
     red[6 * threadIdx.x + 2] = v.hi; red[6 * threadIdx.x + 3] = v.lo;
-    red[6 * threadIdx.x + 4] = 0.0; red[6 * threadIdx.x + 5] = 0.0;
+    red[6 * threadIdx.x + 4] = t.hi; red[6 * threadIdx.x + 5] = t.lo;
     __syncthreads();
     tree_dd3(red);
     if (threadIdx.x == 0) {
-        const DD M = dd_sum(DD{sc->acc[0], sc->acc[1]}, DD{red[0], red[1]});
-        const DD V = dd_sum(DD{sc->acc[2], sc->acc[3]}, DD{red[2], red[3]});
-        sc->acc[0] = M.hi; sc->acc[1] = M.lo;
-        sc->acc[2] = V.hi; sc->acc[3] = V.lo;
+        for (int q = 0; q < 3; ++q) {
+            const DD r = dd_sum(DD{sc->acc[2 * q], sc->acc[2 * q + 1]}, DD{red[2 * q], red[2 * q + 1]});
+            sc->acc[2 * q] = r.hi;
+            sc->acc[2 * q + 1] = r.lo;
+        }
     }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kReduceThreads) void batch_reduce_kernel(const double* blk_part, int nblk,
+                                                                      const double* strad_part, int nstrad,
+                                                                      Scalars* sc) {
+    __shared__ double red[kReduceThreads * 6];
+    if (sc->status || sc->overflow) return;
+    reduce_parts(blk_part, nblk, strad_part, nstrad, sc, red);
 }
 
 // sigf_h = |var_h|^(beta/2) (pyx:2185) and per-workgroup sums, after every
@@ -428,14 +435,19 @@ __global__ __launch_bounds__(256) void sigf_pow_kernel(IterGeom g, double* sigf,
     }
 }
 
-// Iteration close (one workgroup): sum of sigf from the sigf_pow partials,
-// the estimate, sum_sigf bookkeeping and the FP64 running average; resets the
+// Iteration close (one workgroup): the batch's partial sums when given
+// (single-batch iterations), the sum of sigf (accumulated with the sums, or
+// from the sigf_pow partials in exact mode), the estimate, sum_sigf bookkeeping and the FP64 running average; resets the
 // per-iteration scalars for the next iteration of the run.
-__global__ __launch_bounds__(256) void finalize_kernel(IterGeom g, Scalars* sc, vgx_state st, int has_sigf,
-                                                       const double* sigf_part, int nsigf) {
-    __shared__ double red[256 * 2];
+__global__ __launch_bounds__(kReduceThreads) void finalize_kernel(IterGeom g, Scalars* sc, vgx_state st,
+                                                                  int has_sigf, const double* sigf_part,
+                                                                  int nsigf, const double* blk_part, int nblk,
+                                                                  const double* strad_part, int nstrad) {
+    __shared__ double red[kReduceThreads * 6];
+    if (nblk + nstrad > 0 && !(sc->status || sc->overflow))
+        reduce_parts(blk_part, nblk, strad_part, nstrad, sc, red);
     DD t{0, 0};
-    if (g.adaptive_strat)
+    if (g.adaptive_strat && !g.fast_pow)
         for (int k = threadIdx.x; k < nsigf; k += blockDim.x)
             t = dd_sum(t, DD{sigf_part[2 * k], sigf_part[2 * k + 1]});
     red[2 * threadIdx.x] = t.hi;
@@ -463,7 +475,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(IterGeom g, Scalars* sc, 
     const int it = (int)g.itn;
     const double mean = sc->acc[0] + sc->acc[1];
     const double var = sc->acc[2] + sc->acc[3];
-    const double ssig = red[0] + red[1];
+    const double ssig = g.fast_pow ? sc->acc[4] + sc->acc[5] : red[0] + red[1];
     st.itn_mean[it] = mean;
     st.itn_var[it] = var;
     st.itn_neval[it] = (int64_t)sc->total;
@@ -620,6 +632,7 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
     g->exact_div = (c->flags & VGX_FLAG_EXACT_DIV) ? 1 : 0;
     g->weighted = c->weighted;
     g->adapt = c->adapt;
+    g->fast_pow = g->adaptive_strat && !g->exact_div;
     const int64_t nmin = c->min_neval_hcube;
     const int64_t nmax = c->max_neval_hcube > nmin ? c->max_neval_hcube : nmin;
     const int64_t avg = (int64_t)((double)c->neval / (double)c->nhcube);
@@ -689,7 +702,7 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     R->maxtiles = (uint32_t)((R->cap + kT - 1) / kT) + 1;
     R->nblk_alloc = (int)((c->nhcube + kAllocPerBlock - 1) / kAllocPerBlock);
     R->fused_blocks = ctx->info.cus;
-    R->strad_blocks = (int)((2ull * R->maxtiles + 15) / 16);
+    R->strad_blocks = (int)((2ull * R->maxtiles + kStradThreads - 1) / kStradThreads);
     R->lds = fused_lds(g);
     if (R->lds > ctx->info.max_lds) return VGX_ERANGE;
     if ((rc = ctx->nh.ensure((size_t)c->nhcube * 4))) return rc;
@@ -699,7 +712,6 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     if ((rc = ctx->tile_slots.ensure((size_t)R->maxtiles * 2 * sizeof(Slot)))) return rc;
     if ((rc = ctx->blk_part.ensure((size_t)R->fused_blocks * 6 * 8))) return rc;
     if ((rc = ctx->strad_part.ensure((size_t)R->strad_blocks * 6 * 8))) return rc;
-    if ((rc = ctx->wf_stage.ensure((size_t)R->cap * 8 + 64))) return rc;
     R->errv = nullptr;
     if (g.train_errors) {
         if ((rc = ctx->errv.ensure((size_t)c->nhcube * 8))) return rc;
@@ -722,7 +734,6 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     io.sigf = st->sigf;
     io.slots = (Slot*)ctx->tile_slots.ptr;
     io.blk_part = (double*)ctx->blk_part.ptr;
-    io.wf_stage = (double*)ctx->wf_stage.ptr;
     io.sum_f = st->sum_f;
     io.n_f = st->n_f;
     io.errv = R->errv;
@@ -731,7 +742,6 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     sa.errv = R->errv;
     sa.off = io.off;
     sa.slots = io.slots;
-    sa.wf_stage = io.wf_stage;
     sa.sigf = st->sigf;
     sa.strad_part = (double*)ctx->strad_part.ptr;
     sa.sc = R->sc;
@@ -761,15 +771,17 @@ static int alloc_kernels(vgx_ctx* ctx, Run* R, vgx_state* st, hipStream_t s) {
     const IterGeom& g = R->g;
     hipLaunchKernelGGL(alloc_kernel, dim3(R->nblk_alloc), dim3(kAllocThreads), 0, s, g, st->sigf,
                        (const double*)st->sum_sigf, R->nh, R->stat, (const Scalars*)R->sc);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, g, R->stat, R->nblk_alloc, R->sc, R->cap);
     hipLaunchKernelGGL(offsets_kernel, dim3(R->nblk_alloc), dim3(kAllocThreads), 0, s, g,
-                       (const int32_t*)R->nh, (const uint64_t*)R->stat, (uint32_t*)ctx->off.ptr,
-                       (int32_t*)ctx->tile_h.ptr, (const Scalars*)R->sc);
+                       (const int32_t*)R->nh, (const uint64_t*)R->stat, R->nblk_alloc,
+                       (uint32_t*)ctx->off.ptr, (int32_t*)ctx->tile_h.ptr, R->sc, R->cap);
     return check_launch();
 }
 
+// Fused sampler + straddlers for the current batch.  reduce = 1 (staged
+// batches) adds the batch's partials into the iteration sums at once;
+// otherwise finalize_kernel reads them.
 template <class Fcn>
-static int accumulate_kernels(vgx_ctx* ctx, Run* R, const Fcn& fcn, hipStream_t s, int timed) {
+static int accumulate_kernels(vgx_ctx* ctx, Run* R, const Fcn& fcn, hipStream_t s, int timed, int reduce) {
     const IterGeom& g = R->g;
     const int rec = timed && ctx->timing_on && ctx->ntimed < vgx_ctx::kMaxTimed;
     if (rec) VGX_HIP(hipEventRecord(ctx->ev[2 * ctx->ntimed], s));
@@ -779,19 +791,21 @@ static int accumulate_kernels(vgx_ctx* ctx, Run* R, const Fcn& fcn, hipStream_t 
         VGX_HIP(hipEventRecord(ctx->ev[2 * ctx->ntimed + 1], s));
         ctx->ntimed++;
     }
-    const unsigned nslot4 = (unsigned)((2ull * R->maxtiles + 3) / 4);
-    hipLaunchKernelGGL(strad0_kernel, dim3(nslot4), dim3(256), 0, s, R->sa);
-    hipLaunchKernelGGL(strad1_kernel, dim3(nslot4), dim3(256), 0, s, R->sa);
-    hipLaunchKernelGGL(strad2_kernel, dim3(R->strad_blocks), dim3(1024), 0, s, g, R->sa);
-    hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(512), 0, s, (const double*)R->io.blk_part,
-                       R->fused_blocks, (const double*)R->sa.strad_part, R->strad_blocks, R->sc);
+    hipLaunchKernelGGL(strad_kernel, dim3(R->strad_blocks), dim3(kStradThreads), 0, s, g, R->sa);
+    if (reduce)
+        hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(kReduceThreads), 0, s,
+                           (const double*)R->io.blk_part, R->fused_blocks, (const double*)R->sa.strad_part,
+                           R->strad_blocks, R->sc);
     return check_launch();
 }
 
-static int iter_end_kernels(vgx_ctx* ctx, Run* R, const vgx_config* c, vgx_state* st, hipStream_t s) {
+// Iteration close.  parts = 1 when the iteration was one batch whose
+// partials are still unreduced (vgx_integrate).
+static int iter_end_kernels(vgx_ctx* ctx, Run* R, const vgx_config* c, vgx_state* st, hipStream_t s,
+                            int parts) {
     const IterGeom& g = R->g;
     const int has_sigf = c->beta >= 0 && c->nhcube > 1 && !c->adapt_to_errors;
-    if (g.adaptive_strat)
+    if (g.adaptive_strat && !g.fast_pow)
         hipLaunchKernelGGL(sigf_pow_kernel, dim3(256), dim3(256), 0, s, g, st->sigf, R->sigf_part,
                            (const Scalars*)R->sc);
     if (g.train_errors) {
@@ -805,8 +819,10 @@ static int iter_end_kernels(vgx_ctx* ctx, Run* R, const vgx_config* c, vgx_state
                                (const double*)st->grid, (const uint32_t*)ctx->off.ptr, (const double*)R->errv,
                                st->sum_f, st->n_f, (const Scalars*)R->sc);
     }
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(256), 0, s, g, R->sc, *st, has_sigf,
-                       (const double*)R->sigf_part, 256);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kReduceThreads), 0, s, g, R->sc, *st, has_sigf,
+                       (const double*)R->sigf_part, 256, (const double*)R->io.blk_part,
+                       parts ? R->fused_blocks : 0, (const double*)R->sa.strad_part,
+                       parts ? R->strad_blocks : 0);
     int rc = check_launch();
     if (rc) return rc;
     if (c->alpha > 0 && c->adapt) {
@@ -823,15 +839,15 @@ static int run_end(Run* R, const vgx_config* c, vgx_state* st, hipStream_t s) {
 }
 
 static int make_fcn_and_run(vgx_ctx* ctx, Run* R, const vgx_config* c, const vgx_integrand* f,
-                            hipStream_t s, int timed, int convert) {
+                            hipStream_t s, int timed, int convert, int reduce) {
     const bool f32 = c->precision == VGX_PREC_F32;
     if (f->kind == VGX_FCN_BUFFER) {
         if (f32) {
             BufferFcn<true> b{f->fbuf, f->fbase};
-            return accumulate_kernels(ctx, R, b, s, timed);
+            return accumulate_kernels(ctx, R, b, s, timed, reduce);
         }
         BufferFcn<false> b{f->fbuf, f->fbase};
-        return accumulate_kernels(ctx, R, b, s, timed);
+        return accumulate_kernels(ctx, R, b, s, timed, reduce);
     }
     if (f->kind == VGX_FCN_GAUSSMIX || f->kind == VGX_FCN_DIAGRIDGE) {
         const int nc = f->ncomp;
@@ -849,17 +865,17 @@ static int make_fcn_and_run(vgx_ctx* ctx, Run* R, const vgx_config* c, const vgx
             }
             if (f->kind == VGX_FCN_GAUSSMIX) {
                 GaussMix32 fn{nc, A, B, C};
-                return accumulate_kernels(ctx, R, fn, s, timed);
+                return accumulate_kernels(ctx, R, fn, s, timed, reduce);
             }
             DiagRidge32 fn{nc, A, B, C};
-            return accumulate_kernels(ctx, R, fn, s, timed);
+            return accumulate_kernels(ctx, R, fn, s, timed, reduce);
         }
         if (f->kind == VGX_FCN_GAUSSMIX) {
             GaussMix64 fn{nc, f->A, f->B, f->c};
-            return accumulate_kernels(ctx, R, fn, s, timed);
+            return accumulate_kernels(ctx, R, fn, s, timed, reduce);
         }
         DiagRidge64 fn{nc, f->A, f->B, f->c};
-        return accumulate_kernels(ctx, R, fn, s, timed);
+        return accumulate_kernels(ctx, R, fn, s, timed, reduce);
     }
     return VGX_EINVAL;
 }
@@ -881,8 +897,8 @@ extern "C" int vgx_integrate(vgx_ctx* ctx, const vgx_config* c, const vgx_integr
     for (int it = 0; it < c->nitn; ++it) {
         set_iter(&R, c, it);
         if ((rc = alloc_kernels(ctx, &R, st, s))) return rc;
-        if ((rc = make_fcn_and_run(ctx, &R, c, f, s, 1, it == 0))) return rc;
-        if ((rc = iter_end_kernels(ctx, &R, c, st, s))) return rc;
+        if ((rc = make_fcn_and_run(ctx, &R, c, f, s, 1, it == 0, 0))) return rc;
+        if ((rc = iter_end_kernels(ctx, &R, c, st, s, 1))) return rc;
     }
     return run_end(&R, c, st, s);
 }
@@ -953,7 +969,7 @@ extern "C" int vgx_iter_accumulate(vgx_ctx* ctx, const vgx_config* c, vgx_state*
     f.kind = VGX_FCN_BUFFER;
     f.fbuf = fv;
     f.fbase = p0;
-    return make_fcn_and_run(ctx, &R, c, &f, s, 0, 0);
+    return make_fcn_and_run(ctx, &R, c, &f, s, 0, 0, 1);
 }
 
 extern "C" int vgx_iter_end(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, int it, void* stream) {
@@ -964,6 +980,6 @@ extern "C" int vgx_iter_end(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, in
     if (rc) return rc;
     if ((rc = init_axes(&R, s))) return rc;
     set_iter(&R, c, it);
-    if ((rc = iter_end_kernels(ctx, &R, c, st, s))) return rc;
+    if ((rc = iter_end_kernels(ctx, &R, c, st, s, 0))) return rc;
     return run_end(&R, c, st, s);
 }

@@ -59,7 +59,7 @@ struct vgx_ctx {
     // ordered training histogram
     vgx::Scratch hist_cnt, hist_sorted, hist_start;
     // integrator
-    vgx::Scratch nh, off, tile_h, blocksum, tile_slots, blk_part, strad_part, wf_stage, misc, errv;
+    vgx::Scratch nh, off, tile_h, blocksum, tile_slots, blk_part, strad_part, misc, errv;
     // adapt
     vgx::Scratch adapt_tmp;
     static constexpr int kMaxTimed = 256;

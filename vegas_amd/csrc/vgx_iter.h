@@ -12,7 +12,8 @@ constexpr int kAllocThreads = 256;
 constexpr int kAllocItems = 8;
 constexpr int kAllocPerBlock = kAllocThreads * kAllocItems;
 constexpr int kCntFlushRounds = 15; // u16 LDS counts: 15 rounds x 16 tiles x 256 < 65536
-constexpr int kStradBlocks = 256;   // fixed grid of the straddler kernels
+constexpr int kStradThreads = 256;  // straddler kernel: one lane per tile slot
+constexpr int kLaneParts = 8;       // cut hcubes of up to this many tiles: one lane; more: the wave
 constexpr int kKeySentinel = 0x7fff;
 
 // Kernel-argument copy of the iteration geometry (Integrator.set, pyx:1331-1446).
@@ -27,7 +28,8 @@ struct IterGeom {
     int hist_lds, grid_lds;
     int weighted;
     int adapt;
-    int pad_a;
+    int fast_pow;          // sigf = |var|^(beta/2) in the sampler, FP32 log2/exp2
+                           // (else FP64 pow in a separate pass)
     int32_t ninc[VGX_MAXDIM];
     int32_t nstrat[VGX_MAXDIM];
     double inv_nstrat[VGX_MAXDIM];
@@ -60,15 +62,15 @@ struct Scalars {
     double pad1[2];
 };
 
-// Per-tile record of an hcube cut by a tile boundary ("straddler").
+// The part of an hcube cut by a tile edge that lies in one tile: its point
+// count and its two-pass sums about the part's own mean S/n (the straddler
+// kernel combines the parts exactly, Chan-Golub-LeVeque style).
 struct Slot {
     int32_t h;      // hcube, or -1
-    int32_t pad;
-    double S;       // sum of w f over the tile's part of h
-    double Q;       // sum of guarded (w f - mean)^2 over that part
-    double D;       // sum of guarded (w f - mean) over that part
-    double Stot;    // owner's slot only: S over the whole hcube
-    double pad2;
+    uint32_t n;     // points of h in this tile
+    double S;       // sum of w f over them
+    double Q;       // sum of guarded (w f - S/n)^2
+    double D;       // sum of guarded (w f - S/n)
 };
 
 struct AxisC;
@@ -78,9 +80,8 @@ struct FusedPtrs {
     const uint32_t* off;
     const int32_t* tile_h;
     double* sigf;
-    Slot* slots;
+    Slot* slots;        // [2 * tile]: part of the hcube cut by the tile's start / end
     double* blk_part;   // [gridDim.x][6]
-    double* wf_stage;   // [points] w f of straddler points
     double* sum_f;
     double* n_f;
     double* errv;       // adapt_to_errors: |variance| per hcube
