@@ -187,6 +187,62 @@ int vgx_iter_end(vgx_ctx* ctx, const vgx_config* cfg, vgx_state* st, int it,
 int vgx_set_timing(vgx_ctx* ctx, int on);
 int vgx_fused_times(vgx_ctx* ctx, float* ms, int cap);
 
+/* ---------------------------------------------------------------------- */
+/* Batched integrals: n independent vegas+ integrations of single-Gaussian
+ * integrands f_i(x) = A_i exp(-B_i |x - c_i|^2) on [0,1]^dim_i, advanced
+ * together: one allocation, one persistent sampler launch (work items of
+ * whole wave tiles, each inside one integral), one straddler pass, one
+ * close and one adapt per iteration for all of them.  The nitn iterations
+ * of a call are captured once as a hipGraph and replayed by later calls
+ * with the same arguments.  No counterpart in the reference, which runs one
+ * Integrator at a time; each integral follows Integrator.__call__
+ * (pyx:1912-2230) with adapt_to_errors=False and beta > 0, and its samples
+ * are the single-integral Philox stream (same seed, iteration and point
+ * index), so the batch reproduces n separate vgx_integrate runs.          */
+#define VGX_BATCH_MAXDIM 10
+#define VGX_BATCH_MAXINC 500
+#define VGX_BATCH_LD (VGX_BATCH_MAXINC + 1)
+
+typedef struct vgx_batch vgx_batch;
+
+typedef struct {
+    int32_t n;                        /* integrals */
+    const int32_t* dim;               /* host [n], 1..VGX_BATCH_MAXDIM */
+    const int32_t* ninc;              /* host [n][VGX_BATCH_MAXDIM], <= MAXINC */
+    const int32_t* nstrat;            /* host [n][VGX_BATCH_MAXDIM] */
+    const int64_t* neval;             /* host [n] */
+    const int64_t* min_neval_hcube;   /* host [n] */
+    const int64_t* max_neval_hcube;   /* host [n] */
+    const uint64_t* seed;             /* host [n] */
+    double neval_frac;
+    double alpha;
+    double beta;                      /* > 0 */
+    int32_t tiles_per_item;           /* work-item size in wave tiles (1..240; 0 = 128) */
+    int32_t blocks;                   /* sampler workgroups (0 = one per CU) */
+} vgx_batch_config;
+
+/* Caller-owned device state (sizes from vgx_batch_sizes). */
+typedef struct {
+    double* grid;        /* [n][VGX_BATCH_MAXDIM][VGX_BATCH_LD] */
+    double* sigf;        /* [total hcubes], integral i at hcube_base[i] */
+    double* sum_sigf;    /* [n] */
+    double* itn_mean;    /* [nitn][n] */
+    double* itn_var;     /* [nitn][n] */
+    int64_t* itn_neval;  /* [nitn][n] */
+    double* ravg;        /* [n][4]: mean, sdev, chi2, dof of the call */
+    int32_t* status;     /* [1]: 0, VGX_ENAN or VGX_ERANGE */
+} vgx_batch_state;
+
+int vgx_batch_create(vgx_ctx* ctx, const vgx_batch_config* cfg, vgx_batch** out);
+int vgx_batch_destroy(vgx_batch* b);
+/* total hcubes, and hcube_base[n] (host, may be NULL). */
+int vgx_batch_sizes(const vgx_batch* b, int64_t* total_hcubes, int64_t* hcube_base);
+/* Integrand parameters: A, B device double[n]; c device double[n][MAXDIM]. */
+int vgx_batch_integrate(vgx_batch* b, const double* A, const double* B, const double* c,
+                        const vgx_batch_state* st, int nitn, void* stream);
+/* The next call's first iteration number (Philox stream): calls continue it. */
+int vgx_batch_set_iteration(vgx_batch* b, uint32_t iteration);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,153 @@
+"""Batched integrals (vgx_batch_*): each integral of the batch against the
+single-integral path on the same Philox stream, and against its analytic
+value (erf product) at the batched-Gaussian workload's parameters."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def va():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import vegas_amd
+    return vegas_amd
+
+
+def _exact(c, s):
+    out = 1.0
+    for cd in c:
+        out *= 0.5 * (math.erf((1 - cd) / (math.sqrt(2) * s)) + math.erf(cd / (math.sqrt(2) * s)))
+    return out
+
+
+def _problem(n, seed, nlo=1e4, nhi=1e5, slo=0.01, shi=0.1):
+    rng = np.random.default_rng(seed)
+    dims = rng.integers(2, 11, n)
+    cs = [rng.uniform(0.2, 0.8, d) for d in dims]
+    sig = np.exp(rng.uniform(math.log(slo), math.log(shi), n))
+    nev = np.round(np.exp(rng.uniform(math.log(nlo), math.log(nhi), n))).astype(int)
+    return dims, cs, sig, nev
+
+
+def test_batch_first_iteration_equals_single(va):
+    """Iteration 1 of every integral: the batch and separate Integrator runs
+    draw the same points; estimates agree to summation order."""
+    from vegas_amd.batch import BatchIntegrator, gaussian_params
+    dims = [2, 5, 8, 10]
+    cs = [np.full(d, 0.45) for d in dims]
+    sig = [0.1, 0.05, 0.08, 0.1]
+    nev = [20000, 50000, 120000, 80000]
+    A, B = gaussian_params(dims, sig)
+    singles, grids = [], []
+    for i, d in enumerate(dims):
+        integ = va.Integrator([[0, 1]] * d, neval=nev[i], seed=100 + i, precision='f32', maxinc_axis=500)
+        grids.append(integ.map.grid)
+        singles.append(integ)
+    bi = BatchIntegrator(dims, nev, seeds=[100 + i for i in range(4)], grids=grids, tiles_per_item=16)
+    for i, d in enumerate(dims):
+        assert list(bi.nstrat[i, :d]) == list(singles[i].nstrat)
+        assert list(bi.ninc[i, :d]) == list(singles[i].map.ninc)
+    rb = bi(A, B, cs, nitn=1)
+    for i, d in enumerate(dims):
+        f = va.GaussianMixture([A[i]], [B[i]], [cs[i]])
+        r = singles[i](f, nitn=1)
+        assert rb.itn_neval[0, i] == singles[i].last_neval
+        m = r.itn_results[0]
+        assert abs(rb.itn_mean[0, i] - m.mean) <= 1e-11 * abs(m.mean)
+        assert abs(rb.itn_sdev[0, i] - m.sdev) <= 1e-9 * m.sdev
+    # adapted grids after the first iteration: same training data up to
+    # summation order
+    for i, d in enumerate(dims):
+        gb = bi.map_grid(i)
+        gs = singles[i].map.grid
+        for k in range(d):
+            np.testing.assert_allclose(gb[k], gs[k][:len(gb[k])], rtol=0, atol=1e-9)
+
+
+def test_batch_statistics(va):
+    """64 integrals of the batched-Gaussian family with sigma in [0.05, 0.2]
+    and 2e4-1e5 samples (peaks that plain vegas+ resolves in 4 iterations in
+    up to 10 dimensions; the reference-structured oracle resolves all 64):
+    every result within 5 sigma of the erf-product value after a warm-up
+    call."""
+    from vegas_amd.batch import BatchIntegrator, gaussian_params
+    dims, cs, sig, nev = _problem(64, 4096, nlo=2e4, slo=0.05, shi=0.2)
+    A, B = gaussian_params(dims, sig)
+    bi = BatchIntegrator(dims, nev, tiles_per_item=64)
+    bi(A, B, cs, nitn=4)
+    r = bi(A, B, cs, nitn=4)
+    exact = np.array([_exact(c, s) for c, s in zip(cs, sig)])
+    pull = (r.mean - exact) / r.sdev
+    assert np.all(np.isfinite(pull))
+    assert np.max(np.abs(pull)) < 5.0, pull
+    assert np.median(r.sdev / exact) < 5e-3
+    assert np.all(r.itn_neval > 0)
+
+
+def test_batch_hard_cases_match_oracle(va, orc):
+    """The full family (sigma down to 0.01) holds integrals plain vegas+
+    cannot resolve: the batch and the reference-structured oracle agree --
+    both miss a 10-D sigma 0.0115 peak at 1e4 samples (estimate ~0 with a
+    tiny error), both resolve a 2-D and a 4-D one."""
+    from vegas_amd.batch import BatchIntegrator, gaussian_params
+    dims, cs, sig, nev = _problem(64, 4096)
+    pick = [2, 4, 5]
+    dims, cs, sig, nev = dims[pick], [cs[k] for k in pick], sig[pick], nev[pick]
+    A, B = gaussian_params(dims, sig)
+    bi = BatchIntegrator(dims, nev)
+    r = bi(A, B, cs, nitn=4)
+    for i in range(len(pick)):
+        g = orc.GaussMix([A[i]], [B[i]], [cs[i]])
+        o = orc.OracleIntegrator([(0, 1)] * int(dims[i]), neval=int(nev[i]), seed=1, maxinc_axis=500)
+        m, v, _ = o.run(g, 4)
+        om, osd = orc.ravg(m, v)[:2]
+        if om < 1e-6:
+            assert abs(r.mean[i]) < 1e-6
+        else:
+            assert abs(r.mean[i] - om) < 5 * math.hypot(r.sdev[i], osd), (i, r.mean[i], om)
+
+
+def test_batch_replay_and_tile_size(va):
+    """A second call replays the captured graph and continues the Philox
+    stream; the work-item size changes no sample."""
+    from vegas_amd.batch import BatchIntegrator, gaussian_params
+    dims, cs, sig, nev = _problem(24, 7)
+    A, B = gaussian_params(dims, sig)
+    out = {}
+    for tpi in (8, 96):
+        bi = BatchIntegrator(dims, nev, tiles_per_item=tpi)
+        r1 = bi(A, B, cs, nitn=2)
+        r2 = bi(A, B, cs, nitn=2)
+        assert not np.array_equal(r1.itn_mean, r2.itn_mean)
+        out[tpi] = r1
+    a, b = out[8], out[96]
+    np.testing.assert_array_equal(a.itn_neval[0], b.itn_neval[0])
+    np.testing.assert_allclose(a.itn_mean[0], b.itn_mean[0], rtol=1e-12)
+    np.testing.assert_allclose(a.itn_sdev[0], b.itn_sdev[0], rtol=1e-10)
+
+
+def test_batch_autotune(va, tmp_path, monkeypatch):
+    from vegas_amd.batch import BatchIntegrator, gaussian_params
+    monkeypatch.setenv('VEGAS_AMD_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    dims, cs, sig, nev = _problem(32, 11)
+    A, B = gaussian_params(dims, sig)
+    bi = BatchIntegrator(dims, nev)
+    best = bi.autotune(A, B, cs, candidates=(16, 64), nitn=1)
+    assert best in (16, 64) and bi.tiles_per_item == best
+    assert set(bi.tune_times) == {16, 64}
+    bi2 = BatchIntegrator(dims, nev)
+    assert bi2.tiles_per_item == best
+    r = bi(A, B, cs, nitn=3)
+    assert np.all(np.isfinite(r.mean))
+
+
+def test_batch_nan(va):
+    from vegas_amd.batch import BatchIntegrator
+    bi = BatchIntegrator([3, 3], [20000, 20000])
+    with pytest.raises(ValueError):
+        bi([1.0, 1.0], [10.0, float('nan')], [[0.5] * 3, [0.5] * 3], nitn=1)

@@ -1,0 +1,101 @@
+// vgx_adapt.h -- the block-parallel adapt of one axis (AdaptiveMap.adapt,
+// src/vegas/_vegas.pyx:467-594), shared by the single-map and batched kernels.
+#pragma once
+#include "vgx_internal.h"
+
+namespace vgx {
+
+// Deterministic block-wide sum (fixed tree over lanes, then waves).
+// red: at least 33 doubles of LDS.
+__device__ inline double block_sum(double v, double* red) {
+    v = wave_sum_fixed(v);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < nw; ++k) t += red[k];
+        red[32] = t;
+    }
+    __syncthreads();
+    t = red[32];
+    __syncthreads();
+    return t;
+}
+
+// One axis by one workgroup.  av[0..old): the averaged training data
+// sum_f/n_f (or 1 without training), in LDS; g: the old row; ng: the new row
+// (new_ld stride, padding repeats the last node); P: LDS scratch of old
+// doubles.  alpha > 0: 3-tap smoothing, normalisation and the compression
+// (-(1-a)/ln a)^alpha (pyx:535-556); then node i of nn lands in the first old
+// increment j whose prefix P_j >= i F, F = P_old / nn (the increment the
+// reference's sequential walk, pyx:563-587, stops in).
+__device__ inline void adapt_axis_block(double* av, double* P, double* red, const double* g, double* ng,
+                                        int old, int nn, double alpha, int new_ld) {
+    const int T = blockDim.x;
+    if (alpha > 0 && old > 1) {
+        double part = 0.0;
+        for (int i = threadIdx.x; i < old; i += T) {
+            double t;
+            if (i == 0) t = fabs(7. * av[0] + av[1]) / 8.;
+            else if (i == old - 1) t = fabs(7. * av[old - 1] + av[old - 2]) / 8.;
+            else t = fabs(6. * av[i] + av[i - 1] + av[i + 1]) / 8.;
+            P[i] = t;
+            part += t;
+        }
+        const double s = block_sum(part, red);
+        for (int i = threadIdx.x; i < old; i += T) {
+            double v = s > 0 ? P[i] / s + kTiny : kTiny;
+            if (v > 0 && v <= 0.99999999) v = pow(-(1 - v) / log(v), alpha);
+            av[i] = v;
+        }
+        __syncthreads();
+    }
+    // inclusive prefix sums of av (chunked Hillis-Steele; fixed order)
+    double carry = 0.0;
+    for (int base = 0; base < old; base += T) {
+        const int i = base + threadIdx.x;
+        double v = i < old ? av[i] : 0.0;
+        for (int o = 1; o < T; o <<= 1) {
+            double add = 0.0;
+            if (i < old) P[i] = v;
+            __syncthreads();
+            if ((int)threadIdx.x >= o && i < old) add = P[i - o];
+            __syncthreads();
+            v += add;
+        }
+        if (i < old) P[i] = carry + v;
+        __syncthreads();
+        const int last = (base + T < old ? base + T : old) - 1;
+        carry = P[last];
+        __syncthreads();
+    }
+    const double total = P[old - 1];
+    const double F = total / (double)nn;
+    const double glo = g[0], ghi = g[old];
+    for (int i = 1 + threadIdx.x; i < nn; i += T) {
+        const double target = (double)i * F;
+        int lo = 0, hi = old;  // first j with P[j] >= target
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (P[mid] >= target) hi = mid; else lo = mid + 1;
+        }
+        double xv;
+        if (lo >= old) {
+            xv = ghi;
+        } else {
+            const int j = lo;
+            const double acc = P[j] - target;
+            xv = g[j + 1] - (acc / av[j]) * (g[j + 1] - g[j]);
+        }
+        ng[i] = xv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ng[0] = glo;
+        ng[nn] = ghi;
+        for (int i = nn + 1; i < new_ld; ++i) ng[i] = ng[nn];
+    }
+}
+
+}  // namespace vgx

@@ -10,7 +10,7 @@
 //             the first old increment j with prefix P_j >= i*F, exactly the
 //             increment the sequential walk stops in (its running
 //             accumulator equals P_j - i*F); only rounding differs.
-#include "vgx_internal.h"
+#include "vgx_adapt.h"
 
 namespace vgx {
 
@@ -141,23 +141,6 @@ __global__ void adapt_exact_kernel(AdaptArgs a) {
     }
 }
 
-// Deterministic block-wide sum (fixed tree over lanes, then waves).
-__device__ double block_sum(double v, double* red) {
-    v = wave_sum_fixed(v);
-    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if ((threadIdx.x & 63) == 0) red[w] = v;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < nw; ++k) t += red[k];
-        red[32] = t;
-    }
-    __syncthreads();
-    t = red[32];
-    __syncthreads();
-    return t;
-}
-
 constexpr int kAdaptLdsMax = 4000;  // old increments handled in LDS (2 x 31.25 KB)
 
 __global__ __launch_bounds__(1024) void adapt_parallel_kernel(AdaptArgs a) {
@@ -181,68 +164,7 @@ __global__ __launch_bounds__(1024) void adapt_parallel_kernel(AdaptArgs a) {
     }
     __syncthreads();
     clear_row(a, d, T);
-    if (a.alpha > 0 && old > 1) {
-        double part = 0.0;
-        for (int i = threadIdx.x; i < old; i += T) {
-            double t;
-            if (i == 0) t = fabs(7. * av[0] + av[1]) / 8.;
-            else if (i == old - 1) t = fabs(7. * av[old - 1] + av[old - 2]) / 8.;
-            else t = fabs(6. * av[i] + av[i - 1] + av[i + 1]) / 8.;
-            P[i] = t;
-            part += t;
-        }
-        const double s = block_sum(part, red);
-        for (int i = threadIdx.x; i < old; i += T) {
-            double v = s > 0 ? P[i] / s + kTiny : kTiny;
-            if (v > 0 && v <= 0.99999999) v = pow(-(1 - v) / log(v), a.alpha);
-            av[i] = v;
-        }
-        __syncthreads();
-    }
-    // inclusive prefix sums of av (chunked Hillis-Steele; fixed order)
-    double carry = 0.0;
-    for (int base = 0; base < old; base += T) {
-        const int i = base + threadIdx.x;
-        double v = i < old ? av[i] : 0.0;
-        for (int o = 1; o < T; o <<= 1) {
-            double add = 0.0;
-            if (i < old) P[i] = v;
-            __syncthreads();
-            if ((int)threadIdx.x >= o && i < old) add = P[i - o];
-            __syncthreads();
-            v += add;
-        }
-        if (i < old) P[i] = carry + v;
-        __syncthreads();
-        const int last = (base + T < old ? base + T : old) - 1;
-        carry = P[last];
-        __syncthreads();
-    }
-    const double total = P[old - 1];
-    const double F = total / (double)nn;
-    if (threadIdx.x == 0) {
-        ng[0] = g[0];
-        ng[nn] = g[old];
-    }
-    for (int i = 1 + threadIdx.x; i < nn; i += T) {
-        const double target = (double)i * F;
-        int lo = 0, hi = old;  // first j with P[j] >= target
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (P[mid] >= target) hi = mid; else lo = mid + 1;
-        }
-        double xv;
-        if (lo >= old) {
-            xv = g[old];
-        } else {
-            const int j = lo;
-            const double acc = P[j] - target;
-            xv = g[j + 1] - (acc / av[j]) * (g[j + 1] - g[j]);
-        }
-        ng[i] = xv;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) pad_row(ng, nn, a.new_ld);
+    adapt_axis_block(av, P, red, g, ng, old, nn, a.alpha, a.new_ld);
 }
 
 // The integrator's adapt: in place on `grid`, skipped when *status != 0,

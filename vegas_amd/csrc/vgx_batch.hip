@@ -1,0 +1,753 @@
+// vgx_batch.hip -- n independent vegas+ integrals of single-Gaussian
+// integrands advanced together (vgx_batch_*, include/vgx.h).
+//
+// Every integral keeps the single-integral semantics (Integrator.__call__,
+// src/vegas/_vegas.pyx:1912-2230, with beta > 0 and adapt_to_errors off) and
+// its own Philox stream (seed, iteration, integral-local point index), so
+// its samples are exactly those of a vgx_integrate run.  Per iteration, for
+// all integrals at once:
+//   b_alloc    n_h per hcube (pyx:1657-1706), per-workgroup totals
+//   b_plan     per-integral point counts, tile and work-item layout: each
+//              integral's points start on a wave-tile boundary, work items
+//              are runs of whole tiles of one integral
+//   b_offsets  point offsets and tile -> first hcube
+//   b_fused    persistent sampler: workgroups pull work items from a queue,
+//              stage that integral's axes and grid in LDS, run the shared
+//              tile body, and write the item's training histogram and
+//              compensated partial sums (deterministic per item)
+//   strad      cut hcubes (shared kernel), one contribution per tile slot
+//   b_close    per integral: estimate, sum_sigf, weighted running average
+//   b_adapt    per (integral, axis): training histogram summed over the
+//              integral's items in order, then the parallel adapt in place
+// The nitn iterations of a call form one hipGraph, captured on first use.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "vgx_adapt.h"
+#include "vgx_strad.h"
+
+namespace vgx {
+
+constexpr int kBD = VGX_BATCH_MAXDIM;
+constexpr int kBLd = VGX_BATCH_LD;
+constexpr int kBNb = VGX_BATCH_MAXINC;
+constexpr int kBHist = kBD * kBNb;     // histogram bins of one work item
+constexpr int kBMaxTpi = 240;          // u16 LDS counts: 240 tiles x 256 < 65536
+
+// Static description of one integral.
+struct BInt {
+    int32_t dim, nhcube, nmin, nmax;
+    int32_t blk0, nblk, pad0, pad1;
+    uint32_t hbase, seed_lo, seed_hi, pad2;
+    double neval_sig;  // neval_frac * neval: points steered by sigf
+    double dv_y;       // 1 / nhcube
+    int32_t ninc[kBD];
+    int32_t pad3[2];
+};
+
+// Per-iteration layout of one integral.
+struct BPlan {
+    uint32_t T;       // points
+    uint32_t P;       // first point (a multiple of the tile size)
+    uint32_t tb, nt;  // first tile, tiles
+    uint32_t ib, ni;  // first work item, items
+    int32_t nhmin, nhmax;
+};
+
+struct BItem {
+    int32_t i;
+    uint32_t t0, t1, pad;
+};
+
+struct BScal {
+    uint32_t nitems, work, iter0, pad;
+};
+
+struct BArgs {
+    int n;
+    int tpi;
+    uint32_t maxtiles, maxitems;
+    double alpha, beta_half;
+    const BInt* ints;
+    const AxisC* axes;     // [n][kBD]
+    const int2* blk;       // alloc workgroup -> (integral, first local hcube)
+    const int2* adt;       // adapt workgroup -> (integral, axis)
+    double* grid;          // [n][kBD][kBLd]
+    double* sigf;
+    double* sum_sigf;
+    int32_t* reset;        // [n]: sum of sigf was 0, use sigf = 1
+    int32_t* nh;
+    uint64_t* stat;        // [nblk][3]
+    uint32_t* off;
+    int32_t* tile_h;
+    Slot* slots;
+    double* cont;          // [2 maxtiles][3]
+    BPlan* plan;
+    BItem* items;
+    double* item_part;     // [maxitems][6]
+    double* item_hs;       // [maxitems][kBHist]
+    uint32_t* item_hc;     // [maxitems][kBHist]
+    const float* fA;       // integrand parameters in FP32
+    const float* fB;
+    const float* fc;       // [n][kBD]
+    Scalars* sc;
+    BScal* bs;
+};
+
+// ---------------------------------------------------------------- alloc ----
+__global__ __launch_bounds__(kAllocThreads) void b_alloc_kernel(BArgs a) {
+    __shared__ uint64_t red[kAllocThreads / 64];
+    __shared__ int32_t rmin[kAllocThreads / 64], rmax[kAllocThreads / 64];
+    if (a.sc->status) return;
+    const int2 bt = a.blk[blockIdx.x];
+    const BInt& q = a.ints[bt.x];
+    const int reset = a.reset[bt.x];
+    const double ss = reset ? (double)q.nhcube : a.sum_sigf[bt.x];
+    const double neval_sigf = ss > 0 ? q.neval_sig / ss : 0.0;
+    uint64_t sum = 0;
+    int32_t mn = q.nmin, mx = q.nmin;
+    for (int k = 0; k < kAllocItems; ++k) {
+        const int h = bt.y + k * kAllocThreads + (int)threadIdx.x;
+        if (h >= q.nhcube || h >= bt.y + kAllocPerBlock) break;
+        const uint32_t H = q.hbase + (uint32_t)h;
+        double sv;
+        if (reset) {
+            sv = 1.0;
+            a.sigf[H] = 1.0;
+        } else {
+            sv = a.sigf[H];
+        }
+        double v = sv * neval_sigf;
+        v = v < 2147483647.0 ? v : 2147483647.0;
+        int64_t nn = (int64_t)(int32_t)v + q.nmin;
+        if (nn > q.nmax) nn = q.nmax;
+        const int32_t n = (int32_t)nn;
+        mn = n < mn ? n : mn;
+        mx = n > mx ? n : mx;
+        a.nh[H] = n;
+        sum += (uint64_t)n;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o, kWave);
+        mn = min(mn, __shfl_xor(mn, o, kWave));
+        mx = max(mx, __shfl_xor(mx, o, kWave));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w] = sum;
+        rmin[w] = mn;
+        rmax[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s = 0;
+        for (int k = 0; k < kAllocThreads / 64; ++k) {
+            s += red[k];
+            mn = min(mn, rmin[k]);
+            mx = max(mx, rmax[k]);
+        }
+        a.stat[3 * blockIdx.x] = s;
+        a.stat[3 * blockIdx.x + 1] = (uint64_t)(uint32_t)mn;
+        a.stat[3 * blockIdx.x + 2] = (uint64_t)(uint32_t)mx;
+    }
+}
+
+// ----------------------------------------------------------------- plan ----
+// One workgroup of 1024.  Chunked Hillis-Steele scans over the integrals of
+// their tile and item counts; then the work items.
+__global__ __launch_bounds__(1024) void b_plan_kernel(BArgs a) {
+    __shared__ uint32_t st[1024], si[1024];
+    Scalars* sc = a.sc;
+    if (sc->status) return;
+    for (int i = threadIdx.x; i < a.n; i += blockDim.x) {
+        const BInt& q = a.ints[i];
+        uint64_t T = 0;
+        int32_t mn = q.nmin, mx = q.nmin;
+        for (int b = q.blk0; b < q.blk0 + q.nblk; ++b) {
+            T += a.stat[3 * b];
+            mn = min(mn, (int32_t)a.stat[3 * b + 1]);
+            mx = max(mx, (int32_t)a.stat[3 * b + 2]);
+        }
+        BPlan& p = a.plan[i];
+        const uint64_t nt = (T + kT - 1) / kT;
+        p.T = T < 0xFFFFFFFFull ? (uint32_t)T : 0xFFFFFFFFu;
+        p.nt = nt < 0xFFFFFFFFull ? (uint32_t)nt : 0xFFFFFFFFu;
+        p.ni = (uint32_t)((nt + a.tpi - 1) / a.tpi);
+        p.nhmin = mn;
+        p.nhmax = mx;
+    }
+    __syncthreads();
+    uint64_t ct = 0, ci = 0;
+    for (int base = 0; base < a.n; base += 1024) {
+        const int i = base + threadIdx.x;
+        const uint32_t vt = i < a.n ? a.plan[i].nt : 0u, vi = i < a.n ? a.plan[i].ni : 0u;
+        st[threadIdx.x] = vt;
+        si[threadIdx.x] = vi;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const uint32_t at = threadIdx.x >= (unsigned)o ? st[threadIdx.x - o] : 0u;
+            const uint32_t ai = threadIdx.x >= (unsigned)o ? si[threadIdx.x - o] : 0u;
+            __syncthreads();
+            st[threadIdx.x] += at;
+            si[threadIdx.x] += ai;
+            __syncthreads();
+        }
+        if (i < a.n) {
+            BPlan& p = a.plan[i];
+            const uint64_t tb = ct + st[threadIdx.x] - vt;
+            p.tb = (uint32_t)(tb < 0xFFFFFFFFull ? tb : 0xFFFFFFFFull);
+            p.P = (uint32_t)((tb * kT) < 0xFFFFFFFFull ? tb * kT : 0xFFFFFFFFull);
+            p.ib = (uint32_t)(ci + si[threadIdx.x] - vi);
+        }
+        const uint32_t lt = st[1023], li = si[1023];
+        __syncthreads();
+        ct += lt;
+        ci += li;
+    }
+    const bool ok = ct <= a.maxtiles && ci <= a.maxitems && ct * kT < 0xFFFFFFFFull;
+    __syncthreads();
+    if (ok) {
+        for (int i = threadIdx.x; i < a.n; i += blockDim.x) {
+            const BPlan p = a.plan[i];
+            for (uint32_t k = 0; k < p.ni; ++k) {
+                BItem it;
+                it.i = i;
+                it.t0 = p.tb + k * (uint32_t)a.tpi;
+                it.t1 = min(it.t0 + (uint32_t)a.tpi, p.tb + p.nt);
+                it.pad = 0;
+                a.items[p.ib + k] = it;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        sc->overflow = ok ? 0u : 1u;
+        sc->b_ntiles = ok ? (uint32_t)ct : 0u;
+        sc->b_p0 = 0;
+        sc->b_p1 = ok ? (uint32_t)(ct * kT) : 0u;
+        a.bs->nitems = ok ? (uint32_t)ci : 0u;
+        a.bs->work = 0;
+    }
+}
+
+// -------------------------------------------------------------- offsets ----
+__device__ inline int b_pad(int j) { return j + (j >> 3); }
+
+__global__ __launch_bounds__(kAllocThreads) void b_offsets_kernel(BArgs a) {
+    __shared__ uint32_t ln[kAllocPerBlock + kAllocPerBlock / 8];
+    __shared__ uint64_t wred[kAllocThreads / 64];
+    __shared__ uint32_t wsum[kAllocThreads / 64];
+    if (a.sc->status || a.sc->overflow) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int2 bt = a.blk[blockIdx.x];
+    const BInt& q = a.ints[bt.x];
+    const BPlan& pl = a.plan[bt.x];
+    // points of the integral's workgroups before this one
+    uint64_t pre = 0;
+    for (int b = q.blk0 + (int)threadIdx.x; b < (int)blockIdx.x; b += kAllocThreads) pre += a.stat[3 * b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) pre += __shfl_xor(pre, o, kWave);
+    if (lane == 0) wred[w] = pre;
+    __syncthreads();
+    pre = 0;
+    for (int k = 0; k < kAllocThreads / 64; ++k) pre += wred[k];
+    const int hs = bt.y;
+    const int cnt = min(kAllocPerBlock, q.nhcube - hs);
+    for (int i = threadIdx.x; i < kAllocPerBlock; i += kAllocThreads)
+        ln[b_pad(i)] = i < cnt ? (uint32_t)a.nh[q.hbase + hs + i] : 0u;
+    __syncthreads();
+    uint32_t loc[kAllocItems];
+    uint32_t run = 0;
+    for (int i = 0; i < kAllocItems; ++i) {
+        loc[i] = run;
+        run += ln[b_pad(threadIdx.x * kAllocItems + i)];
+    }
+    uint32_t incl = run;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t wbase = 0;
+    for (int k = 0; k < w; ++k) wbase += wsum[k];
+    const uint32_t tbase = pl.P + (uint32_t)pre + wbase + incl - run;
+    for (int i = 0; i < kAllocItems; ++i) {
+        const int j = threadIdx.x * kAllocItems + i;
+        const uint32_t lo = tbase + loc[i];
+        const uint32_t n = ln[b_pad(j)];
+        if (j < cnt) {
+            const uint32_t hi = lo + n;
+            for (uint32_t t = (lo + kT - 1) / kT; t < (hi + kT - 1) / kT; ++t)
+                a.tile_h[t] = (int32_t)(q.hbase + hs + j);
+        }
+        loc[i] = lo;
+    }
+    __syncthreads();
+    for (int i = 0; i < kAllocItems; ++i) ln[b_pad(threadIdx.x * kAllocItems + i)] = loc[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += kAllocThreads) a.off[q.hbase + hs + i] = ln[b_pad(i)];
+}
+
+// ---------------------------------------------------------------- fused ----
+// LDS: axes | grid rows | histogram sums | u16 pair-packed counts | offsets
+// slices | reduction scratch.
+constexpr size_t kBLdsAxes = 16 * sizeof(AxisC);
+constexpr size_t kBLdsGrid = (size_t)kBD * kBLd * sizeof(double);
+constexpr size_t kBLdsSums = (size_t)kBHist * sizeof(double);
+constexpr size_t kBLdsCnts = (size_t)(kBHist / 2) * sizeof(uint32_t);
+constexpr size_t kBLdsOffs = (size_t)16 * kNoff * sizeof(uint32_t);
+constexpr size_t kBLds = kBLdsAxes + kBLdsGrid + kBLdsSums + kBLdsCnts + kBLdsOffs + 16 * 6 * sizeof(double);
+
+__global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it, int train) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t s_item;
+    if (a.sc->status || a.sc->overflow) return;
+    AxisC* ax = reinterpret_cast<AxisC*>(smem);
+    double* G = reinterpret_cast<double*>(smem + kBLdsAxes);
+    double* Ls = reinterpret_cast<double*>(smem + kBLdsAxes + kBLdsGrid);
+    uint32_t* Lc = reinterpret_cast<uint32_t*>(smem + kBLdsAxes + kBLdsGrid + kBLdsSums);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t* offs = reinterpret_cast<uint32_t*>(smem + kBLdsAxes + kBLdsGrid + kBLdsSums + kBLdsCnts) +
+                     wave * kNoff;
+    double* Lred = reinterpret_cast<double*>(smem + kBLdsAxes + kBLdsGrid + kBLdsSums + kBLdsCnts + kBLdsOffs);
+    const uint32_t nitems = a.bs->nitems;
+    const uint32_t iteration = a.bs->iter0 + (uint32_t)it;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&a.bs->work, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= nitems) break;
+        const BItem itm = a.items[item];
+        const int i = itm.i;
+        const BInt& q = a.ints[i];
+        const BPlan& pl = a.plan[i];
+        const int dim = q.dim;
+        TileGeom tg;
+        tg.dim = dim;
+        tg.ld = kBLd;
+        tg.nb = kBNb;
+        tg.exact_div = 0;
+        tg.hist_lds = 1;
+        tg.train_points = train;
+        tg.adaptive_strat = 1;
+        tg.fast_pow = 1;
+        tg.train_errors = 0;
+        tg.iteration = iteration;
+        tg.seed_lo = q.seed_lo;
+        tg.seed_hi = q.seed_hi;
+        tg.beta_half = a.beta_half;
+        tg.dv_y = q.dv_y;
+        if (threadIdx.x < (unsigned)dim) ax[threadIdx.x] = a.axes[(size_t)i * kBD + threadIdx.x];
+        const double* grow = a.grid + (size_t)i * kBD * kBLd;
+        for (int k = threadIdx.x; k < dim * kBLd; k += blockDim.x) G[k] = grow[k];
+        const int tot = dim * kBNb;
+        for (int k = threadIdx.x; k < tot; k += blockDim.x) Ls[k] = 0.0;
+        for (int k = threadIdx.x; k < (tot + 1) / 2; k += blockDim.x) Lc[k] = 0u;
+        __syncthreads();
+        const GaussMix32 fcn{1, a.fA + i, a.fB + i, a.fc + (size_t)i * kBD};
+        const TileIO tio{a.off, a.sigf, a.slots, nullptr, nullptr, nullptr};
+        TileAcc acc{{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, 0.0, 0};
+        const uint32_t hend = q.hbase + (uint32_t)q.nhcube, off_end = pl.P + pl.T;
+        const uint32_t nr = (itm.t1 - itm.t0 + 15) / 16;
+        for (uint32_t r = 0; r < nr; ++r) {
+            const uint32_t t = itm.t0 + r * 16 + wave;
+            if (t < itm.t1) {
+                const uint32_t plo = t * kT;
+                const TilePos tp{t, plo, min(plo + (uint32_t)kT, off_end), (uint32_t)a.tile_h[t], hend, off_end,
+                                 q.hbase, pl.P};
+                tile_body<-kBD>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
+            }
+        }
+        __syncthreads();
+        if (train) {
+            double* hs = a.item_hs + (size_t)item * kBHist;
+            uint32_t* hc = a.item_hc + (size_t)item * kBHist;
+            for (int k = threadIdx.x; k < tot; k += blockDim.x) {
+                hs[k] = Ls[k];
+                hc[k] = (Lc[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+            }
+        }
+        block_close(acc, Lred, a.item_part + 6 * (size_t)item, a.sc);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- close ----
+// One wave per integral: the iteration's sums over the integral's work
+// items and cut-hcube slots (fixed order), sum_sigf (pyx:2209-2215) and the
+// weighted running average of the call (pyx:2392-2410).
+__global__ __launch_bounds__(256) void b_close_kernel(BArgs a, vgx_batch_state st, int it, int nitn) {
+    const Scalars* sc = a.sc;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (sc->overflow && !sc->status) a.sc->status = VGX_ERANGE;
+        if (st.status) *st.status = a.sc->status;
+        if (it == nitn - 1) a.bs->iter0 += (uint32_t)nitn;
+    }
+    if (i >= a.n || sc->status || sc->overflow) return;
+    const BPlan pl = a.plan[i];
+    DD m{0, 0}, v{0, 0}, s{0, 0};
+    for (uint32_t k = lane; k < pl.ni; k += kWave) {
+        const double* p = a.item_part + 6 * (size_t)(pl.ib + k);
+        m = dd_sum(m, DD{p[0], p[1]});
+        v = dd_sum(v, DD{p[2], p[3]});
+        s = dd_sum(s, DD{p[4], p[5]});
+    }
+    for (uint32_t k = 2 * pl.tb + lane; k < 2 * (pl.tb + pl.nt); k += kWave) {
+        const double* c = a.cont + 3 * (size_t)k;
+        dd_add(m, c[0]);
+        dd_add(v, c[1]);
+        dd_add(s, c[2]);
+    }
+    m = wave_sum_dd(m);
+    v = wave_sum_dd(v);
+    s = wave_sum_dd(s);
+    if (lane != 0) return;
+    const int n = a.n;
+    const double mean = m.hi + m.lo, var = v.hi + v.lo, ssig = s.hi + s.lo;
+    st.itn_mean[(size_t)it * n + i] = mean;
+    st.itn_var[(size_t)it * n + i] = var;
+    st.itn_neval[(size_t)it * n + i] = (int64_t)pl.T;
+    if (ssig > 0) {
+        st.sum_sigf[i] = ssig;
+        a.reset[i] = 0;
+    } else {
+        st.sum_sigf[i] = (double)a.ints[i].nhcube;
+        a.reset[i] = 1;
+    }
+    double sw = 0.0, swm = 0.0;
+    for (int k = 0; k <= it; ++k) {
+        const double sd = sqrt(st.itn_var[(size_t)k * n + i]);
+        const double vk = sd * sd;
+        const double w = 1. / (vk > kTiny ? vk : kTiny);
+        sw += w;
+        swm += w * st.itn_mean[(size_t)k * n + i];
+    }
+    const double rm = swm / sw;
+    double chi2 = 0.0;
+    if (it > 0)
+        for (int k = 0; k <= it; ++k) {
+            const double sd = sqrt(st.itn_var[(size_t)k * n + i]);
+            const double vk = sd * sd;
+            const double w = 1. / (vk > kTiny ? vk : kTiny);
+            const double d = rm - st.itn_mean[(size_t)k * n + i];
+            chi2 += d * d * w;
+        }
+    st.ravg[4 * i] = rm;
+    st.ravg[4 * i + 1] = sqrt(1. / sw);
+    st.ravg[4 * i + 2] = chi2;
+    st.ravg[4 * i + 3] = (double)it;
+}
+
+// ---------------------------------------------------------------- adapt ----
+// One workgroup per (integral, axis): training data sum_f, n_f of the axis
+// from the integral's work items in order (n_f = count, or TINY when empty,
+// as TINY + count in the reference), then AdaptiveMap.adapt(alpha) in place
+// (pyx:467-594), same ninc.
+__global__ __launch_bounds__(256) void b_adapt_kernel(BArgs a, int train) {
+    __shared__ double av[kBNb], P[kBNb], g[kBLd], red[40];
+    if (a.sc->status || a.sc->overflow) return;
+    const int2 id = a.adt[blockIdx.x];
+    const int i = id.x, d = id.y;
+    const BInt& q = a.ints[i];
+    const BPlan& pl = a.plan[i];
+    const int old = q.ninc[d];
+    double* row = a.grid + ((size_t)i * kBD + d) * kBLd;
+    for (int j = threadIdx.x; j <= old; j += blockDim.x) g[j] = row[j];
+    for (int j = threadIdx.x; j < old; j += blockDim.x) {
+        double val = 1.0;
+        if (a.alpha != 0 && old > 1 && train) {
+            double sf = 0.0;
+            uint32_t cnt = 0;
+            for (uint32_t k = 0; k < pl.ni; ++k) {
+                const size_t o = (size_t)(pl.ib + k) * kBHist + (size_t)d * kBNb + j;
+                sf += a.item_hs[o];
+                cnt += a.item_hc[o];
+            }
+            val = cnt > 0 ? sf / (double)cnt : 0.0;
+        }
+        av[j] = val;
+    }
+    __syncthreads();
+    adapt_axis_block(av, P, red, g, row, old, old, a.alpha, kBLd);
+}
+
+__global__ void b_to_float_kernel(const double* A, const double* B, const double* c, float* fA, float* fB,
+                                  float* fc, int n) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n * kBD; k += gridDim.x * blockDim.x) {
+        fc[k] = (float)c[k];
+        if (k < n) {
+            fA[k] = (float)A[k];
+            fB[k] = (float)B[k];
+        }
+    }
+}
+
+}  // namespace vgx
+
+using namespace vgx;
+
+struct vgx_batch {
+    vgx_ctx* ctx = nullptr;
+    int n = 0, tpi = 128, blocks = 0, nblk = 0, nad = 0, strad_blocks = 0;
+    uint32_t maxtiles = 0, maxitems = 0;
+    int64_t htot = 0;
+    double alpha = 0, beta = 0;
+    std::vector<int64_t> hbase;
+    std::vector<void*> bufs;
+    BArgs a{};
+    IterGeom sg{};  // shared flags for the straddler kernel
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double* strad_part = nullptr;
+    // arguments of the captured graph
+    int g_nitn = -1;
+    const double *g_A = nullptr, *g_B = nullptr, *g_c = nullptr;
+    vgx_batch_state g_st{};
+};
+
+static int b_alloc(vgx_batch* b, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(p, bytes) != hipSuccess) return VGX_ENOMEM;
+    b->bufs.push_back(*p);
+    return VGX_OK;
+}
+
+static void b_free(vgx_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    for (void* p : b->bufs) (void)hipFree(p);
+    b->bufs.clear();
+}
+
+extern "C" int vgx_batch_create(vgx_ctx* ctx, const vgx_batch_config* c, vgx_batch** out) {
+    if (!ctx || !c || !out || c->n < 1 || !c->dim || !c->ninc || !c->nstrat || !c->neval ||
+        !c->min_neval_hcube || !c->max_neval_hcube || !c->seed)
+        return VGX_EINVAL;
+    if (!(c->beta > 0) || c->neval_frac < 0 || c->neval_frac >= 1) return VGX_EINVAL;
+    const int tpi = c->tiles_per_item ? c->tiles_per_item : 128;
+    if (tpi < 1 || tpi > kBMaxTpi) return VGX_ERANGE;
+    const int n = c->n;
+    std::vector<BInt> ints(n);
+    std::vector<AxisC> axes((size_t)n * kBD);
+    std::vector<int2> blk, adt;
+    vgx_batch* b = new vgx_batch();
+    b->ctx = ctx;
+    b->n = n;
+    b->tpi = tpi;
+    b->alpha = c->alpha;
+    b->beta = c->beta;
+    b->hbase.resize(n);
+    uint64_t maxtiles = 0, maxitems = 0;
+    int64_t H = 0;
+    for (int i = 0; i < n; ++i) {
+        BInt& q = ints[i];
+        memset(&q, 0, sizeof(q));
+        const int dim = c->dim[i];
+        if (dim < 1 || dim > kBD) { delete b; return VGX_ERANGE; }
+        int64_t nh = 1;
+        for (int d = 0; d < dim; ++d) {
+            const int ni = c->ninc[i * kBD + d], ns = c->nstrat[i * kBD + d];
+            if (ni < 1 || ni > kBNb || ns < 1) { delete b; return VGX_EINVAL; }
+            q.ninc[d] = ni;
+            nh *= ns;
+            const DivMagic dm = make_divmagic((uint32_t)ns);
+            const uint32_t per = (ni % ns == 0 && ni / ns < (1 << 20)) ? (uint32_t)(ni / ns) : 0u;
+            axes[(size_t)i * kBD + d] = AxisC{dm.m, dm.l, (uint32_t)ns, (uint32_t)ni, 1.0 / ns, per, 0u};
+        }
+        const int64_t nmin = c->min_neval_hcube[i];
+        const int64_t nmax = c->max_neval_hcube[i] > nmin ? c->max_neval_hcube[i] : nmin;
+        if (nh < 2 || nh >= 0x7FFFFFFF || nmin < 2 || nmax > 0x7FFFFFFF) { delete b; return VGX_ERANGE; }
+        q.dim = dim;
+        q.nhcube = (int32_t)nh;
+        q.nmin = (int32_t)nmin;
+        q.nmax = (int32_t)nmax;
+        q.hbase = (uint32_t)H;
+        q.seed_lo = (uint32_t)(c->seed[i] & 0xffffffffu);
+        q.seed_hi = (uint32_t)(c->seed[i] >> 32);
+        q.neval_sig = c->neval_frac * (double)c->neval[i];
+        q.dv_y = 1. / (double)nh;
+        q.blk0 = (int)blk.size();
+        q.nblk = (int)((nh + kAllocPerBlock - 1) / kAllocPerBlock);
+        for (int k = 0; k < q.nblk; ++k) blk.push_back(int2{i, k * kAllocPerBlock});
+        for (int d = 0; d < dim; ++d) adt.push_back(int2{i, d});
+        b->hbase[i] = H;
+        H += nh;
+        // the allocation's largest point count (as max_points of one integral)
+        const uint64_t cap = (uint64_t)(q.neval_sig * (1.0 + 1e-9)) + 2 + (uint64_t)nh * (uint64_t)nmin;
+        const uint64_t tiles = (cap + kT - 1) / kT;
+        maxtiles += tiles;
+        maxitems += (tiles + tpi - 1) / tpi;
+    }
+    if (H >= 0x7FFFFFFF || maxtiles * kT >= 0xFFFFFFF0ull) { delete b; return VGX_ERANGE; }
+    b->htot = H;
+    b->nblk = (int)blk.size();
+    b->nad = (int)adt.size();
+    b->maxtiles = (uint32_t)maxtiles;
+    b->maxitems = (uint32_t)maxitems;
+    b->strad_blocks = (int)((2ull * maxtiles + kStradThreads - 1) / kStradThreads);
+    b->blocks = c->blocks > 0 ? c->blocks : ctx->info.cus;
+    BArgs& a = b->a;
+    a.n = n;
+    a.tpi = tpi;
+    a.maxtiles = b->maxtiles;
+    a.maxitems = b->maxitems;
+    a.alpha = c->alpha;
+    a.beta_half = c->beta / 2.;
+    int rc = VGX_OK;
+    void* p;
+#define BA(field, type, count)                                                       \
+    if ((rc = b_alloc(b, &p, (size_t)(count) * sizeof(type)))) { b_free(b); delete b; return rc; } \
+    a.field = (type*)p;
+    BA(ints, BInt, n);
+    BA(axes, AxisC, (size_t)n * kBD);
+    BA(blk, int2, b->nblk);
+    BA(adt, int2, b->nad);
+    BA(reset, int32_t, n);
+    BA(nh, int32_t, H);
+    BA(stat, uint64_t, 3 * (size_t)b->nblk);
+    BA(off, uint32_t, H);
+    BA(tile_h, int32_t, maxtiles);
+    BA(slots, Slot, 2 * maxtiles);
+    BA(cont, double, 6 * maxtiles);
+    BA(plan, BPlan, n);
+    BA(items, BItem, maxitems);
+    BA(item_part, double, 6 * maxitems);
+    BA(item_hs, double, (size_t)maxitems * kBHist);
+    BA(item_hc, uint32_t, (size_t)maxitems * kBHist);
+    BA(fA, float, n);
+    BA(fB, float, n);
+    BA(fc, float, (size_t)n * kBD);
+    BA(sc, Scalars, 1);
+    BA(bs, BScal, 1);
+    if ((rc = b_alloc(b, &p, (size_t)b->strad_blocks * 6 * sizeof(double)))) { b_free(b); delete b; return rc; }
+    double* strad_part = (double*)p;
+#undef BA
+    if (hipMemcpy((void*)a.ints, ints.data(), n * sizeof(BInt), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((void*)a.axes, axes.data(), axes.size() * sizeof(AxisC), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((void*)a.blk, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((void*)a.adt, adt.data(), adt.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(a.reset, 0, n * sizeof(int32_t)) != hipSuccess ||
+        hipMemset(a.sc, 0, sizeof(Scalars)) != hipSuccess || hipMemset(a.bs, 0, sizeof(BScal)) != hipSuccess) {
+        b_free(b);
+        delete b;
+        return VGX_EHIP;
+    }
+    // shared flags of the straddler kernel
+    IterGeom& g = b->sg;
+    memset(&g, 0, sizeof(g));
+    g.adaptive_strat = 1;
+    g.fast_pow = 1;
+    g.beta_half = c->beta / 2.;
+    b->strad_part = strad_part;
+    if (kBLds + 64 > ctx->info.max_lds) { b_free(b); delete b; return VGX_ERANGE; }
+    if (hipFuncSetAttribute((const void*)b_fused_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kBLds) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess) {
+        b_free(b);
+        delete b;
+        return VGX_EHIP;
+    }
+    *out = b;
+    return VGX_OK;
+}
+
+extern "C" int vgx_batch_destroy(vgx_batch* b) {
+    if (!b) return VGX_OK;
+    (void)hipDeviceSynchronize();
+    b_free(b);
+    delete b;
+    return VGX_OK;
+}
+
+extern "C" int vgx_batch_sizes(const vgx_batch* b, int64_t* total_hcubes, int64_t* hcube_base) {
+    if (!b) return VGX_EINVAL;
+    if (total_hcubes) *total_hcubes = b->htot;
+    if (hcube_base)
+        for (int i = 0; i < b->n; ++i) hcube_base[i] = b->hbase[i];
+    return VGX_OK;
+}
+
+extern "C" int vgx_batch_set_iteration(vgx_batch* b, uint32_t iteration) {
+    if (!b) return VGX_EINVAL;
+    BScal bs{};
+    if (hipMemcpy(&bs, b->a.bs, sizeof(BScal), hipMemcpyDeviceToHost) != hipSuccess) return VGX_EHIP;
+    bs.iter0 = iteration;
+    if (hipMemcpy(b->a.bs, &bs, sizeof(BScal), hipMemcpyHostToDevice) != hipSuccess) return VGX_EHIP;
+    return VGX_OK;
+}
+
+// The kernels of nitn iterations on stream s (recorded into the graph).
+static int b_enqueue(vgx_batch* b, const double* A, const double* B, const double* c, const vgx_batch_state& st,
+                     int nitn, hipStream_t s) {
+    BArgs a = b->a;
+    a.grid = st.grid;
+    a.sigf = st.sigf;
+    a.sum_sigf = st.sum_sigf;
+    const int train = b->alpha > 0 ? 1 : 0;
+    hipLaunchKernelGGL(b_to_float_kernel, dim3(64), dim3(256), 0, s, A, B, c, (float*)a.fA, (float*)a.fB,
+                       (float*)a.fc, b->n);
+    StradArgs sa{};
+    sa.errv = nullptr;
+    sa.off = a.off;
+    sa.slots = a.slots;
+    sa.sigf = st.sigf;
+    sa.strad_part = b->strad_part;
+    sa.cont = a.cont;
+    sa.sc = a.sc;
+    for (int it = 0; it < nitn; ++it) {
+        hipLaunchKernelGGL(b_alloc_kernel, dim3(b->nblk), dim3(kAllocThreads), 0, s, a);
+        hipLaunchKernelGGL(b_plan_kernel, dim3(1), dim3(1024), 0, s, a);
+        hipLaunchKernelGGL(b_offsets_kernel, dim3(b->nblk), dim3(kAllocThreads), 0, s, a);
+        hipLaunchKernelGGL(b_fused_kernel, dim3(b->blocks), dim3(kFusedThreads), kBLds, s, a, it, train);
+        hipLaunchKernelGGL(strad_kernel, dim3(b->strad_blocks), dim3(kStradThreads), 0, s, b->sg, sa);
+        hipLaunchKernelGGL(b_close_kernel, dim3((b->n + 3) / 4), dim3(256), 0, s, a, st, it, nitn);
+        hipLaunchKernelGGL(b_adapt_kernel, dim3(b->nad), dim3(256), 0, s, a, train);
+    }
+    return check_launch();
+}
+
+extern "C" int vgx_batch_integrate(vgx_batch* b, const double* A, const double* B, const double* c,
+                                   const vgx_batch_state* st, int nitn, void* stream) {
+    if (!b || !A || !B || !c || !st || !st->grid || !st->sigf || !st->sum_sigf || !st->itn_mean ||
+        !st->itn_var || !st->itn_neval || !st->ravg || nitn < 1)
+        return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    const bool same = b->exec && b->g_nitn == nitn && b->g_A == A && b->g_B == B && b->g_c == c &&
+                      memcmp(&b->g_st, st, sizeof(vgx_batch_state)) == 0;
+    if (!same) {
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        // status flags of the previous run (NaN, overflow) do not carry over
+        if (hipMemsetAsync(b->a.sc, 0, sizeof(Scalars), s) != hipSuccess) return VGX_EHIP;
+        if (hipStreamSynchronize(s) != hipSuccess) return VGX_EHIP;
+        VGX_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+        const int rc = b_enqueue(b, A, B, c, *st, nitn, b->cap);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->cap, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        if (e != hipSuccess || !gr) return VGX_EHIP;
+        b->graph = gr;
+        if (hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0) != hipSuccess) {
+            b->exec = nullptr;
+            return VGX_EHIP;
+        }
+        b->g_nitn = nitn;
+        b->g_A = A;
+        b->g_B = B;
+        b->g_c = c;
+        b->g_st = *st;
+    }
+    VGX_HIP(hipGraphLaunch(b->exec, s));
+    return VGX_OK;
+}

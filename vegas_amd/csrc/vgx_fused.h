@@ -14,6 +14,34 @@
 
 namespace vgx {
 
+// Scalars of one integral's iteration that the sampler and the tile body
+// read (a kernel-argument IterGeom, or one integral of a batch).
+struct TileGeom {
+    int dim, ld, nb;
+    int exact_div, hist_lds, train_points, adaptive_strat, fast_pow, train_errors;
+    uint32_t iteration, seed_lo, seed_hi;
+    double beta_half, dv_y;
+};
+
+__host__ __device__ inline TileGeom tile_geom(const IterGeom& g) {
+    TileGeom t;
+    t.dim = g.dim;
+    t.ld = g.ld;
+    t.nb = g.nb;
+    t.exact_div = g.exact_div;
+    t.hist_lds = g.hist_lds;
+    t.train_points = g.train_points;
+    t.adaptive_strat = g.adaptive_strat;
+    t.fast_pow = g.fast_pow;
+    t.train_errors = g.train_errors;
+    t.iteration = g.iteration;
+    t.seed_lo = g.seed_lo;
+    t.seed_hi = g.seed_hi;
+    t.beta_half = g.beta_half;
+    t.dv_y = g.dv_y;
+    return t;
+}
+
 // Per-axis constants of the sampler, staged in LDS by the fused kernel.
 struct AxisC {
     uint32_t m, l, nstrat, ninc;  // DivMagic of nstrat, strata and increments
@@ -30,8 +58,9 @@ struct AxisC {
 //   otherwise (53-bit uniforms, or unaligned axes, or exact_div) y is formed
 //       in FP64 as the reference does, y = (c+u)/nstrat (pyx:1741), and the
 //       map computes t = y*ninc (pyx:351).
+//   DIM > 0: that dimension; DIM <= 0: runtime g.dim, at most nd_max(DIM).
 template <int DIM, bool R32>
-__device__ __forceinline__ void eval_point(const IterGeom& g, const AxisC* __restrict__ ax,
+__device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __restrict__ ax,
                                            const double* __restrict__ G, uint32_t h, uint32_t p,
                                            double* x, double& J, int* kbin) {
     const int dim = DIM > 0 ? DIM : g.dim;
@@ -39,8 +68,8 @@ __device__ __forceinline__ void eval_point(const IterGeom& g, const AxisC* __res
     U4 w{0, 0, 0, 0};
     J = 1.0;
 #pragma unroll
-    for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
-        if (DIM == 0 && d >= dim) break;
+    for (int d = 0; d < nd_max(DIM); ++d) {
+        if (DIM <= 0 && d >= dim) break;
         const AxisC a = ax[d];
         const uint32_t q = (uint32_t)(((uint64_t)__umulhi(t, a.m) + t) >> a.l);
         const uint32_t c = t - q * a.nstrat;
@@ -91,7 +120,7 @@ __device__ __forceinline__ void eval_point(const IterGeom& g, const AxisC* __res
 
 // y of point p of hcube h, formed as eval_point forms it (staged sampler).
 template <bool R32>
-__device__ inline void point_y(const IterGeom& g, const AxisC* __restrict__ ax, uint32_t h,
+__device__ inline void point_y(const TileGeom& g, const AxisC* __restrict__ ax, uint32_t h,
                                uint32_t p, double* y) {
     uint32_t t = h;
     U4 w{0, 0, 0, 0};
@@ -171,12 +200,304 @@ __device__ __forceinline__ void seg_prefix(const int (&key)[kK], const double (&
         if (key[j] == key[0]) P[j] += cin;
 }
 
+// Global pointers the tile body writes or reads.
+struct TileIO {
+    const uint32_t* off;     // point offset of every hcube
+    double* sigf;
+    Slot* slots;
+    double* errv;
+    double* sum_f;           // global histogram (when it is not in LDS)
+    double* n_f;
+};
+
+// Per-wave accumulators across tiles.
+struct TileAcc {
+    DD am, av, as;           // sum of S + D, of var_h, of sigf_h
+    double wmax;
+    int nan_seen;
+};
+
+// Where the tile sits: global tile t with points [plo, phi), first hcube hf;
+// hcubes [.., hend) of the batch (offset off_end at hend); the integral's
+// first hcube hbase and the shift from global to integral-local point index
+// (both 0 for a single integral).
+struct TilePos {
+    uint32_t t, plo, phi, hf, hend, off_end, hbase, pshift;
+};
+
+// One wave tile (kT points, kK per lane): sample, map, integrand, training
+// histogram, and the reference's two passes per hcube as segmented scans.
+template <int DIM, class Fcn>
+__device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, const Fcn& fcn,
+                                          const AxisC* __restrict__ ax, const double* __restrict__ G,
+                                          double* Ls, uint32_t* Lc, uint32_t* offs, const TilePos& tp,
+                                          int lane, TileAcc& acc) {
+    constexpr bool R32 = Fcn::kRng32;
+    constexpr int ND = nd_max(DIM);
+    const int dim = DIM > 0 ? DIM : g.dim;
+    const double eps = kEpsilon;
+    const uint32_t t = tp.t, plo = tp.plo, phi = tp.phi, hf = tp.hf;
+    for (int i = lane; i < kNoff; i += kWave) {
+        const uint32_t hh = hf + i;
+        offs[i] = hh < tp.hend ? io.off[hh] : tp.off_end;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // hcube (local index) of this lane's first point
+    const uint32_t pl0 = plo + lane * kK;
+    int li = 0;
+    {
+        int lo = 0, hi = kNoff - 2;  // largest i with offs[i] <= pl0
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= pl0) lo = mid; else hi = mid - 1;
+        }
+        li = lo;
+    }
+    // Points are evaluated one at a time (no unrolling: one point's
+    // registers live at once); results rotate through a register window so
+    // element j ends in key[j], wf[j], nn[j].
+    int key[kK];
+    double wf[kK];
+    uint32_t nn[kK];
+#pragma unroll
+    for (int j = 0; j < kK; ++j) {
+        key[j] = kKeySentinel;
+        wf[j] = 0.0;
+        nn[j] = 1;
+    }
+    int cur_i = -1;
+    double wdiv = 0.0;
+    uint32_t ncur = 1;
+#pragma unroll 1
+    for (int j = 0; j < kK; ++j) {
+        const uint32_t pj = pl0 + j;
+        int kj = kKeySentinel;
+        double vj = 0.0;
+        uint32_t nj = 1;
+        if (pj < phi) {
+            while (li < kNoff - 2 && offs[li + 1] <= pj) ++li;
+            if (li != cur_i) {
+                cur_i = li;
+                ncur = offs[li + 1] - offs[li];
+                wdiv = g.dv_y / (double)ncur;
+            }
+            kj = li;
+            nj = ncur;
+            double x[ND];
+            int kb[ND];
+            double J;
+            eval_point<DIM, R32>(g, ax, G, hf + li - tp.hbase, pj - tp.pshift, x, J, kb);
+            const double w = J * wdiv;
+            const double f = fcn.template eval<DIM>(x, dim, pj);
+            if (f != f) acc.nan_seen = 1;
+            vj = w * f;
+            const double tn = vj * (double)ncur;
+            acc.wmax = fmax(acc.wmax, fabs(tn));
+            if (g.train_points) {
+                const double c2 = tn * tn;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    if (DIM <= 0 && d >= dim) break;
+                    const int k = kb[d];
+                    if (k < 0) continue;
+                    const int idx = d * g.nb + k;
+                    if (g.hist_lds) {
+                        unsafeAtomicAdd(&Ls[idx], c2);
+                        atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
+                    } else {
+                        unsafeAtomicAdd(&io.sum_f[idx], c2);
+                        unsafeAtomicAdd(&io.n_f[idx], 1.0);
+                    }
+                }
+            }
+        }
+        // rotate: element j enters at the back of the window
+#pragma unroll
+        for (int q = 0; q < kK - 1; ++q) {
+            key[q] = key[q + 1];
+            wf[q] = wf[q + 1];
+            nn[q] = nn[q + 1];
+        }
+        key[kK - 1] = kj;
+        wf[kK - 1] = vj;
+        nn[kK - 1] = nj;
+    }
+    // ---- pass 1: S per segment (pyx:2147-2152)
+    double P[kK];
+    seg_prefix(key, wf, P, lane);
+    int endf[kK];
+    const int next_first = __shfl_down(key[0], 1, kWave);
+#pragma unroll
+    for (int j = 0; j < kK - 1; ++j) endf[j] = key[j + 1] != key[j];
+    endf[kK - 1] = (lane == kWave - 1) || (next_first != key[kK - 1]);
+    // Total of the lane's last segment.  If it ends in this lane it is
+    // P[kK-1]; otherwise it ends inside the first run of the nearest later
+    // lane whose first run ends there: copy that run's closing prefix
+    // backwards across the lanes that only continue.
+    double fv = P[0];
+#pragma unroll
+    for (int j = 1; j < kK; ++j)
+        if (key[j] == key[0]) fv = P[j];
+    double rv = fv;
+    int rf = (key[0] != key[kK - 1]) || endf[kK - 1];
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const double nv = __shfl_down(rv, o, kWave);
+        const int nf = __shfl_down(rf, o, kWave);
+        if (!rf && lane + o < kWave) {
+            rv = nv;
+            rf = nf;
+        }
+    }
+    const double next_rv = __shfl_down(rv, 1, kWave);
+    const double vlast = endf[kK - 1] ? P[kK - 1] : next_rv;
+    double S[kK];
+    {
+        double c = vlast;
+#pragma unroll
+        for (int j = kK - 1; j >= 0; --j) {
+            if (j < kK - 1 && endf[j]) c = P[j];
+            S[j] = c;
+        }
+    }
+    // straddling segments: first (local 0) and last (kL)
+    int kL;
+    {
+        int lo = 0, hi = kNoff - 2;  // largest i with offs[i] < phi
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (offs[mid] < phi) lo = mid; else hi = mid - 1;
+        }
+        kL = lo;
+    }
+    const bool sA = offs[0] < plo || offs[1] > phi;
+    const bool sB = offs[kL + 1] > phi;
+    // ---- pass 2: guarded deviations (pyx:2153-2169).  A cut hcube's part
+    // uses the part's own mean; the straddler kernel combines the parts.
+    double qv[kK], dv[kK];
+#pragma unroll
+    for (int j = 0; j < kK; ++j) {
+        qv[j] = 0.0;
+        dv[j] = 0.0;
+        const int k = key[j];
+        if (k == kKeySentinel) continue;
+        const bool strad = (k == 0 && sA) || (k == kL && sB);
+        const uint32_t cnt = strad ? min(offs[k + 1], phi) - max(offs[k], plo) : nn[j];
+        const double m = S[j] / (double)cnt;
+        double dd = wf[j] - m;
+        const double thr = eps * fabs(m);
+        if (fabs(dd) < thr) {
+            qv[j] = thr * thr;
+            dv[j] = 0.0;
+        } else {
+            qv[j] = dd * dd;
+            dv[j] = dd;
+        }
+    }
+    seg_prefix2(key, qv, dv, lane);
+    // Segment ends, one element at a time: element j sits at index 0 of the
+    // rotating windows.
+#pragma unroll 1
+    for (int j = 0; j < kK; ++j) {
+        const int k = key[0];
+        if (endf[0] && k != kKeySentinel) {
+            const bool strad = (k == 0 && sA) || (k == kL && sB);
+            const uint32_t h = hf + k;
+            if (!strad) {
+                const double n = (double)nn[0];
+                const double Q = qv[0], D = dv[0];
+                const double varc = (n * Q - D * D) / (n - 1.);
+                dd_add(acc.am, S[0] + D);
+                dd_add(acc.av, varc);
+                // sigf_h = |var_h|^(beta/2) (pyx:2183-2186); in exact mode a
+                // separate FP64 pass raises it to the power
+                if (g.adaptive_strat) {
+                    if (g.fast_pow) {
+                        const double sv = pow_fast(fabs(varc), g.beta_half);
+                        io.sigf[h] = sv;
+                        dd_add(acc.as, sv);
+                    } else {
+                        io.sigf[h] = fabs(varc);
+                    }
+                }
+                if (g.train_errors) io.errv[h] = fabs(varc);
+            }
+            // cut hcubes: the tile's first segment and its last one
+            const bool slA = k == 0, slB = pl0 + j == phi - 1;
+            if (slA || slB) {
+                Slot sl{-1, 0u, 0.0, 0.0, 0.0};
+                if (strad && (slA ? sA : kL != 0)) {
+                    sl.h = (int32_t)h;
+                    sl.n = min(offs[k + 1], phi) - max(offs[k], plo);
+                    sl.S = S[0];
+                    sl.Q = qv[0];
+                    sl.D = dv[0];
+                }
+                io.slots[2 * (size_t)t + (slA ? 0 : 1)] = sl;
+                if (slA && slB) io.slots[2 * (size_t)t + 1].h = -1;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kK - 1; ++q) {
+            key[q] = key[q + 1];
+            endf[q] = endf[q + 1];
+            nn[q] = nn[q + 1];
+            S[q] = S[q + 1];
+            qv[q] = qv[q + 1];
+            dv[q] = dv[q + 1];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Adds the LDS u16 pair-packed counts into n_f and clears them.
+__device__ inline void flush_counts(uint32_t* Lc, int tot, double* n_f, bool clear) {
+    for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) {
+        const uint32_t wv = Lc[i];
+        if (wv) {
+            if (wv & 0xffffu) unsafeAtomicAdd(&n_f[2 * i], (double)(wv & 0xffffu));
+            if ((wv >> 16) && 2 * i + 1 < tot) unsafeAtomicAdd(&n_f[2 * i + 1], (double)(wv >> 16));
+            if (clear) Lc[i] = 0u;
+        }
+    }
+}
+
+// Workgroup reduction of the wave accumulators into out[0..5] (DD mean,
+// var, sigf sums, fixed order) and the run-wide max |w f n| / NaN flag.
+__device__ inline void block_close(TileAcc& acc, double* Lred, double* out, Scalars* sc) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+    acc.am = wave_sum_dd(acc.am);
+    acc.av = wave_sum_dd(acc.av);
+    acc.as = wave_sum_dd(acc.as);
+    const double wmax = wave_max(acc.wmax);
+    const int anynan = __any(acc.nan_seen);
+    if (lane == 0) {
+        Lred[wave * 6 + 0] = acc.am.hi; Lred[wave * 6 + 1] = acc.am.lo;
+        Lred[wave * 6 + 2] = acc.av.hi; Lred[wave * 6 + 3] = acc.av.lo;
+        Lred[wave * 6 + 4] = acc.as.hi; Lred[wave * 6 + 5] = acc.as.lo;
+        if (wmax > 0.0) atomicMax((unsigned long long*)&sc->wfn_max_bits,
+                                  (unsigned long long)__double_as_longlong(wmax));
+        if (anynan) atomicOr(&sc->status, (int32_t)VGX_ENAN);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD m{0, 0}, v{0, 0}, s{0, 0};
+        for (int k = 0; k < nwave; ++k) {
+            m = dd_sum(m, DD{Lred[k * 6 + 0], Lred[k * 6 + 1]});
+            v = dd_sum(v, DD{Lred[k * 6 + 2], Lred[k * 6 + 3]});
+            s = dd_sum(s, DD{Lred[k * 6 + 4], Lred[k * 6 + 5]});
+        }
+        out[0] = m.hi; out[1] = m.lo; out[2] = v.hi; out[3] = v.lo; out[4] = s.hi; out[5] = s.lo;
+    }
+}
+
 template <int DIM, class Fcn>
 __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedPtrs io, Fcn fcn) {
-    constexpr bool R32 = Fcn::kRng32;
     extern __shared__ __align__(16) unsigned char smem[];
     Scalars* sc = io.sc;
     if (sc->status) return;  // NaN seen earlier in this run: nothing more is done
+    const TileGeom tg = tile_geom(g);
     const int dim = DIM > 0 ? DIM : g.dim;
     const uint32_t ntiles = sc->b_ntiles, p0 = sc->b_p0, p1 = sc->b_p1, hend = sc->b_h1;
     const int tot = dim * g.nb;
@@ -208,246 +529,24 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     double* Lred = reinterpret_cast<double*>(((uintptr_t)cur + 15) & ~(uintptr_t)15);
     __syncthreads();
 
+    const TileIO tio{io.off, io.sigf, io.slots, io.errv, io.sum_f, io.n_f};
     const uint32_t chunk = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t tb = blockIdx.x * chunk;
     const uint32_t te = min(tb + chunk, ntiles);
     const uint32_t nrounds = (chunk + nwave - 1) / nwave;
+    const uint32_t off_end = io.off[hend];
 
-    DD am{0.0, 0.0}, av{0.0, 0.0}, as{0.0, 0.0};
-    double wmax = 0.0;
-    int nan_seen = 0;
-    const double eps = kEpsilon;
-
+    TileAcc acc{{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, 0.0, 0};
     for (uint32_t r = 0; r < nrounds; ++r) {
         const uint32_t t = tb + r * nwave + wave;
         if (t < te) {
-            // ------------------------------------------------ one wave tile
             const uint32_t plo = p0 + t * kT;
-            const uint32_t phi = min(plo + (uint32_t)kT, p1);
-            const uint32_t hf = (uint32_t)io.tile_h[t];
-            for (int i = lane; i < kNoff; i += kWave) {
-                uint32_t hh = hf + i;
-                offs[i] = io.off[hh < hend ? hh : hend];
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // hcube (local index) of this lane's first point
-            const uint32_t pl0 = plo + lane * kK;
-            int li = 0;
-            {
-                int lo = 0, hi = kNoff - 2;  // largest i with offs[i] <= pl0
-                while (lo < hi) {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (offs[mid] <= pl0) lo = mid; else hi = mid - 1;
-                }
-                li = lo;
-            }
-            // Points are evaluated one at a time (no unrolling: one point's
-            // registers live at once); results rotate through a register
-            // window so element j ends in key[j], wf[j], nn[j].
-            int key[kK];
-            double wf[kK];
-            uint32_t nn[kK];
-#pragma unroll
-            for (int j = 0; j < kK; ++j) {
-                key[j] = kKeySentinel;
-                wf[j] = 0.0;
-                nn[j] = 1;
-            }
-            int cur_i = -1;
-            double wdiv = 0.0;
-            uint32_t ncur = 1;
-#pragma unroll 1
-            for (int j = 0; j < kK; ++j) {
-                const uint32_t pj = pl0 + j;
-                int kj = kKeySentinel;
-                double vj = 0.0;
-                uint32_t nj = 1;
-                if (pj < phi) {
-                    while (li < kNoff - 2 && offs[li + 1] <= pj) ++li;
-                    if (li != cur_i) {
-                        cur_i = li;
-                        ncur = offs[li + 1] - offs[li];
-                        wdiv = g.dv_y / (double)ncur;
-                    }
-                    kj = li;
-                    nj = ncur;
-                    double x[DIM > 0 ? DIM : VGX_MAXDIM];
-                    int kb[DIM > 0 ? DIM : VGX_MAXDIM];
-                    double J;
-                    eval_point<DIM, R32>(g, ax, G, hf + li, pj, x, J, kb);
-                    const double w = J * wdiv;
-                    const double f = fcn.template eval<DIM>(x, dim, pj);
-                    if (f != f) nan_seen = 1;
-                    vj = w * f;
-                    const double tn = vj * (double)ncur;
-                    wmax = fmax(wmax, fabs(tn));
-                    if (g.train_points) {
-                        const double c2 = tn * tn;
-#pragma unroll
-                        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
-                            if (DIM == 0 && d >= dim) break;
-                            const int k = kb[d];
-                            if (k < 0) continue;
-                            const int idx = d * g.nb + k;
-                            if (g.hist_lds) {
-                                unsafeAtomicAdd(&Ls[idx], c2);
-                                atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
-                            } else {
-                                unsafeAtomicAdd(&io.sum_f[idx], c2);
-                                unsafeAtomicAdd(&io.n_f[idx], 1.0);
-                            }
-                        }
-                    }
-                }
-                // rotate: element j enters at the back of the window
-#pragma unroll
-                for (int q = 0; q < kK - 1; ++q) {
-                    key[q] = key[q + 1];
-                    wf[q] = wf[q + 1];
-                    nn[q] = nn[q + 1];
-                }
-                key[kK - 1] = kj;
-                wf[kK - 1] = vj;
-                nn[kK - 1] = nj;
-            }
-            // ---- pass 1: S per segment (pyx:2147-2152)
-            double P[kK];
-            seg_prefix(key, wf, P, lane);
-            int endf[kK];
-            const int next_first = __shfl_down(key[0], 1, kWave);
-#pragma unroll
-            for (int j = 0; j < kK - 1; ++j) endf[j] = key[j + 1] != key[j];
-            endf[kK - 1] = (lane == kWave - 1) || (next_first != key[kK - 1]);
-            // Total of the lane's last segment.  If it ends in this lane it
-            // is P[kK-1]; otherwise it ends inside the first run of the
-            // nearest later lane whose first run ends there: copy that run's
-            // closing prefix backwards across the lanes that only continue.
-            double fv = P[0];
-#pragma unroll
-            for (int j = 1; j < kK; ++j)
-                if (key[j] == key[0]) fv = P[j];
-            double rv = fv;
-            int rf = (key[0] != key[kK - 1]) || endf[kK - 1];
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const double nv = __shfl_down(rv, o, kWave);
-                const int nf = __shfl_down(rf, o, kWave);
-                if (!rf && lane + o < kWave) {
-                    rv = nv;
-                    rf = nf;
-                }
-            }
-            const double next_rv = __shfl_down(rv, 1, kWave);
-            const double vlast = endf[kK - 1] ? P[kK - 1] : next_rv;
-            double S[kK];
-            {
-                double c = vlast;
-#pragma unroll
-                for (int j = kK - 1; j >= 0; --j) {
-                    if (j < kK - 1 && endf[j]) c = P[j];
-                    S[j] = c;
-                }
-            }
-            // straddling segments: first (local 0) and last (kL)
-            int kL;
-            {
-                int lo = 0, hi = kNoff - 2;  // largest i with offs[i] < phi
-                while (lo < hi) {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (offs[mid] < phi) lo = mid; else hi = mid - 1;
-                }
-                kL = lo;
-            }
-            const bool sA = offs[0] < plo || offs[1] > phi;
-            const bool sB = offs[kL + 1] > phi;
-            // ---- pass 2: guarded deviations (pyx:2153-2169).  A cut hcube's
-            // part uses the part's own mean; the straddler kernel combines.
-            double qv[kK], dv[kK];
-#pragma unroll
-            for (int j = 0; j < kK; ++j) {
-                qv[j] = 0.0;
-                dv[j] = 0.0;
-                const int k = key[j];
-                if (k == kKeySentinel) continue;
-                const bool strad = (k == 0 && sA) || (k == kL && sB);
-                const uint32_t cnt = strad ? min(offs[k + 1], phi) - max(offs[k], plo) : nn[j];
-                const double m = S[j] / (double)cnt;
-                double dd = wf[j] - m;
-                const double thr = eps * fabs(m);
-                if (fabs(dd) < thr) {
-                    qv[j] = thr * thr;
-                    dv[j] = 0.0;
-                } else {
-                    qv[j] = dd * dd;
-                    dv[j] = dd;
-                }
-            }
-            seg_prefix2(key, qv, dv, lane);
-            // Segment ends, one element at a time: element j sits at index 0
-            // of the rotating windows.
-#pragma unroll 1
-            for (int j = 0; j < kK; ++j) {
-                const int k = key[0];
-                if (endf[0] && k != kKeySentinel) {
-                    const bool strad = (k == 0 && sA) || (k == kL && sB);
-                    const uint32_t h = hf + k;
-                    if (!strad) {
-                        const double n = (double)nn[0];
-                        const double Q = qv[0], D = dv[0];
-                        const double varc = (n * Q - D * D) / (n - 1.);
-                        dd_add(am, S[0] + D);
-                        dd_add(av, varc);
-                        // sigf_h = |var_h|^(beta/2) (pyx:2183-2186); in exact
-                        // mode a separate FP64 pass raises it to the power
-                        if (g.adaptive_strat) {
-                            if (g.fast_pow) {
-                                const double sv = pow_fast(fabs(varc), g.beta_half);
-                                io.sigf[h] = sv;
-                                dd_add(as, sv);
-                            } else {
-                                io.sigf[h] = fabs(varc);
-                            }
-                        }
-                        if (g.train_errors) io.errv[h] = fabs(varc);
-                    }
-                    // cut hcubes: the tile's first segment and its last one
-                    const bool slA = k == 0, slB = pl0 + j == phi - 1;
-                    if (slA || slB) {
-                        Slot sl{-1, 0u, 0.0, 0.0, 0.0};
-                        if (strad && (slA ? sA : kL != 0)) {
-                            sl.h = (int32_t)h;
-                            sl.n = min(offs[k + 1], phi) - max(offs[k], plo);
-                            sl.S = S[0];
-                            sl.Q = qv[0];
-                            sl.D = dv[0];
-                        }
-                        io.slots[2 * (size_t)t + (slA ? 0 : 1)] = sl;
-                        if (slA && slB) io.slots[2 * (size_t)t + 1].h = -1;
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < kK - 1; ++q) {
-                    key[q] = key[q + 1];
-                    endf[q] = endf[q + 1];
-                    nn[q] = nn[q + 1];
-                    S[q] = S[q + 1];
-                    qv[q] = qv[q + 1];
-                    dv[q] = dv[q + 1];
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
+            const TilePos tp{t, plo, min(plo + (uint32_t)kT, p1), (uint32_t)io.tile_h[t], hend, off_end, 0u, 0u};
+            tile_body<DIM>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
         }
         if (g.hist_lds && g.train_points && ((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
             __syncthreads();
-            for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) {
-                const uint32_t wv = Lc[i];
-                if (wv) {
-                    if (wv & 0xffffu) unsafeAtomicAdd(&io.n_f[2 * i], (double)(wv & 0xffffu));
-                    if (wv >> 16) unsafeAtomicAdd(&io.n_f[2 * i + 1], (double)(wv >> 16));
-                    Lc[i] = 0u;
-                }
-            }
+            flush_counts(Lc, tot, io.n_f, true);
             __syncthreads();
         }
     }
@@ -459,36 +558,9 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
             const double s = Ls[i];
             if (s != 0.0) unsafeAtomicAdd(&io.sum_f[i], s);
         }
-        for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) {
-            const uint32_t wv = Lc[i];
-            if (wv & 0xffffu) unsafeAtomicAdd(&io.n_f[2 * i], (double)(wv & 0xffffu));
-            if ((wv >> 16) && 2 * i + 1 < tot) unsafeAtomicAdd(&io.n_f[2 * i + 1], (double)(wv >> 16));
-        }
+        flush_counts(Lc, tot, io.n_f, false);
     }
-    am = wave_sum_dd(am);
-    av = wave_sum_dd(av);
-    as = wave_sum_dd(as);
-    wmax = wave_max(wmax);
-    const int anynan = __any(nan_seen);
-    if (lane == 0) {
-        Lred[wave * 6 + 0] = am.hi; Lred[wave * 6 + 1] = am.lo;
-        Lred[wave * 6 + 2] = av.hi; Lred[wave * 6 + 3] = av.lo;
-        Lred[wave * 6 + 4] = as.hi; Lred[wave * 6 + 5] = as.lo;
-        if (wmax > 0.0) atomicMax((unsigned long long*)&sc->wfn_max_bits,
-                                  (unsigned long long)__double_as_longlong(wmax));
-        if (anynan) atomicOr(&sc->status, (int32_t)VGX_ENAN);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        DD m{0, 0}, v{0, 0}, s{0, 0};
-        for (int k = 0; k < nwave; ++k) {
-            m = dd_sum(m, DD{Lred[k * 6 + 0], Lred[k * 6 + 1]});
-            v = dd_sum(v, DD{Lred[k * 6 + 2], Lred[k * 6 + 3]});
-            s = dd_sum(s, DD{Lred[k * 6 + 4], Lred[k * 6 + 5]});
-        }
-        double* o = io.blk_part + 6 * (size_t)blockIdx.x;
-        o[0] = m.hi; o[1] = m.lo; o[2] = v.hi; o[3] = v.lo; o[4] = s.hi; o[5] = s.lo;
-    }
+    block_close(acc, Lred, io.blk_part + 6 * (size_t)blockIdx.x, sc);
 }
 
 template <int DIM, class Fcn>

@@ -16,6 +16,10 @@ constexpr int kStradThreads = 256;  // straddler kernel: one lane per tile slot
 constexpr int kLaneParts = 8;       // cut hcubes of up to this many tiles: one lane; more: the wave
 constexpr int kKeySentinel = 0x7fff;
 
+// Register-array bound of a dimension template argument: DIM > 0 is exact;
+// DIM <= 0 is a runtime dimension of at most -DIM (all 64 for DIM = 0).
+__host__ __device__ constexpr int nd_max(int DIM) { return DIM > 0 ? DIM : (DIM < 0 ? -DIM : VGX_MAXDIM); }
+
 // Kernel-argument copy of the iteration geometry (Integrator.set, pyx:1331-1446).
 struct IterGeom {
     int dim, ld, nb;
@@ -106,8 +110,8 @@ struct GaussMix64 {
         for (int k = 0; k < ncomp; ++k) {
             double r2 = 0.0;
 #pragma unroll
-            for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
-                if (DIM == 0 && d >= nd) break;
+            for (int d = 0; d < nd_max(DIM); ++d) {
+                if (DIM <= 0 && d >= nd) break;
                 double t = x[d] - c[k * nd + d];
                 r2 += t * t;
             }
@@ -126,18 +130,18 @@ struct GaussMix32 {
     template <int DIM>
     __device__ double eval(const double* x, int dim, uint32_t) const {
         const int nd = DIM > 0 ? DIM : dim;
-        float xf[DIM > 0 ? DIM : VGX_MAXDIM];
+        float xf[nd_max(DIM)];
 #pragma unroll
-        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
-            if (DIM == 0 && d >= nd) break;
+        for (int d = 0; d < nd_max(DIM); ++d) {
+            if (DIM <= 0 && d >= nd) break;
             xf[d] = (float)x[d];
         }
         float acc = 0.0f;
         for (int k = 0; k < ncomp; ++k) {
             float r2 = 0.0f;
 #pragma unroll
-            for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
-                if (DIM == 0 && d >= nd) break;
+            for (int d = 0; d < nd_max(DIM); ++d) {
+                if (DIM <= 0 && d >= nd) break;
                 float t = xf[d] - c[k * nd + d];
                 r2 = __builtin_fmaf(t, t, r2);
             }
@@ -158,19 +162,19 @@ struct DiagRidge32 {
     template <int DIM>
     __device__ double eval(const double* x, int dim, uint32_t) const {
         const int nd = DIM > 0 ? DIM : dim;
-        float xf[DIM > 0 ? DIM : VGX_MAXDIM];
+        float xf[nd_max(DIM)];
         float s = 0.0f;
 #pragma unroll
-        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
-            if (DIM == 0 && d >= nd) break;
+        for (int d = 0; d < nd_max(DIM); ++d) {
+            if (DIM <= 0 && d >= nd) break;
             xf[d] = (float)x[d];
             s += xf[d];
         }
         const float xbar = s / (float)nd;
         float P = 0.0f;
 #pragma unroll
-        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
-            if (DIM == 0 && d >= nd) break;
+        for (int d = 0; d < nd_max(DIM); ++d) {
+            if (DIM <= 0 && d >= nd) break;
             float t = xf[d] - xbar;
             P = __builtin_fmaf(t, t, P);
         }
@@ -196,15 +200,15 @@ struct DiagRidge64 {
         const int nd = DIM > 0 ? DIM : dim;
         double s = 0.0;
 #pragma unroll
-        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
-            if (DIM == 0 && d >= nd) break;
+        for (int d = 0; d < nd_max(DIM); ++d) {
+            if (DIM <= 0 && d >= nd) break;
             s += x[d];
         }
         const double xbar = s / (double)nd;
         double P = 0.0;
 #pragma unroll
-        for (int d = 0; d < (DIM > 0 ? DIM : VGX_MAXDIM); ++d) {
-            if (DIM == 0 && d >= nd) break;
+        for (int d = 0; d < nd_max(DIM); ++d) {
+            if (DIM <= 0 && d >= nd) break;
             double t = x[d] - xbar;
             P += t * t;
         }
