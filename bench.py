@@ -38,7 +38,60 @@ def parse():
     p.add_argument("--cpu-iters", type=int, default=5)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 512 MiB device to device 4 times (HBM counter calibration)")
+    p.add_argument("--workload", default="double_peak_8d",
+                   choices=["double_peak_8d", "batched_4096"],
+                   help="double_peak_8d (default): the headline; batched_4096: 4096 independent "
+                        "Gaussian integrals in one graph, nitn 8 per call (--steps calls timed)")
+    p.add_argument("--no-tune", action="store_true", help="batched: skip the work-item autotuner")
     return p.parse_args()
+
+
+def batched(a):
+    """4096 independent single-Gaussian integrals (dim 2-10, sigma 0.01-0.1,
+    neval 1e4-1e6 log-uniform), 8 iterations per call, one hipGraph."""
+    import numpy as np
+    import torch
+    from vegas_amd import workloads
+    from vegas_amd.batch import BatchIntegrator, gaussian_params
+    probs = workloads.batched_gaussians(4096)
+    dims = [p["dim"] for p in probs]
+    nev = [p["neval"] for p in probs]
+    cs = [p["center"] for p in probs]
+    sig = [p["sigma"] for p in probs]
+    exact = np.array([p["exact"] for p in probs])
+    A, B = gaussian_params(dims, sig)
+    bi = BatchIntegrator(dims, nev)
+    tuned = None
+    if not a.no_tune:
+        tuned = bi.autotune(A, B, cs, nitn=2)
+    for _ in range(max(a.warmup, 1)):
+        bi(A, B, cs, nitn=8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        bi.launch(A, B, cs, nitn=8)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    r = bi(A, B, cs, nitn=8)
+    evals = float(r.sum_neval)  # per call (the timed calls draw the same counts to within the allocation)
+    pull = (r.mean - exact) / np.where(r.sdev > 0, r.sdev, np.inf)
+    # plain vegas+ misses some narrow high-dimensional peaks at 1e4 samples
+    # (as the reference does): those return ~0 with a tiny error
+    resolved = r.mean > 0.5 * exact
+    out = {
+        "metric": METRIC, "value": evals * a.steps / dt, "unit": "evals/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": dt * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (device Philox samples)",
+        "config": {"workload": "4096 independent Gaussian integrals, dim U{2..10}, sigma log-U[0.01,0.1], "
+                               "neval log-U[1e4,1e6], ninc <= 500, nitn 8 per step (one hipGraph launch)",
+                   "integrals": 4096, "evals_per_step": int(evals), "tiles_per_item": bi.tiles_per_item,
+                   "autotune_ms": getattr(bi, "tune_times", None), "parallelism": "single GPU"},
+        "result": {"integrals": len(exact), "resolved": int(resolved.sum()),
+                   "resolved_within_5sigma": int((np.abs(pull[resolved]) < 5).sum()),
+                   "median_rel_sdev_resolved": float(np.median(r.sdev[resolved] / exact[resolved]))},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
 
 
 def copy_calibration():
@@ -77,6 +130,9 @@ def main():
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.workload == "batched_4096":
+        batched(a)
         return
     import torch
 

@@ -224,7 +224,8 @@ typedef struct {
 /* Caller-owned device state (sizes from vgx_batch_sizes). */
 typedef struct {
     double* grid;        /* [n][VGX_BATCH_MAXDIM][VGX_BATCH_LD] */
-    double* sigf;        /* [total hcubes], integral i at hcube_base[i] */
+    double* sigf;        /* [total]: integral i's hcubes from hcube_base[i],
+                            one spare entry after each integral */
     double* sum_sigf;    /* [n] */
     double* itn_mean;    /* [nitn][n] */
     double* itn_var;     /* [nitn][n] */
@@ -235,7 +236,7 @@ typedef struct {
 
 int vgx_batch_create(vgx_ctx* ctx, const vgx_batch_config* cfg, vgx_batch** out);
 int vgx_batch_destroy(vgx_batch* b);
-/* total hcubes, and hcube_base[n] (host, may be NULL). */
+/* Length of sigf (sum of nhcube_i + 1), and hcube_base[n] (host, may be NULL). */
 int vgx_batch_sizes(const vgx_batch* b, int64_t* total_hcubes, int64_t* hcube_base);
 /* Integrand parameters: A, B device double[n]; c device double[n][MAXDIM]. */
 int vgx_batch_integrate(vgx_batch* b, const double* A, const double* B, const double* c,

@@ -197,8 +197,8 @@ class BatchIntegrator:
     def _params(self, A, B, centers):
         torch = self._torch
         dev = self.grid.device
-        A = np.broadcast_to(np.asarray(A, np.float64), (self.n,))
-        B = np.broadcast_to(np.asarray(B, np.float64), (self.n,))
+        A = np.array(np.broadcast_to(np.asarray(A, np.float64), (self.n,)))
+        B = np.array(np.broadcast_to(np.asarray(B, np.float64), (self.n,)))
         c = np.zeros((self.n, MAXDIM))
         for i, ci in enumerate(centers):
             ci = np.asarray(ci, np.float64)

@@ -288,6 +288,9 @@ __global__ __launch_bounds__(kAllocThreads) void b_offsets_kernel(BArgs a) {
     for (int i = 0; i < kAllocItems; ++i) ln[b_pad(threadIdx.x * kAllocItems + i)] = loc[i];
     __syncthreads();
     for (int i = threadIdx.x; i < cnt; i += kAllocThreads) a.off[q.hbase + hs + i] = ln[b_pad(i)];
+    // the integral's end offset follows its last hcube, so off[h + 1] is the
+    // end of hcube h for every h of the integral
+    if (hs + cnt == q.nhcube && threadIdx.x == 0) a.off[q.hbase + q.nhcube] = pl.P + pl.T;
 }
 
 // ---------------------------------------------------------------- fused ----
@@ -577,7 +580,7 @@ extern "C" int vgx_batch_create(vgx_ctx* ctx, const vgx_batch_config* c, vgx_bat
         for (int k = 0; k < q.nblk; ++k) blk.push_back(int2{i, k * kAllocPerBlock});
         for (int d = 0; d < dim; ++d) adt.push_back(int2{i, d});
         b->hbase[i] = H;
-        H += nh;
+        H += nh + 1;  // one more offset entry: the integral's end
         // the allocation's largest point count (as max_points of one integral)
         const uint64_t cap = (uint64_t)(q.neval_sig * (1.0 + 1e-9)) + 2 + (uint64_t)nh * (uint64_t)nmin;
         const uint64_t tiles = (cap + kT - 1) / kT;

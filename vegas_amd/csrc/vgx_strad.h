@@ -82,6 +82,11 @@ static __global__ __launch_bounds__(kStradThreads) void strad_kernel(IterGeom g,
                 own = lo >= p0 + tf * kT;  // the hcube starts in this tile
                 tl = (hi - 1 - p0) / kT;
                 nh = hi - lo;
+                // a part list must end inside the batch's tiles
+                if (own && (hi <= lo || tl >= ntiles)) {
+                    atomicOr(&a.sc->status, (int32_t)VGX_ERANGE);
+                    own = false;
+                }
             }
         }
         const bool in_lane = own && tl - tf < (uint32_t)kLaneParts;
