@@ -244,6 +244,58 @@ int vgx_batch_integrate(vgx_batch* b, const double* A, const double* B, const do
 /* The next call's first iteration number (Philox stream): calls continue it. */
 int vgx_batch_set_iteration(vgx_batch* b, uint32_t iteration);
 
+/* ---------------------------------------------------------------------- */
+/* Learned map: an affine-coupling normalizing flow on [0,1]^dim (dim <= 15,
+ * nlayers <= 8), z -> u0 = logit(z) -> couplings -> x = sigmoid(u).  Layer
+ * k conditions on the dims in amask[k]; its conditioner is the MLP
+ * (dim -> 128 -> 128 -> 2 dim) with bf16 weights and FP32 accumulation on
+ * MFMA; the log-determinant is FP32.  No counterpart in the reference, whose
+ * only map is the separable grid (pyx:39-832); the flow stands in for it on
+ * non-separable integrands, and is used only when its measured per-sample
+ * variance is lower.  Parameters (FP32 master, per layer:
+ * W1[128][16] (column 15 is the bias) | W2[128][128] | b2[128] | W3[32][128]
+ * | b3[32]; rows 0..15 of W3 give s, rows 16..31 give t).
+ * Points are float [n][16] (dims padded to 16).                           */
+typedef struct vgx_flow vgx_flow;
+int vgx_flow_create(int dim, int nlayers, const uint32_t* amask, vgx_flow** out);
+int vgx_flow_destroy(vgx_flow* fl);
+int vgx_flow_nparams(const vgx_flow* fl, int64_t* n);
+/* host float[nparams]; resets the Adam moments */
+int vgx_flow_set_params(vgx_flow* fl, const float* host, void* stream);
+/* host copies of the parameters and of the last gradient (either may be NULL) */
+int vgx_flow_get_params(const vgx_flow* fl, float* host, float* grad_host, void* stream);
+/* Importance-sampling estimate: n points z (Philox: key seed, counter
+ * (base + i, iteration, lane group)), x = F(z), f(x) J(z) summed:
+ * sums[0] = sum f J, sums[1] = sum (f J)^2 (host).  fcn: GAUSSMIX or
+ * DIAGRIDGE with device FP64 parameters (evaluated in FP32).  xout, fout
+ * (device, optional): the points and f J.  status: device int32, set to
+ * VGX_ENAN when f is NaN.                                                 */
+int vgx_flow_sample(vgx_flow* fl, const vgx_integrand* fcn, int64_t n, uint64_t base, uint64_t seed,
+                    uint32_t iteration, double* sums, float* xout, float* fout, int32_t* status,
+                    void* stream);
+/* log q(x) (device float[n]) of points x (device float[n][16]); u0
+ * (device float[n][16], optional): logit(z) of their preimages.           */
+int vgx_flow_logq(vgx_flow* fl, const float* x, int64_t n, float* logq, float* u0, void* stream);
+/* Gradient of  L = -sum_i c_i log q(x_i)  for samples x (device
+ * float[n][16]) with integrand values f and log sampling weights log w
+ * (device float[n]), c normalised to sum 1:
+ *   VGX_FLOW_LOSS_VARIANCE  c_i ~ w_i f_i^2 / q(x_i): the gradient of
+ *                           sum_i w_i f_i^2 / q(x_i) = Var(f/q) + I^2 (est.),
+ *                           up to a positive factor;
+ *   VGX_FLOW_LOSS_KL        c_i ~ w_i |f_i|: forward KL from |f|/I to q.
+ * Kept on the device for vgx_flow_adam (vgx_flow_get_params reads it).   */
+enum { VGX_FLOW_LOSS_VARIANCE = 0, VGX_FLOW_LOSS_KL = 1 };
+int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const float* logw, int64_t n, int loss,
+                  void* stream);
+/* One Adam step with the last gradient; bf16 copies refreshed.            */
+int vgx_flow_adam(vgx_flow* fl, float lr, float beta1, float beta2, float eps, float clip, void* stream);
+/* f at points x (device float[n][16]) into f (device float[n]).           */
+int vgx_flow_eval(vgx_flow* fl, const vgx_integrand* fcn, const float* x, int64_t n, float* f,
+                  void* stream);
+/* D = A B for 16x16 bf16 A, B (device, row-major) on one MFMA: checks the
+ * instruction's lane layout.                                              */
+int vgx_flow_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

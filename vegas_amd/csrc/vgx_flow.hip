@@ -1,0 +1,1083 @@
+// vgx_flow.hip -- the learned map: an affine-coupling normalizing flow on
+// [0,1]^D (D <= 15) with hand-written BF16 MFMA kernels (gfx950
+// v_mfma_f32_16x16x16_bf16, FP32 accumulation, FP32 log-det).
+//
+// Map (the same arithmetic as oracle/flow_oracle.py): z -> u0 = logit(z) ->
+// L couplings -> x = sigmoid(uL).  Coupling k conditions on the set A_k:
+//   a = (u_d, d in A_k; 0 else; a_15 = 1),  h1 = relu(W1 a),
+//   h2 = relu(W2 h1 + b2),  o = W3 h2 + b3,  s = 4 tanh(o_0..15 / 4),  t = o_16..31
+//   u_B <- u_B e^s + t (forward),  u_B <- (u_B - t) e^-s (inverse).
+//
+// MFMA orientation.  Every GEMM is computed transposed, C[out][sample] =
+// W[out][in] X[in][sample], on tiles of 16 samples.  With the 16x16x16
+// instruction, lane l holds C[(l/16)*4 + i][l%16] (i < 4) and the B operand
+// X[(l/16)*4 + j][l%16]: one layer's output tile mt, after bias, ReLU and the
+// bf16 rounding, is the next layer's k-step mt in registers, with no data
+// movement.  The coupling state u (dims (l/16)*4 + i of sample l%16) sits in
+// the same lanes as s (output tile 0) and t (output tile 1), so the update is
+// lane-local; per-sample sums reduce across the four lane groups.
+//
+// Kernels:
+//   flow_sample   z (Philox) -> x, log J, f(x), sums of f J and (f J)^2: the
+//                 estimator; layers streamed through LDS, samples in registers
+//   flow_inverse  x -> z, log q(x), each layer's input saved (training)
+//   flow_lossw    variance-loss weights c_i ~ w_i f_i^2 / q(x_i), normalised
+//   flow_backward one layer: recompute, backprop, write the GEMM operands
+//   flow_wgrad    weight gradients of one layer: MFMA over the sample axis
+//   flow_adam     Adam on the FP32 master weights, bf16 copies refreshed
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "vgx_internal.h"
+
+namespace vgx {
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kFH = 128;             // hidden width
+constexpr int kFI = 16;              // padded input width, column 15 = layer-1 bias
+constexpr int kFO = 32;              // output rows: s 0..15, t 16..31
+constexpr int kFMaxL = 8;            // layers
+constexpr float kSMax = 4.0f;
+// FP32 master parameters of one layer: W1 | W2 | b2 | W3 | b3
+constexpr int kPW1 = 0, kPW2 = kPW1 + kFH * kFI, kPB2 = kPW2 + kFH * kFH, kPW3 = kPB2 + kFH,
+              kPB3 = kPW3 + kFO * kFH, kPLayer = kPB3 + kFO;
+// bf16 weights of one layer: W1 | W2 | W3 (and the transposes W1t | W2t | W3t)
+constexpr int kHW1 = 0, kHW2 = kHW1 + kFH * kFI, kHW3 = kHW2 + kFH * kFH, kHLayer = kHW3 + kFO * kFH;
+// backward GEMM operands of one 16-sample tile: rows of 16 bf16
+constexpr int kRa = 0, kRh1 = 16, kRh2 = kRh1 + kFH, kRdo = kRh2 + kFH, kRdh2 = kRdo + kFO,
+              kRdh1 = kRdh2 + kFH, kRows = kRdh1 + kFH;  // 560 rows
+
+__device__ inline uint16_t f2bf(float f) {
+    uint32_t b = __float_as_uint(f);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ inline bf16x4_t pack_bf(const f32x4_t v) {
+    bf16x4_t r;
+    r[0] = (short)f2bf(v[0]);
+    r[1] = (short)f2bf(v[1]);
+    r[2] = (short)f2bf(v[2]);
+    r[3] = (short)f2bf(v[3]);
+    return r;
+}
+
+__device__ inline f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// A operand fragment of W (bf16 row-major, `in` columns): rows 16 mt ..,
+// columns 16 kt + (l/16)*4 .. +3.
+__device__ inline bf16x4_t frag_a(const uint16_t* W, int in, int mt, int kt, int lane) {
+    return *reinterpret_cast<const bf16x4_t*>(W + (16 * mt + (lane & 15)) * in + 16 * kt + (lane >> 4) * 4);
+}
+
+__device__ inline float logsigp(float u) {
+    const float au = fabsf(u);
+    return -au - 2.0f * __logf(1.0f + __expf(-au));
+}
+
+__device__ inline float tanh_f(float x) {
+    const float e = __expf(-2.0f * fabsf(x));
+    const float t = (1.0f - e) / (1.0f + e);
+    return copysignf(t, x);
+}
+
+__device__ inline float group_sum(float v) {
+    v += __shfl_xor(v, 16, kWave);
+    v += __shfl_xor(v, 32, kWave);
+    return v;
+}
+
+struct FlowArgs {
+    int D, L;
+    uint32_t amask[kFMaxL];       // conditioning set of each layer (bit d)
+    const float* P;               // FP32 master parameters [L][kPLayer]
+    const uint16_t* H;            // bf16 weights [L][kHLayer]
+    const uint16_t* HT;           // bf16 transposes [L][kHLayer]
+};
+
+// One layer's weights and biases in LDS.
+struct LayerLds {
+    uint16_t W[kHLayer];
+    float b2[kFH], b3[kFO];
+};
+
+__device__ inline void load_layer(const FlowArgs& f, int k, LayerLds* s, bool transposed, uint16_t* Wt) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(f.H + (size_t)k * kHLayer);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(s->W);
+    for (int i = threadIdx.x; i < kHLayer / 2; i += blockDim.x) dst[i] = src[i];
+    if (transposed) {
+        const uint32_t* st = reinterpret_cast<const uint32_t*>(f.HT + (size_t)k * kHLayer);
+        uint32_t* dt = reinterpret_cast<uint32_t*>(Wt);
+        for (int i = threadIdx.x; i < kHLayer / 2; i += blockDim.x) dt[i] = st[i];
+    }
+    const float* p = f.P + (size_t)k * kPLayer;
+    for (int i = threadIdx.x; i < kFH; i += blockDim.x) s->b2[i] = p[kPB2 + i];
+    for (int i = threadIdx.x; i < kFO; i += blockDim.x) s->b3[i] = p[kPB3 + i];
+}
+
+// The conditioning input a of one tile as the B operand (dims (l/16)*4 + j).
+__device__ inline bf16x4_t cond_input(const f32x4_t u, uint32_t amask, int D, int lane) {
+    f32x4_t a;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int d = (lane >> 4) * 4 + j;
+        a[j] = (d < D && ((amask >> d) & 1u)) ? u[j] : (d == kFI - 1 ? 1.0f : 0.0f);
+    }
+    return pack_bf(a);
+}
+
+// The conditioner MLP of one tile: o[0] = raw s, o[1] = t rows (l/16)*4+i.
+// Optionally keeps h1, h2 (bf16 fragments) and their ReLU masks.
+struct MlpKeep {
+    bf16x4_t h1[8], h2[8];
+    uint32_t m1, m2;   // bit 4 mt + i: pre-activation > 0
+};
+
+template <bool KEEP>
+__device__ inline void mlp(const LayerLds* s, bf16x4_t a, int lane, f32x4_t* o, MlpKeep* keep) {
+    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+    bf16x4_t h1[8];
+    uint32_t m1 = 0, m2 = 0;
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt) {
+        f32x4_t c = mfma16(frag_a(s->W + kHW1, kFI, mt, 0, lane), a, zero);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            m1 |= (c[i] > 0.f ? 1u : 0u) << (4 * mt + i);
+            c[i] = fmaxf(c[i], 0.f);
+        }
+        h1[mt] = pack_bf(c);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    bf16x4_t h2[8];
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt) {
+        f32x4_t c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i] = s->b2[16 * mt + (lane >> 4) * 4 + i];
+#pragma unroll
+        for (int kt = 0; kt < 8; ++kt) c = mfma16(frag_a(s->W + kHW2, kFH, mt, kt, lane), h1[kt], c);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            m2 |= (c[i] > 0.f ? 1u : 0u) << (4 * mt + i);
+            c[i] = fmaxf(c[i], 0.f);
+        }
+        h2[mt] = pack_bf(c);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        f32x4_t c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i] = s->b3[16 * mt + (lane >> 4) * 4 + i];
+#pragma unroll
+        for (int kt = 0; kt < 8; ++kt) c = mfma16(frag_a(s->W + kHW3, kFH, mt, kt, lane), h2[kt], c);
+        o[mt] = c;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (KEEP) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            keep->h1[q] = h1[q];
+            keep->h2[q] = h2[q];
+        }
+        keep->m1 = m1;
+        keep->m2 = m2;
+    }
+}
+
+// ------------------------------------------------------------- sampling ----
+constexpr int kFlowThreads = 1024;
+constexpr int kFlowTiles = 4;        // sample tiles (16 samples) per wave per pass
+constexpr int kFlowPass = (kFlowThreads / 64) * kFlowTiles * 16;  // 1024 samples
+
+struct FlowFcn {
+    int kind;            // VGX_FCN_GAUSSMIX or VGX_FCN_DIAGRIDGE
+    int ncomp;
+    const float* A;
+    const float* B;
+    const float* c;      // GAUSSMIX: [ncomp][D]; DIAGRIDGE: [ncomp]
+};
+
+struct SampleArgs {
+    uint64_t n;          // samples
+    uint64_t base;       // first sample index (Philox counter)
+    uint32_t seed_lo, seed_hi, iteration;
+    double* part;        // [gridDim.x][4]: DD sum f J, DD sum (f J)^2
+    float* xout;         // optional [n][16] x, and fout [n] = f J
+    float* fout;
+    int32_t* status;
+};
+
+// f at x for the sample of this lane (dims (l/16)*4 + i), all four lanes of
+// the sample cooperating; every lane of the sample returns f.
+__device__ inline float flow_fcn(const FlowFcn& F, const float* Lc, f32x4_t x, int D, int lane) {
+    const int g = lane >> 4;
+    float acc = 0.f;
+    if (F.kind == VGX_FCN_DIAGRIDGE) {
+        float s1 = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (4 * g + i < D) s1 += x[i];
+        const float xbar = group_sum(s1) / (float)D;
+        float p = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (4 * g + i < D) p += (x[i] - xbar) * (x[i] - xbar);
+        const float P = group_sum(p);
+        const float fd = (float)D;
+        for (int k = g; k < F.ncomp; k += 4) {
+            const float t = xbar - Lc[2 * F.ncomp + k];
+            acc = __builtin_fmaf(Lc[k], __expf(-Lc[F.ncomp + k] * __builtin_fmaf(fd * t, t, P)), acc);
+        }
+        return group_sum(acc);
+    }
+    for (int k = 0; k < F.ncomp; ++k) {
+        float r = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (4 * g + i < D) {
+                const float t = x[i] - Lc[2 * F.ncomp + k * D + 4 * g + i];
+                r = __builtin_fmaf(t, t, r);
+            }
+        r = group_sum(r);
+        acc = __builtin_fmaf(Lc[k], __expf(-Lc[F.ncomp + k] * r), acc);
+    }
+    return acc;
+}
+
+// Per-wave tile state in LDS (kFlowTiles tiles: u as float4 per lane, and
+// the log-Jacobian), so that tiles are indexed at run time without scratch.
+struct TileState {
+    f32x4_t u[kFlowTiles][64];
+    float l[kFlowTiles][64];
+};
+constexpr size_t kStateLds = 16 * sizeof(TileState);
+
+__global__ __launch_bounds__(kFlowThreads) void flow_sample_kernel(FlowArgs f, FlowFcn F, SampleArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    LayerLds* L = reinterpret_cast<LayerLds*>(smem);
+    TileState* TS = reinterpret_cast<TileState*>(smem + ((sizeof(LayerLds) + 15) & ~(size_t)15)) +
+                    (threadIdx.x >> 6);
+    float* Lc = reinterpret_cast<float*>(smem + ((sizeof(LayerLds) + 15) & ~(size_t)15) + kStateLds);
+    __shared__ double red[16 * 4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4;
+    const int ncp = F.kind == VGX_FCN_DIAGRIDGE ? 3 * F.ncomp : F.ncomp * (2 + f.D);
+    for (int i = threadIdx.x; i < ncp; i += blockDim.x)
+        Lc[i] = i < F.ncomp ? F.A[i] : i < 2 * F.ncomp ? F.B[i - F.ncomp] : F.c[i - 2 * F.ncomp];
+    DD s1{0, 0}, s2{0, 0};
+    for (uint64_t pb = (uint64_t)blockIdx.x * kFlowPass; pb < a.n; pb += (uint64_t)gridDim.x * kFlowPass) {
+#pragma unroll 1
+        for (int q = 0; q < kFlowTiles; ++q) {
+            const uint64_t sidx = a.base + pb + (uint64_t)(wave * kFlowTiles + q) * 16 + (lane & 15);
+            const U4 w = philox4x32_10(U4{(uint32_t)sidx, (uint32_t)(sidx >> 32), a.iteration, (uint32_t)g},
+                                       a.seed_lo, a.seed_hi);
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+            float l = 0.f;
+            f32x4_t uq;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float z = (float)(ws[i] >> 8) * (1.0f / 16777216.0f) + (1.0f / 33554432.0f);
+                const float uu = __logf(z) - __logf(1.0f - z);
+                uq[i] = uu;
+                if (4 * g + i < f.D) l -= logsigp(uu);
+            }
+            TS->u[q][lane] = uq;
+            TS->l[q][lane] = l;
+        }
+        for (int k = 0; k < f.L; ++k) {
+            __syncthreads();
+            load_layer(f, k, L, false, nullptr);
+            __syncthreads();
+            const uint32_t am = f.amask[k];
+#pragma unroll 1
+            for (int q = 0; q < kFlowTiles; ++q) {
+                f32x4_t uq = TS->u[q][lane];
+                float l = TS->l[q][lane];
+                f32x4_t o[2];
+                mlp<false>(L, cond_input(uq, am, f.D, lane), lane, o, nullptr);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int d = 4 * g + i;
+                    if (d < f.D && !((am >> d) & 1u)) {
+                        const float sv = kSMax * tanh_f(o[0][i] / kSMax);
+                        uq[i] = uq[i] * __expf(sv) + o[1][i];
+                        l += sv;
+                    }
+                }
+                TS->u[q][lane] = uq;
+                TS->l[q][lane] = l;
+            }
+        }
+#pragma unroll 1
+        for (int q = 0; q < kFlowTiles; ++q) {
+            const f32x4_t uq = TS->u[q][lane];
+            f32x4_t x;
+            float l = TS->l[q][lane];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                x[i] = 1.0f / (1.0f + __expf(-uq[i]));
+                if (4 * g + i < f.D) l += logsigp(uq[i]);
+            }
+            const float LJ = group_sum(l);
+            const float fv = flow_fcn(F, Lc, x, f.D, lane);
+            const uint64_t sl = pb + (uint64_t)(wave * kFlowTiles + q) * 16 + (lane & 15);
+            const bool live = sl < a.n;
+            // f = 0 where the map stretches enough for J to overflow is 0
+            const double fj = (live && fv != 0.f) ? (double)fv * exp((double)LJ) : 0.0;
+            if (fv != fv && live && a.status) atomicOr(a.status, (int32_t)VGX_ENAN);
+            if (g == 0 && live) {
+                dd_add(s1, fj);
+                dd_add(s2, fj * fj);
+            }
+            if (a.xout && live) {
+                *reinterpret_cast<f32x4_t*>(a.xout + sl * 16 + 4 * g) = x;
+                if (g == 0) a.fout[sl] = (float)fj;
+            }
+        }
+    }
+    s1 = wave_sum_dd(s1);
+    s2 = wave_sum_dd(s2);
+    if (lane == 0) {
+        red[4 * wave] = s1.hi;
+        red[4 * wave + 1] = s1.lo;
+        red[4 * wave + 2] = s2.hi;
+        red[4 * wave + 3] = s2.lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD t1{0, 0}, t2{0, 0};
+        for (int w = 0; w < 16; ++w) {
+            t1 = dd_sum(t1, DD{red[4 * w], red[4 * w + 1]});
+            t2 = dd_sum(t2, DD{red[4 * w + 2], red[4 * w + 3]});
+        }
+        double* o = a.part + 4 * (size_t)blockIdx.x;
+        o[0] = t1.hi; o[1] = t1.lo; o[2] = t2.hi; o[3] = t2.lo;
+    }
+}
+
+// The integrand at given points x [n][16] (training data): f [n].
+__global__ __launch_bounds__(256) void flow_eval_kernel(FlowFcn F, int D, const float* x, uint64_t n, float* fout) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* Lc = reinterpret_cast<float*>(smem);
+    const int ncp = F.kind == VGX_FCN_DIAGRIDGE ? 3 * F.ncomp : F.ncomp * (2 + D);
+    for (int i = threadIdx.x; i < ncp; i += blockDim.x)
+        Lc[i] = i < F.ncomp ? F.A[i] : i < 2 * F.ncomp ? F.B[i - F.ncomp] : F.c[i - 2 * F.ncomp];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const uint64_t ntiles = (n + 15) / 16;
+    for (uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < ntiles;
+         t += (uint64_t)gridDim.x * (blockDim.x >> 6)) {
+        const uint64_t sl = t * 16 + (lane & 15);
+        f32x4_t xv = {0.5f, 0.5f, 0.5f, 0.5f};
+        if (sl < n) xv = *reinterpret_cast<const f32x4_t*>(x + sl * 16 + 4 * g);
+        const float fv = flow_fcn(F, Lc, xv, D, lane);
+        if (g == 0 && sl < n) fout[sl] = fv;
+    }
+}
+
+// -------------------------------------------------------------- inverse ----
+struct InvArgs {
+    uint64_t n;
+    const float* x;      // [n][16]
+    float* saved;        // [L][n][16]: input of each layer
+    float* u0;           // [n][16]
+    float* logq;         // [n]
+};
+
+__global__ __launch_bounds__(kFlowThreads) void flow_inverse_kernel(FlowArgs f, InvArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    LayerLds* L = reinterpret_cast<LayerLds*>(smem);
+    TileState* TS = reinterpret_cast<TileState*>(smem + ((sizeof(LayerLds) + 15) & ~(size_t)15)) +
+                    (threadIdx.x >> 6);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4;
+    for (uint64_t pb = (uint64_t)blockIdx.x * kFlowPass; pb < a.n; pb += (uint64_t)gridDim.x * kFlowPass) {
+#pragma unroll 1
+        for (int q = 0; q < kFlowTiles; ++q) {
+            const uint64_t sl = pb + (uint64_t)(wave * kFlowTiles + q) * 16 + (lane & 15);
+            f32x4_t xv = {0.5f, 0.5f, 0.5f, 0.5f};
+            if (sl < a.n) xv = *reinterpret_cast<const f32x4_t*>(a.x + sl * 16 + 4 * g);
+            float l = 0.f;
+            f32x4_t uq;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float xc = fminf(fmaxf(xv[i], 1e-7f), 1.0f - 1e-7f);
+                const float uu = __logf(xc) - __logf(1.0f - xc);
+                uq[i] = (4 * g + i < f.D) ? uu : 0.f;
+                if (4 * g + i < f.D) l -= logsigp(uu);
+            }
+            TS->u[q][lane] = uq;
+            TS->l[q][lane] = l;
+        }
+        for (int k = f.L - 1; k >= 0; --k) {
+            __syncthreads();
+            load_layer(f, k, L, false, nullptr);
+            __syncthreads();
+            const uint32_t am = f.amask[k];
+#pragma unroll 1
+            for (int q = 0; q < kFlowTiles; ++q) {
+                const uint64_t sl = pb + (uint64_t)(wave * kFlowTiles + q) * 16 + (lane & 15);
+                f32x4_t uq = TS->u[q][lane];
+                float l = TS->l[q][lane];
+                if (sl < a.n) *reinterpret_cast<f32x4_t*>(a.saved + ((size_t)k * a.n + sl) * 16 + 4 * g) = uq;
+                f32x4_t o[2];
+                mlp<false>(L, cond_input(uq, am, f.D, lane), lane, o, nullptr);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int d = 4 * g + i;
+                    if (d < f.D && !((am >> d) & 1u)) {
+                        const float sv = kSMax * tanh_f(o[0][i] / kSMax);
+                        uq[i] = (uq[i] - o[1][i]) * __expf(-sv);
+                        l -= sv;
+                    }
+                }
+                TS->u[q][lane] = uq;
+                TS->l[q][lane] = l;
+            }
+        }
+#pragma unroll 1
+        for (int q = 0; q < kFlowTiles; ++q) {
+            const uint64_t sl = pb + (uint64_t)(wave * kFlowTiles + q) * 16 + (lane & 15);
+            const f32x4_t uq = TS->u[q][lane];
+            float l = TS->l[q][lane];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (4 * g + i < f.D) l += logsigp(uq[i]);
+            const float LQ = group_sum(l);
+            if (sl < a.n) {
+                *reinterpret_cast<f32x4_t*>(a.u0 + sl * 16 + 4 * g) = uq;
+                if (g == 0) a.logq[sl] = LQ;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------- loss weights ----
+// log c_i = log w_i + 2 log|f_i| - log q_i (w given by its logarithm, so
+// weights 1/(n q) of the flow's own samples cannot overflow); c normalised
+// to sum 1 (two launches: the maximum, then the sum of exp(log c - max)).
+// Terms that are not finite count as 0.
+// pf, pq: exponents of |f| and q -- (2, 1) the variance loss, (1, 0) the
+// forward KL divergence (maximum likelihood of the density |f|/I).
+__device__ inline float lossw_log(const float* lw, const float* fv, const float* logq, uint64_t i, float pf,
+                                  float pq) {
+    const float fa = fabsf(fv[i]);
+    if (!(fa > 0.f)) return -INFINITY;
+    const float v = lw[i] + pf * __logf(fa) - pq * logq[i];
+    return (v == v && fabsf(v) < INFINITY) ? v : -INFINITY;
+}
+
+__device__ inline uint32_t ord_f(float v) {
+    const uint32_t b = __float_as_uint(v);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ inline float unord_f(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+__global__ void flow_lossw_max_kernel(const float* w, const float* fv, const float* logq, uint64_t n,
+                                      float pf, float pq, uint32_t* mx) {
+    float m = -INFINITY;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, lossw_log(w, fv, logq, i, pf, pq));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
+    if ((threadIdx.x & 63) == 0 && m > -INFINITY) atomicMax(mx, ord_f(m));
+}
+
+__global__ void flow_lossw_sum_kernel(const float* w, const float* fv, const float* logq, uint64_t n,
+                                      float pf, float pq, const uint32_t* mx, float* c, double* part) {
+    __shared__ double red[16];
+    const bool none = *mx == 0u;  // no sample with f != 0
+    const float m = none ? 0.f : unord_f(*mx);
+    double s = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const float v = none ? 0.f : __expf(lossw_log(w, fv, logq, i, pf, pq) - m);
+        c[i] = v;
+        s += v;
+    }
+    s = wave_sum_fixed(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += red[k];
+        part[blockIdx.x] = t;
+    }
+}
+
+// ------------------------------------------------------------- backward ----
+struct BwdArgs {
+    uint64_t n;
+    int k;                    // layer
+    const float* saved;       // [L][n][16]
+    const float* u0;          // [n][16]
+    const float* c;           // [n] unnormalised weights
+    const double* csum_part;  // [ncpart] partial sums of c
+    int ncpart;
+    float* G;                 // [n][16] dL/du (in: layer output; out: layer input)
+    uint16_t* ops;            // [ntiles][kRows][16] GEMM operands (bf16)
+};
+
+// Writes a C-layout tile (rows rt*16 + (l/16)*4 + i, sample l%16) into the
+// operand block of the sample tile: element (row, sample) at row*16 + sample.
+__device__ inline void put_rows(uint16_t* tileops, int row0, const bf16x4_t v, int lane) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) tileops[(row0 + (lane >> 4) * 4 + i) * 16 + (lane & 15)] = (uint16_t)v[i];
+}
+
+constexpr int kBwdThreads = 512;
+
+__global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, BwdArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    LayerLds* L = reinterpret_cast<LayerLds*>(smem);
+    uint16_t* Wt = reinterpret_cast<uint16_t*>(smem + sizeof(LayerLds));
+    __shared__ double csum_s;
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const int k = a.k;
+    load_layer(f, k, L, true, Wt);
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < a.ncpart; ++i) t += a.csum_part[i];
+        csum_s = t;
+    }
+    __syncthreads();
+    const float inv_c = csum_s > 0 ? (float)(1.0 / csum_s) : 0.f;
+    const uint32_t am = f.amask[k];
+    const uint64_t ntiles = (a.n + 15) / 16;
+    const int nw = blockDim.x >> 6;
+    for (uint64_t tile = (uint64_t)blockIdx.x * nw + (threadIdx.x >> 6); tile < ntiles;
+         tile += (uint64_t)gridDim.x * nw) {
+        const uint64_t sl = tile * 16 + (lane & 15);
+        const bool live = sl < a.n;
+        f32x4_t uin = {0.f, 0.f, 0.f, 0.f}, G = {0.f, 0.f, 0.f, 0.f};
+        float cw = 0.f;
+        if (live) {
+            uin = *reinterpret_cast<const f32x4_t*>(a.saved + ((size_t)k * a.n + sl) * 16 + 4 * g);
+            cw = a.c[sl] * inv_c;
+            if (k == 0) {
+                const f32x4_t u0 = *reinterpret_cast<const f32x4_t*>(a.u0 + sl * 16 + 4 * g);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    G[i] = (4 * g + i < f.D) ? -cw * (1.0f - 2.0f / (1.0f + __expf(-u0[i]))) : 0.f;
+            } else {
+                G = *reinterpret_cast<const f32x4_t*>(a.G + sl * 16 + 4 * g);
+            }
+        }
+        const bf16x4_t ain = cond_input(uin, am, f.D, lane);
+        f32x4_t o[2];
+        MlpKeep kp;
+        mlp<true>(L, ain, lane, o, &kp);
+        // coupling backward (inverse direction)
+        f32x4_t dout0, dout1, Gin;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int d = 4 * g + i;
+            dout0[i] = 0.f;
+            dout1[i] = 0.f;
+            Gin[i] = G[i];
+            if (d < f.D && !((am >> d) & 1u)) {
+                const float th = tanh_f(o[0][i] / kSMax);
+                const float sv = kSMax * th;
+                const float es = __expf(-sv);
+                const float uout = (uin[i] - o[1][i]) * es;
+                const float ds = -G[i] * uout + cw;
+                dout0[i] = live ? ds * (1.0f - th * th) : 0.f;
+                dout1[i] = live ? -G[i] * es : 0.f;
+                Gin[i] = G[i] * es;
+            }
+        }
+        const bf16x4_t do0 = pack_bf(dout0), do1 = pack_bf(dout1);
+        // dh2 = W3^T dout (relu mask h2); dh1 = W2^T dh2 (mask h1); da = W1^T dh1
+        const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+        bf16x4_t dh2[8];
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) {
+            f32x4_t cc = mfma16(frag_a(Wt + kHW3, kFO, mt, 0, lane), do0, zero);
+            cc = mfma16(frag_a(Wt + kHW3, kFO, mt, 1, lane), do1, cc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (!((kp.m2 >> (4 * mt + i)) & 1u)) cc[i] = 0.f;
+            dh2[mt] = pack_bf(cc);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        bf16x4_t dh1[8];
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) {
+            f32x4_t cc = zero;
+#pragma unroll
+            for (int kt = 0; kt < 8; ++kt) cc = mfma16(frag_a(Wt + kHW2, kFH, mt, kt, lane), dh2[kt], cc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (!((kp.m1 >> (4 * mt + i)) & 1u)) cc[i] = 0.f;
+            dh1[mt] = pack_bf(cc);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        f32x4_t da = zero;
+#pragma unroll
+        for (int kt = 0; kt < 8; ++kt) da = mfma16(frag_a(Wt + kHW1, kFH, 0, kt, lane), dh1[kt], da);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int d = 4 * g + i;
+            if (d < f.D && ((am >> d) & 1u)) Gin[i] += da[i];
+        }
+        if (live) *reinterpret_cast<f32x4_t*>(a.G + sl * 16 + 4 * g) = Gin;
+        // GEMM operands of the weight gradients
+        uint16_t* to = a.ops + (size_t)tile * kRows * 16;
+        put_rows(to, kRa, ain, lane);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            put_rows(to, kRh1 + 16 * q, kp.h1[q], lane);
+            put_rows(to, kRh2 + 16 * q, kp.h2[q], lane);
+            put_rows(to, kRdh2 + 16 * q, dh2[q], lane);
+            put_rows(to, kRdh1 + 16 * q, dh1[q], lane);
+        }
+        put_rows(to, kRdo, do0, lane);
+        put_rows(to, kRdo + 16, do1, lane);
+    }
+}
+
+// ---------------------------------------------------------- weight grads ----
+// dW[ra][rb] = sum_samples A[ra][s] B[rb][s] over the operand tiles, with a
+// virtual all-ones B row tile for the biases.  8 waves: wave w accumulates
+// dW2 rows 16w.. (8 tiles + bias), dW1 rows 16w.. (1 tile) and, for w < 2,
+// dW3 rows 16w.. (8 tiles + bias).  Partial sums per workgroup.
+constexpr int kWgThreads = 512;
+constexpr int kWgChunk = 256;   // sample tiles per workgroup
+
+__device__ inline bf16x4_t frag_op(const uint16_t* tileops, int row0, int lane) {
+    return *reinterpret_cast<const bf16x4_t*>(tileops + (row0 + (lane & 15)) * 16 + (lane >> 4) * 4);
+}
+
+__global__ __launch_bounds__(kWgThreads) void flow_wgrad_kernel(const uint16_t* ops, uint64_t ntiles, float* part) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+    f32x4_t acc2[9], acc1, acc3[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        acc2[q] = zero;
+        acc3[q] = zero;
+    }
+    acc1 = zero;
+    bf16x4_t ones;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ones[j] = (lane & 15) == 0 ? (short)0x3F80 : (short)0;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kWgChunk, t1 = min(t0 + kWgChunk, ntiles);
+    for (uint64_t t = t0; t < t1; ++t) {
+        const uint16_t* to = ops + (size_t)t * kRows * 16;
+        const bf16x4_t a2 = frag_op(to, kRdh2 + 16 * w, lane);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc2[q] = mfma16(a2, frag_op(to, kRh1 + 16 * q, lane), acc2[q]);
+        acc2[8] = mfma16(a2, ones, acc2[8]);
+        acc1 = mfma16(frag_op(to, kRdh1 + 16 * w, lane), frag_op(to, kRa, lane), acc1);
+        if (w < 2) {
+            const bf16x4_t a3 = frag_op(to, kRdo + 16 * w, lane);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc3[q] = mfma16(a3, frag_op(to, kRh2 + 16 * q, lane), acc3[q]);
+            acc3[8] = mfma16(a3, ones, acc3[8]);
+        }
+    }
+    // C layout: lane l, i -> dW[16 w + (l/16)*4 + i][16 q + l%16]
+    float* o = part + (size_t)blockIdx.x * kPLayer;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = 16 * w + (lane >> 4) * 4 + i;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) o[kPW2 + r * kFH + 16 * q + (lane & 15)] = acc2[q][i];
+        if ((lane & 15) == 0) o[kPB2 + r] = acc2[8][i];
+        o[kPW1 + r * kFI + (lane & 15)] = acc1[i];
+        if (w < 2) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) o[kPW3 + r * kFH + 16 * q + (lane & 15)] = acc3[q][i];
+            if ((lane & 15) == 0) o[kPB3 + r] = acc3[8][i];
+        }
+    }
+}
+
+// Sum of the workgroup partials (fixed order) into grad [kPLayer].
+__global__ void flow_wgrad_reduce_kernel(const float* part, int nparts, float* grad) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kPLayer; i += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int p = 0; p < nparts; ++p) s += part[(size_t)p * kPLayer + i];
+        grad[i] = (float)s;
+    }
+}
+
+// ------------------------------------------------------------------ adam ----
+struct AdamArgs {
+    float* P;
+    const float* grad;
+    float* m;
+    float* v;
+    uint16_t* H;
+    uint16_t* HT;
+    int L;
+    float lr, b1, b2, eps, c1, c2;   // c1 = 1 - b1^t, c2 = 1 - b2^t
+    float clip;                       // gradient clip (max abs), 0 = none
+};
+
+__global__ void flow_adam_kernel(AdamArgs a) {
+    const int total = a.L * kPLayer;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        float gr = a.grad[i];
+        if (!(fabsf(gr) < INFINITY)) gr = 0.f;  // a non-finite component is skipped
+        if (a.clip > 0) gr = fminf(fmaxf(gr, -a.clip), a.clip);
+        const float m = a.b1 * a.m[i] + (1.f - a.b1) * gr;
+        const float v = a.b2 * a.v[i] + (1.f - a.b2) * gr * gr;
+        a.m[i] = m;
+        a.v[i] = v;
+        a.P[i] = a.P[i] - a.lr * (m / a.c1) / (sqrtf(v / a.c2) + a.eps);
+    }
+}
+
+// bf16 copies (and transposes) of the weights after an update.
+__global__ void flow_cast_kernel(const float* P, uint16_t* H, uint16_t* HT, int L) {
+    const int total = L * kHLayer;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int k = i / kHLayer, r = i % kHLayer;
+        const float* p = P + (size_t)k * kPLayer;
+        float v;
+        int rt;
+        if (r < kHW2) {          // W1 [128][16] -> W1t [16][128]
+            const int o = r / kFI, c = r % kFI;
+            v = p[kPW1 + r];
+            rt = kHW1 + c * kFH + o;
+        } else if (r < kHW3) {   // W2 [128][128]
+            const int q = r - kHW2, o = q / kFH, c = q % kFH;
+            v = p[kPW2 + q];
+            rt = kHW2 + c * kFH + o;
+        } else {                 // W3 [32][128] -> W3t [128][32]
+            const int q = r - kHW3, o = q / kFH, c = q % kFH;
+            v = p[kPW3 + q];
+            rt = kHW3 + c * kFO + o;
+        }
+        const uint16_t h = f2bf(v);
+        H[(size_t)k * kHLayer + r] = h;
+        HT[(size_t)k * kHLayer + rt] = h;
+    }
+}
+
+__global__ void flow_zero_u32_kernel(uint32_t* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+// MFMA lane-layout probe: D = A B for 16x16 bf16 A, B given row-major.
+__global__ void flow_mfma_probe_kernel(const uint16_t* A, const uint16_t* B, float* D) {
+    const int lane = threadIdx.x & 63;
+    bf16x4_t a, b;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        a[j] = (short)A[(lane & 15) * 16 + (lane >> 4) * 4 + j];
+        b[j] = (short)B[((lane >> 4) * 4 + j) * 16 + (lane & 15)];
+    }
+    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+    const f32x4_t c = mfma16(a, b, zero);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) D[((lane >> 4) * 4 + i) * 16 + (lane & 15)] = c[i];
+}
+
+}  // namespace vgx
+
+// ------------------------------------------------------------ host side ----
+using namespace vgx;
+
+struct vgx_flow {
+    int D = 0, L = 0;
+    uint32_t amask[kFMaxL] = {};
+    float* P = nullptr;      // FP32 master [L][kPLayer]
+    float* m = nullptr;
+    float* v = nullptr;
+    float* grad = nullptr;
+    uint16_t* H = nullptr;
+    uint16_t* HT = nullptr;
+    // per-batch scratch (grown on demand)
+    Scratch saved, u0, logq, c, G, ops, wpart, cpart, spart, misc, fpar;
+    int step = 0;
+};
+
+static FlowArgs flow_args(const vgx_flow* fl) {
+    FlowArgs f{};
+    f.D = fl->D;
+    f.L = fl->L;
+    for (int k = 0; k < fl->L; ++k) f.amask[k] = fl->amask[k];
+    f.P = fl->P;
+    f.H = fl->H;
+    f.HT = fl->HT;
+    return f;
+}
+
+static size_t lds_layer() { return (sizeof(LayerLds) + 15) & ~(size_t)15; }
+
+static int flow_cast(vgx_flow* fl, hipStream_t s) {
+    hipLaunchKernelGGL(flow_cast_kernel, dim3(256), dim3(256), 0, s, (const float*)fl->P, fl->H, fl->HT, fl->L);
+    return check_launch();
+}
+
+extern "C" int vgx_flow_create(int dim, int nlayers, const uint32_t* amask, vgx_flow** out) {
+    if (!out || !amask || dim < 1 || dim > kFI - 1 || nlayers < 1 || nlayers > kFMaxL) return VGX_EINVAL;
+    vgx_flow* fl = new vgx_flow();
+    fl->D = dim;
+    fl->L = nlayers;
+    for (int k = 0; k < nlayers; ++k) fl->amask[k] = amask[k] & ((1u << dim) - 1u);
+    const size_t np = (size_t)nlayers * kPLayer, nh = (size_t)nlayers * kHLayer;
+    if (hipMalloc(&fl->P, np * 4) != hipSuccess || hipMalloc(&fl->m, np * 4) != hipSuccess ||
+        hipMalloc(&fl->v, np * 4) != hipSuccess || hipMalloc(&fl->grad, np * 4) != hipSuccess ||
+        hipMalloc(&fl->H, nh * 2) != hipSuccess || hipMalloc(&fl->HT, nh * 2) != hipSuccess) {
+        delete fl;
+        return VGX_ENOMEM;
+    }
+    if (hipMemset(fl->P, 0, np * 4) != hipSuccess || hipMemset(fl->m, 0, np * 4) != hipSuccess ||
+        hipMemset(fl->v, 0, np * 4) != hipSuccess || hipMemset(fl->grad, 0, np * 4) != hipSuccess)
+        return VGX_EHIP;
+    if (hipFuncSetAttribute((const void*)flow_backward_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(lds_layer() + sizeof(uint16_t) * kHLayer)) != hipSuccess ||
+        hipFuncSetAttribute((const void*)flow_inverse_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(lds_layer() + kStateLds)) != hipSuccess)
+        return VGX_EHIP;
+    *out = fl;
+    return VGX_OK;
+}
+
+extern "C" int vgx_flow_destroy(vgx_flow* fl) {
+    if (!fl) return VGX_OK;
+    (void)hipDeviceSynchronize();
+    void* ps[] = {fl->P, fl->m, fl->v, fl->grad, fl->H, fl->HT};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    Scratch* ss[] = {&fl->saved, &fl->u0, &fl->logq, &fl->c, &fl->G, &fl->ops, &fl->wpart, &fl->cpart, &fl->spart,
+                     &fl->misc, &fl->fpar};
+    for (Scratch* q : ss) q->release();
+    delete fl;
+    return VGX_OK;
+}
+
+extern "C" int vgx_flow_nparams(const vgx_flow* fl, int64_t* n) {
+    if (!fl || !n) return VGX_EINVAL;
+    *n = (int64_t)fl->L * kPLayer;
+    return VGX_OK;
+}
+
+extern "C" int vgx_flow_set_params(vgx_flow* fl, const float* host, void* stream) {
+    if (!fl || !host) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t np = (size_t)fl->L * kPLayer;
+    VGX_HIP(hipMemcpyAsync(fl->P, host, np * 4, hipMemcpyHostToDevice, s));
+    VGX_HIP(hipMemsetAsync(fl->m, 0, np * 4, s));
+    VGX_HIP(hipMemsetAsync(fl->v, 0, np * 4, s));
+    fl->step = 0;
+    int rc = flow_cast(fl, s);
+    if (rc) return rc;
+    VGX_HIP(hipStreamSynchronize(s));
+    return VGX_OK;
+}
+
+extern "C" int vgx_flow_get_params(const vgx_flow* fl, float* host, float* grad_host, void* stream) {
+    if (!fl) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t np = (size_t)fl->L * kPLayer;
+    if (host) VGX_HIP(hipMemcpyAsync(host, fl->P, np * 4, hipMemcpyDeviceToHost, s));
+    if (grad_host) VGX_HIP(hipMemcpyAsync(grad_host, fl->grad, np * 4, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    return VGX_OK;
+}
+
+// Device integrand parameters converted to FP32 into fl->fpar.
+static int flow_fcn_params(vgx_flow* fl, const vgx_integrand* fcn, FlowFcn* F, hipStream_t s) {
+    if (!fcn || (fcn->kind != VGX_FCN_GAUSSMIX && fcn->kind != VGX_FCN_DIAGRIDGE) || fcn->ncomp < 1)
+        return VGX_EINVAL;
+    const int nc = fcn->ncomp;
+    const int ncc = fcn->kind == VGX_FCN_GAUSSMIX ? nc * fl->D : nc;
+    const size_t tot = (size_t)2 * nc + ncc;
+    if (tot * 4 > 48 * 1024) return VGX_ERANGE;  // staged in LDS with the layer
+    int rc = fl->fpar.ensure(tot * 4);
+    if (rc) return rc;
+    float* d = (float*)fl->fpar.ptr;
+    std::vector<double> h(tot);
+    VGX_HIP(hipMemcpyAsync(h.data(), fcn->A, nc * 8, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipMemcpyAsync(h.data() + nc, fcn->B, nc * 8, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipMemcpyAsync(h.data() + 2 * nc, fcn->c, ncc * 8, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    std::vector<float> hf(tot);
+    for (size_t i = 0; i < tot; ++i) hf[i] = (float)h[i];
+    VGX_HIP(hipMemcpyAsync(d, hf.data(), tot * 4, hipMemcpyHostToDevice, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    F->kind = fcn->kind;
+    F->ncomp = nc;
+    F->A = d;
+    F->B = d + nc;
+    F->c = d + 2 * nc;
+    return VGX_OK;
+}
+
+extern "C" int vgx_flow_sample(vgx_flow* fl, const vgx_integrand* fcn, int64_t n, uint64_t base, uint64_t seed,
+                               uint32_t iteration, double* sums, float* xout, float* fout, int32_t* status,
+                               void* stream) {
+    if (!fl || !sums || n < 1) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    FlowFcn F{};
+    int rc = flow_fcn_params(fl, fcn, &F, s);
+    if (rc) return rc;
+    DeviceInfo info;
+    if ((rc = device_info(&info))) return rc;
+    const int blocks = (int)std::min<int64_t>((int64_t)info.cus, (n + kFlowPass - 1) / kFlowPass);
+    if ((rc = fl->spart.ensure((size_t)blocks * 4 * 8))) return rc;
+    SampleArgs a{};
+    a.n = (uint64_t)n;
+    a.base = base;
+    a.seed_lo = (uint32_t)seed;
+    a.seed_hi = (uint32_t)(seed >> 32);
+    a.iteration = iteration;
+    a.part = (double*)fl->spart.ptr;
+    a.xout = xout;
+    a.fout = fout;
+    a.status = status;
+    const size_t ncp = F.kind == VGX_FCN_DIAGRIDGE ? 3 * (size_t)F.ncomp : (size_t)F.ncomp * (2 + fl->D);
+    const size_t lds = lds_layer() + kStateLds + ncp * 4;
+    if (hipFuncSetAttribute((const void*)flow_sample_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+        return VGX_EHIP;
+    hipLaunchKernelGGL(flow_sample_kernel, dim3(blocks), dim3(kFlowThreads), lds, s, flow_args(fl), F, a);
+    if ((rc = check_launch())) return rc;
+    std::vector<double> h((size_t)blocks * 4);
+    VGX_HIP(hipMemcpyAsync(h.data(), a.part, h.size() * 8, hipMemcpyDeviceToHost, s));
+    VGX_HIP(hipStreamSynchronize(s));
+    DD t1{0, 0}, t2{0, 0};
+    for (int b = 0; b < blocks; ++b) {
+        t1 = dd_sum(t1, DD{h[4 * b], h[4 * b + 1]});
+        t2 = dd_sum(t2, DD{h[4 * b + 2], h[4 * b + 3]});
+    }
+    sums[0] = t1.hi + t1.lo;
+    sums[1] = t2.hi + t2.lo;
+    return VGX_OK;
+}
+
+static int flow_inverse(vgx_flow* fl, const float* x, int64_t n, hipStream_t s) {
+    int rc;
+    const size_t nn = (size_t)n;
+    if ((rc = fl->saved.ensure((size_t)fl->L * nn * 64))) return rc;
+    if ((rc = fl->u0.ensure(nn * 64))) return rc;
+    if ((rc = fl->logq.ensure(nn * 4))) return rc;
+    DeviceInfo info;
+    if ((rc = device_info(&info))) return rc;
+    const int blocks = (int)std::min<int64_t>((int64_t)info.cus * 2, (n + kFlowPass - 1) / kFlowPass);
+    InvArgs a{(uint64_t)n, x, (float*)fl->saved.ptr, (float*)fl->u0.ptr, (float*)fl->logq.ptr};
+    hipLaunchKernelGGL(flow_inverse_kernel, dim3(blocks), dim3(kFlowThreads), lds_layer() + kStateLds, s,
+                       flow_args(fl), a);
+    return check_launch();
+}
+
+extern "C" int vgx_flow_logq(vgx_flow* fl, const float* x, int64_t n, float* logq, float* u0, void* stream) {
+    if (!fl || !x || !logq || n < 1) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = flow_inverse(fl, x, n, s);
+    if (rc) return rc;
+    VGX_HIP(hipMemcpyAsync(logq, fl->logq.ptr, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    if (u0) VGX_HIP(hipMemcpyAsync(u0, fl->u0.ptr, (size_t)n * 64, hipMemcpyDeviceToDevice, s));
+    return VGX_OK;
+}
+
+extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const float* logw, int64_t n,
+                             int loss, void* stream) {
+    if (!fl || !x || !f || !logw || n < 1) return VGX_EINVAL;
+    if (loss != VGX_FLOW_LOSS_VARIANCE && loss != VGX_FLOW_LOSS_KL) return VGX_EINVAL;
+    const float* w = logw;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = flow_inverse(fl, x, n, s);
+    if (rc) return rc;
+    const size_t nn = (size_t)n;
+    const uint64_t ntiles = (nn + 15) / 16;
+    const int nc = 1024;
+    if ((rc = fl->c.ensure(nn * 4))) return rc;
+    if ((rc = fl->G.ensure(nn * 64))) return rc;
+    if ((rc = fl->ops.ensure((size_t)ntiles * kRows * 16 * 2))) return rc;
+    if ((rc = fl->cpart.ensure(nc * 8))) return rc;
+    if ((rc = fl->misc.ensure(256))) return rc;
+    const int nparts = (int)((ntiles + kWgChunk - 1) / kWgChunk);
+    if ((rc = fl->wpart.ensure((size_t)nparts * kPLayer * 4))) return rc;
+    uint32_t* mx = (uint32_t*)fl->misc.ptr;
+    hipLaunchKernelGGL(flow_zero_u32_kernel, dim3(1), dim3(64), 0, s, mx, 1);
+    const float pf = loss == VGX_FLOW_LOSS_KL ? 1.f : 2.f, pq = loss == VGX_FLOW_LOSS_KL ? 0.f : 1.f;
+    hipLaunchKernelGGL(flow_lossw_max_kernel, dim3(nc), dim3(256), 0, s, w, f, (const float*)fl->logq.ptr,
+                       (uint64_t)n, pf, pq, mx);
+    hipLaunchKernelGGL(flow_lossw_sum_kernel, dim3(nc), dim3(256), 0, s, w, f, (const float*)fl->logq.ptr,
+                       (uint64_t)n, pf, pq, (const uint32_t*)mx, (float*)fl->c.ptr, (double*)fl->cpart.ptr);
+    if ((rc = check_launch())) return rc;
+    DeviceInfo info;
+    if ((rc = device_info(&info))) return rc;
+    const size_t blds = lds_layer() + sizeof(uint16_t) * kHLayer;
+    for (int k = 0; k < fl->L; ++k) {
+        BwdArgs b{};
+        b.n = (uint64_t)n;
+        b.k = k;
+        b.saved = (const float*)fl->saved.ptr;
+        b.u0 = (const float*)fl->u0.ptr;
+        b.c = (const float*)fl->c.ptr;
+        b.csum_part = (const double*)fl->cpart.ptr;
+        b.ncpart = nc;
+        b.G = (float*)fl->G.ptr;
+        b.ops = (uint16_t*)fl->ops.ptr;
+        const int bb = (int)std::min<uint64_t>((uint64_t)info.cus * 2, (ntiles + 7) / 8);
+        hipLaunchKernelGGL(flow_backward_kernel, dim3(bb), dim3(kBwdThreads), blds, s, flow_args(fl), b);
+        hipLaunchKernelGGL(flow_wgrad_kernel, dim3(nparts), dim3(kWgThreads), 0, s, (const uint16_t*)fl->ops.ptr,
+                           ntiles, (float*)fl->wpart.ptr);
+        hipLaunchKernelGGL(flow_wgrad_reduce_kernel, dim3(64), dim3(256), 0, s, (const float*)fl->wpart.ptr, nparts,
+                           fl->grad + (size_t)k * kPLayer);
+        if ((rc = check_launch())) return rc;
+    }
+    return VGX_OK;
+}
+
+extern "C" int vgx_flow_adam(vgx_flow* fl, float lr, float b1, float b2, float eps, float clip, void* stream) {
+    if (!fl) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    fl->step += 1;
+    AdamArgs a{};
+    a.P = fl->P;
+    a.grad = fl->grad;
+    a.m = fl->m;
+    a.v = fl->v;
+    a.H = fl->H;
+    a.HT = fl->HT;
+    a.L = fl->L;
+    a.lr = lr;
+    a.b1 = b1;
+    a.b2 = b2;
+    a.eps = eps;
+    a.c1 = 1.f - powf(b1, (float)fl->step);
+    a.c2 = 1.f - powf(b2, (float)fl->step);
+    a.clip = clip;
+    hipLaunchKernelGGL(flow_adam_kernel, dim3(256), dim3(256), 0, s, a);
+    int rc = check_launch();
+    if (rc) return rc;
+    return flow_cast(fl, s);
+}
+
+extern "C" int vgx_flow_eval(vgx_flow* fl, const vgx_integrand* fcn, const float* x, int64_t n, float* f,
+                             void* stream) {
+    if (!fl || !x || !f || n < 1) return VGX_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    FlowFcn F{};
+    int rc = flow_fcn_params(fl, fcn, &F, s);
+    if (rc) return rc;
+    const size_t ncp = F.kind == VGX_FCN_DIAGRIDGE ? 3 * (size_t)F.ncomp : (size_t)F.ncomp * (2 + fl->D);
+    const uint64_t ntiles = ((uint64_t)n + 15) / 16;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(4096, (ntiles + 3) / 4);
+    if (hipFuncSetAttribute((const void*)flow_eval_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(ncp * 4 + 16)) != hipSuccess)
+        return VGX_EHIP;
+    hipLaunchKernelGGL(flow_eval_kernel, dim3(blocks), dim3(256), ncp * 4 + 16, s, F, fl->D, x, (uint64_t)n, f);
+    return check_launch();
+}
+
+extern "C" int vgx_flow_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, void* stream) {
+    if (!A || !B || !D) return VGX_EINVAL;
+    hipLaunchKernelGGL(flow_mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, A, B, D);
+    return check_launch();
+}

@@ -429,9 +429,9 @@ class Integrator:
         self._finish(status, res, 1, result)
 
     # -------------------------------------------------------- sampler APIs
-    def random_batch(self, yield_hcube=False, yield_y=False):
-        """Batches of integration points and weights for one iteration
-        (pyx:1601-1634, 1636-1765): yields (x, [y,] wgt[, hcube])."""
+    def _device_batches(self, yield_hcube=False, yield_y=False):
+        """Device tensors (x, [y,] wgt[, hcube]) of one iteration's points,
+        in the batches _random_batch cuts (pyx:1636-1765)."""
         torch = _lib.torch_cuda()
         L = _lib.lib()
         st, bufs = self._state(torch, 1)
@@ -452,13 +452,19 @@ class Integrator:
             hc = torch.empty(n, dtype=torch.int64, device='cuda') if yield_hcube else None
             _lib.check(L.vgx_iter_sample(cx, C.byref(cfg), C.byref(st), 0, h0, h1, p0, p1, _lib.ptr(x),
                                          _lib.ptr(y), _lib.ptr(w), _lib.ptr(hc), sp), 'iter_sample')
-            ans = (x.cpu().numpy(),)
+            ans = (x,)
             if yield_y:
-                ans += (y.cpu().numpy(),)
-            ans += (w.cpu().numpy(),)
+                ans += (y,)
+            ans += (w,)
             if yield_hcube:
-                ans += (hc.cpu().numpy(),)
+                ans += (hc,)
             yield ans
+
+    def random_batch(self, yield_hcube=False, yield_y=False):
+        """Batches of integration points and weights for one iteration
+        (pyx:1601-1634, 1636-1765): yields (x, [y,] wgt[, hcube])."""
+        for t in self._device_batches(yield_hcube, yield_y):
+            yield tuple(a.cpu().numpy() for a in t)
 
     random_vec = random_batch
 
