@@ -39,10 +39,11 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 512 MiB device to device 4 times (HBM counter calibration)")
     p.add_argument("--workload", default="double_peak_8d",
-                   choices=["double_peak_8d", "batched_4096"],
+                   choices=["double_peak_8d", "batched_4096", "flow_ridge_9d"],
                    help="double_peak_8d (default): the headline; batched_4096: 4096 independent "
                         "Gaussian integrals in one graph, nitn 8 per call (--steps calls timed)")
     p.add_argument("--no-tune", action="store_true", help="batched: skip the work-item autotuner")
+    p.add_argument("--rtol", type=float, default=1e-4, help="flow_ridge_9d: target relative error")
     return p.parse_args()
 
 
@@ -122,6 +123,64 @@ def cpu_baseline(wl, iters):
                        "the FP64 C oracle, 1 thread, %.1f s" % (iters, int(nev.sum()), dt))
 
 
+def flow_ridge(a):
+    """GPU wall-time to a relative error rtol on the 9-D ridge (500 diagonal
+    Gaussians, sigma 0.01): the learned map (vegas warm-up, 2e7 vegas
+    samples, flow training, flow sampling) against the plain vegas+ grid
+    (neval 2e7 iterations), which gets 3x the flow's time."""
+    import torch
+    import vegas_amd as va
+    from vegas_amd import workloads
+    from vegas_amd.flow import FlowMap
+    wl = workloads.ridge_9d(diagonal=True)
+    f, exact = wl["f"], wl["exact"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    integ = va.Integrator([[0, 1]] * 9, neval=wl["neval"], seed=5)
+    integ(f, nitn=10)
+    fm = FlowMap(9, seed=1)
+    fm.fit(f, integ, ntrain=2e7)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    est, nflow = fm.integrate(f, rtol=a.rtol, batch=20000000, max_samples=2e10)
+    torch.cuda.synchronize()
+    t_flow = time.perf_counter() - t0
+    flow_ok = est.sdev < a.rtol * abs(est.mean) and abs(est.mean - exact) < 5 * est.sdev
+    # plain vegas+: iterations of 2e7 until sdev < rtol |mean| or 3 t_flow
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    vi = va.Integrator([[0, 1]] * 9, neval=wl["neval"], seed=5)
+    r = None
+    nit = 0
+    while time.perf_counter() - t1 < 3 * t_flow:
+        r = vi(f, nitn=10)
+        nit += 10
+        if r.sdev < a.rtol * abs(r.mean):
+            break
+    torch.cuda.synchronize()
+    t_vegas = time.perf_counter() - t1
+    vegas_ok = r.sdev < a.rtol * abs(r.mean) and abs(r.mean - exact) < 5 * r.sdev
+    out = {
+        "metric": METRIC, "value": t_flow, "unit": "s (GPU wall-time to rel err %g, learned map)" % a.rtol,
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": t_flow * 1e3, "higher_is_better": False,
+        "scaling": "strong", "vs_baseline": None, "dtype": "bf16 (flow MLP), f32 (integrand), f64 (sums)",
+        "data": "synthetic (device Philox samples)",
+        "config": {"workload": "9-D ridge: 500 diagonal Gaussians sigma 0.01, [0,1]^9; 6-layer affine coupling "
+                               "flow, MLP 9-128-128 per layer, trained on 2e7 vegas samples in 1e6 batches",
+                   "parallelism": "single GPU"},
+        "result": {"flow": {"mean": est.mean, "sdev": est.sdev, "samples": int(nflow), "train_s": t_train,
+                            "reached_target": bool(flow_ok)},
+                   "vegas": {"mean": r.mean, "sdev": r.sdev, "iterations": nit, "seconds": t_vegas,
+                             "chi2_dof": r.chi2 / max(r.dof, 1), "reached_target": bool(vegas_ok)},
+                   "exact": exact,
+                   "speedup_vs_vegas": (t_vegas / t_flow) if vegas_ok else ">= %.1f (vegas did not reach the "
+                                                                          "target in 3x the flow's time)" % (
+                                                                              t_vegas / t_flow)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -133,6 +192,9 @@ def main():
         return
     if a.workload == "batched_4096":
         batched(a)
+        return
+    if a.workload == "flow_ridge_9d":
+        flow_ridge(a)
         return
     import torch
 
