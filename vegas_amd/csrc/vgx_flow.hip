@@ -46,25 +46,27 @@ constexpr float kSMax = 4.0f;
 // FP32 master parameters of one layer: W1 | W2 | b2 | W3 | b3
 constexpr int kPW1 = 0, kPW2 = kPW1 + kFH * kFI, kPB2 = kPW2 + kFH * kFH, kPW3 = kPB2 + kFH,
               kPB3 = kPW3 + kFO * kFH, kPLayer = kPB3 + kFO;
-// bf16 weights of one layer: W1 | W2 | W3 (and the transposes W1t | W2t | W3t)
-constexpr int kHW1 = 0, kHW2 = kHW1 + kFH * kFI, kHW3 = kHW2 + kFH * kFH, kHLayer = kHW3 + kFO * kFH;
+// bf16 weights of one layer: W1 | W2 | W3, and the transposes W1t | W2t | W3t,
+// with padded row strides so that the 16 rows of an MFMA A fragment fall in
+// distinct LDS banks (a 256-byte stride puts them all in one bank)
+constexpr int kS1 = 20, kS2 = 132, kS3 = 132;          // W1 [128][16], W2 [128][128], W3 [32][128]
+constexpr int kT1 = 132, kT2 = 132, kT3 = 36;          // W1t [16][128], W2t [128][128], W3t [128][32]
+constexpr int kHW1 = 0, kHW2 = kHW1 + kFH * kS1, kHW3 = kHW2 + kFH * kS2, kHLayer = kHW3 + kFO * kS3;
+constexpr int kTW1 = 0, kTW2 = kTW1 + kFI * kT1, kTW3 = kTW2 + kFH * kT2;
+static_assert(kTW3 + kFH * kT3 <= kHLayer, "transposed layout fits the layer block");
 // backward GEMM operands of one 16-sample tile: rows of 16 bf16
 constexpr int kRa = 0, kRh1 = 16, kRh2 = kRh1 + kFH, kRdo = kRh2 + kFH, kRdh2 = kRdo + kFO,
               kRdh1 = kRdh2 + kFH, kRows = kRdh1 + kFH;  // 560 rows
 
-__device__ inline uint16_t f2bf(float f) {
-    uint32_t b = __float_as_uint(f);
-    b += 0x7FFFu + ((b >> 16) & 1u);
-    return (uint16_t)(b >> 16);
-}
-__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// FP32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 __device__ inline bf16x4_t pack_bf(const f32x4_t v) {
     bf16x4_t r;
-    r[0] = (short)f2bf(v[0]);
-    r[1] = (short)f2bf(v[1]);
-    r[2] = (short)f2bf(v[2]);
-    r[3] = (short)f2bf(v[3]);
+    r[0] = __builtin_bit_cast(short, (__bf16)v[0]);
+    r[1] = __builtin_bit_cast(short, (__bf16)v[1]);
+    r[2] = __builtin_bit_cast(short, (__bf16)v[2]);
+    r[3] = __builtin_bit_cast(short, (__bf16)v[3]);
     return r;
 }
 
@@ -72,10 +74,10 @@ __device__ inline f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// A operand fragment of W (bf16 row-major, `in` columns): rows 16 mt ..,
+// A operand fragment of W (bf16 row-major, row stride `ld`): rows 16 mt ..,
 // columns 16 kt + (l/16)*4 .. +3.
-__device__ inline bf16x4_t frag_a(const uint16_t* W, int in, int mt, int kt, int lane) {
-    return *reinterpret_cast<const bf16x4_t*>(W + (16 * mt + (lane & 15)) * in + 16 * kt + (lane >> 4) * 4);
+__device__ inline bf16x4_t frag_a(const uint16_t* W, int ld, int mt, int kt, int lane) {
+    return *reinterpret_cast<const bf16x4_t*>(W + (16 * mt + (lane & 15)) * ld + 16 * kt + (lane >> 4) * 4);
 }
 
 __device__ inline float logsigp(float u) {
@@ -148,41 +150,50 @@ __device__ inline void mlp(const LayerLds* s, bf16x4_t a, int lane, f32x4_t* o, 
     uint32_t m1 = 0, m2 = 0;
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
-        f32x4_t c = mfma16(frag_a(s->W + kHW1, kFI, mt, 0, lane), a, zero);
+        f32x4_t c = mfma16(frag_a(s->W + kHW1, kS1, mt, 0, lane), a, zero);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             m1 |= (c[i] > 0.f ? 1u : 0u) << (4 * mt + i);
             c[i] = fmaxf(c[i], 0.f);
         }
         h1[mt] = pack_bf(c);
+    }
+    // h2: the eight output tiles accumulate side by side (independent MFMA
+    // chains), k-step by k-step
+    f32x4_t acc[8];
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[mt][i] = s->b2[16 * mt + (lane >> 4) * 4 + i];
+#pragma unroll
+    for (int kt = 0; kt < 8; ++kt) {
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) acc[mt] = mfma16(frag_a(s->W + kHW2, kS2, mt, kt, lane), h1[kt], acc[mt]);
         __builtin_amdgcn_sched_barrier(0);
     }
     bf16x4_t h2[8];
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
-        f32x4_t c;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) c[i] = s->b2[16 * mt + (lane >> 4) * 4 + i];
-#pragma unroll
-        for (int kt = 0; kt < 8; ++kt) c = mfma16(frag_a(s->W + kHW2, kFH, mt, kt, lane), h1[kt], c);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            m2 |= (c[i] > 0.f ? 1u : 0u) << (4 * mt + i);
-            c[i] = fmaxf(c[i], 0.f);
+            m2 |= (acc[mt][i] > 0.f ? 1u : 0u) << (4 * mt + i);
+            acc[mt][i] = fmaxf(acc[mt][i], 0.f);
         }
-        h2[mt] = pack_bf(c);
-        __builtin_amdgcn_sched_barrier(0);
+        h2[mt] = pack_bf(acc[mt]);
+    }
+    f32x4_t o0, o1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        o0[i] = s->b3[(lane >> 4) * 4 + i];
+        o1[i] = s->b3[16 + (lane >> 4) * 4 + i];
     }
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-        f32x4_t c;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) c[i] = s->b3[16 * mt + (lane >> 4) * 4 + i];
-#pragma unroll
-        for (int kt = 0; kt < 8; ++kt) c = mfma16(frag_a(s->W + kHW3, kFH, mt, kt, lane), h2[kt], c);
-        o[mt] = c;
-        __builtin_amdgcn_sched_barrier(0);
+    for (int kt = 0; kt < 8; ++kt) {
+        o0 = mfma16(frag_a(s->W + kHW3, kS3, 0, kt, lane), h2[kt], o0);
+        o1 = mfma16(frag_a(s->W + kHW3, kS3, 1, kt, lane), h2[kt], o1);
     }
+    o[0] = o0;
+    o[1] = o1;
     if (KEEP) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -600,29 +611,33 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
         bf16x4_t dh2[8];
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
-            f32x4_t cc = mfma16(frag_a(Wt + kHW3, kFO, mt, 0, lane), do0, zero);
-            cc = mfma16(frag_a(Wt + kHW3, kFO, mt, 1, lane), do1, cc);
+            f32x4_t cc = mfma16(frag_a(Wt + kTW3, kT3, mt, 0, lane), do0, zero);
+            cc = mfma16(frag_a(Wt + kTW3, kT3, mt, 1, lane), do1, cc);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 if (!((kp.m2 >> (4 * mt + i)) & 1u)) cc[i] = 0.f;
             dh2[mt] = pack_bf(cc);
+        }
+        f32x4_t acc[8];
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt) acc[mt] = zero;
+#pragma unroll
+        for (int kt = 0; kt < 8; ++kt) {
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt) acc[mt] = mfma16(frag_a(Wt + kTW2, kT2, mt, kt, lane), dh2[kt], acc[mt]);
             __builtin_amdgcn_sched_barrier(0);
         }
         bf16x4_t dh1[8];
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
-            f32x4_t cc = zero;
-#pragma unroll
-            for (int kt = 0; kt < 8; ++kt) cc = mfma16(frag_a(Wt + kHW2, kFH, mt, kt, lane), dh2[kt], cc);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (!((kp.m1 >> (4 * mt + i)) & 1u)) cc[i] = 0.f;
-            dh1[mt] = pack_bf(cc);
-            __builtin_amdgcn_sched_barrier(0);
+                if (!((kp.m1 >> (4 * mt + i)) & 1u)) acc[mt][i] = 0.f;
+            dh1[mt] = pack_bf(acc[mt]);
         }
         f32x4_t da = zero;
 #pragma unroll
-        for (int kt = 0; kt < 8; ++kt) da = mfma16(frag_a(Wt + kHW1, kFH, 0, kt, lane), dh1[kt], da);
+        for (int kt = 0; kt < 8; ++kt) da = mfma16(frag_a(Wt + kTW1, kT1, 0, kt, lane), dh1[kt], da);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int d = 4 * g + i;
@@ -737,29 +752,34 @@ __global__ void flow_adam_kernel(AdamArgs a) {
     }
 }
 
-// bf16 copies (and transposes) of the weights after an update.
+// bf16 copies (and transposes) of the weights after an update, in the
+// padded layouts.
 __global__ void flow_cast_kernel(const float* P, uint16_t* H, uint16_t* HT, int L) {
-    const int total = L * kHLayer;
+    constexpr int nlog = kFH * kFI + kFH * kFH + kFO * kFH;
+    const int total = L * nlog;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        const int k = i / kHLayer, r = i % kHLayer;
+        const int k = i / nlog, r = i % nlog;
         const float* p = P + (size_t)k * kPLayer;
         float v;
-        int rt;
-        if (r < kHW2) {          // W1 [128][16] -> W1t [16][128]
+        int rh, rt;
+        if (r < kFH * kFI) {                 // W1 [128][16] -> W1t [16][128]
             const int o = r / kFI, c = r % kFI;
             v = p[kPW1 + r];
-            rt = kHW1 + c * kFH + o;
-        } else if (r < kHW3) {   // W2 [128][128]
-            const int q = r - kHW2, o = q / kFH, c = q % kFH;
+            rh = kHW1 + o * kS1 + c;
+            rt = kTW1 + c * kT1 + o;
+        } else if (r < kFH * kFI + kFH * kFH) {  // W2 [128][128]
+            const int q = r - kFH * kFI, o = q / kFH, c = q % kFH;
             v = p[kPW2 + q];
-            rt = kHW2 + c * kFH + o;
-        } else {                 // W3 [32][128] -> W3t [128][32]
-            const int q = r - kHW3, o = q / kFH, c = q % kFH;
+            rh = kHW2 + o * kS2 + c;
+            rt = kTW2 + c * kT2 + o;
+        } else {                             // W3 [32][128] -> W3t [128][32]
+            const int q = r - kFH * kFI - kFH * kFH, o = q / kFH, c = q % kFH;
             v = p[kPW3 + q];
-            rt = kHW3 + c * kFO + o;
+            rh = kHW3 + o * kS3 + c;
+            rt = kTW3 + c * kT3 + o;
         }
         const uint16_t h = f2bf(v);
-        H[(size_t)k * kHLayer + r] = h;
+        H[(size_t)k * kHLayer + rh] = h;
         HT[(size_t)k * kHLayer + rt] = h;
     }
 }
@@ -820,6 +840,8 @@ static int flow_cast(vgx_flow* fl, hipStream_t s) {
     return check_launch();
 }
 
+extern "C" int vgx_flow_destroy(vgx_flow* fl);
+
 extern "C" int vgx_flow_create(int dim, int nlayers, const uint32_t* amask, vgx_flow** out) {
     if (!out || !amask || dim < 1 || dim > kFI - 1 || nlayers < 1 || nlayers > kFMaxL) return VGX_EINVAL;
     vgx_flow* fl = new vgx_flow();
@@ -834,13 +856,18 @@ extern "C" int vgx_flow_create(int dim, int nlayers, const uint32_t* amask, vgx_
         return VGX_ENOMEM;
     }
     if (hipMemset(fl->P, 0, np * 4) != hipSuccess || hipMemset(fl->m, 0, np * 4) != hipSuccess ||
-        hipMemset(fl->v, 0, np * 4) != hipSuccess || hipMemset(fl->grad, 0, np * 4) != hipSuccess)
+        hipMemset(fl->v, 0, np * 4) != hipSuccess || hipMemset(fl->grad, 0, np * 4) != hipSuccess ||
+        hipMemset(fl->H, 0, nh * 2) != hipSuccess || hipMemset(fl->HT, 0, nh * 2) != hipSuccess) {
+        vgx_flow_destroy(fl);
         return VGX_EHIP;
+    }
     if (hipFuncSetAttribute((const void*)flow_backward_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(lds_layer() + sizeof(uint16_t) * kHLayer)) != hipSuccess ||
         hipFuncSetAttribute((const void*)flow_inverse_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(lds_layer() + kStateLds)) != hipSuccess)
+                            (int)(lds_layer() + kStateLds)) != hipSuccess) {
+        vgx_flow_destroy(fl);
         return VGX_EHIP;
+    }
     *out = fl;
     return VGX_OK;
 }
