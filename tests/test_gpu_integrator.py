@@ -161,8 +161,8 @@ def test_nan_raises(va):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_config1_statistics(va, prec):
-    # 4-D Gaussian, neval 1e5, nitn 10
+def test_gaussian_4d_statistics(va, prec):
+    # 4-D Gaussian workload, neval 1e5, nitn 10
     N = (100 / math.pi) ** 2 / math.erf(5) ** 4
     f = va.GaussianMixture([N], [100.], [[0.5] * 4])
     integ = va.Integrator([[0, 1]] * 4, neval=1e5, seed=2026, precision=prec)
@@ -173,8 +173,8 @@ def test_config1_statistics(va, prec):
     assert r.sdev < 2e-4
 
 
-def test_config2_statistics(va):
-    # 8-D double peak, neval 1e7, nitn 15 (FP32 integrand)
+def test_double_peak_8d_statistics(va):
+    # 8-D double-peak workload, neval 1e7, nitn 15 (FP32 integrand)
     f = double_peak(va)
     integ = va.Integrator([[0, 1]] * 8, neval=1e7, seed=7)
     assert list(integ.nstrat) == [6] * 6 + [5] * 2

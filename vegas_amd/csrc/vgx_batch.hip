@@ -303,6 +303,7 @@ constexpr size_t kBLdsCnts = (size_t)(kBHist / 2) * sizeof(uint32_t);
 constexpr size_t kBLdsOffs = (size_t)16 * kNoff * sizeof(uint32_t);
 constexpr size_t kBLds = kBLdsAxes + kBLdsGrid + kBLdsSums + kBLdsCnts + kBLdsOffs + 16 * 6 * sizeof(double);
 
+template <bool PER>
 __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it, int train) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t s_item;
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it,
                 const uint32_t plo = t * kT;
                 const TilePos tp{t, plo, min(plo + (uint32_t)kT, off_end), (uint32_t)a.tile_h[t], hend, off_end,
                                  q.hbase, pl.P};
-                tile_body<-kBD>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
+                tile_body<-kBD, PER>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
             }
         }
         __syncthreads();
@@ -496,6 +497,7 @@ using namespace vgx;
 struct vgx_batch {
     vgx_ctx* ctx = nullptr;
     int n = 0, tpi = 128, blocks = 0, nblk = 0, nad = 0, strad_blocks = 0;
+    int per_all = 1;  // every axis of every integral has ninc = per * nstrat
     uint32_t maxtiles = 0, maxitems = 0;
     int64_t htot = 0;
     double alpha = 0, beta = 0;
@@ -562,6 +564,7 @@ extern "C" int vgx_batch_create(vgx_ctx* ctx, const vgx_batch_config* c, vgx_bat
             const DivMagic dm = make_divmagic((uint32_t)ns);
             const uint32_t per = (ni % ns == 0 && ni / ns < (1 << 20)) ? (uint32_t)(ni / ns) : 0u;
             axes[(size_t)i * kBD + d] = AxisC{dm.m, dm.l, (uint32_t)ns, (uint32_t)ni, 1.0 / ns, per, 0u};
+            if (per == 0) b->per_all = 0;
         }
         const int64_t nmin = c->min_neval_hcube[i];
         const int64_t nmax = c->max_neval_hcube[i] > nmin ? c->max_neval_hcube[i] : nmin;
@@ -649,7 +652,9 @@ extern "C" int vgx_batch_create(vgx_ctx* ctx, const vgx_batch_config* c, vgx_bat
     g.beta_half = c->beta / 2.;
     b->strad_part = strad_part;
     if (kBLds + 64 > ctx->info.max_lds) { b_free(b); delete b; return VGX_ERANGE; }
-    if (hipFuncSetAttribute((const void*)b_fused_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (hipFuncSetAttribute((const void*)b_fused_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kBLds) != hipSuccess ||
+        hipFuncSetAttribute((const void*)b_fused_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)kBLds) != hipSuccess ||
         hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess) {
         b_free(b);
@@ -707,7 +712,10 @@ static int b_enqueue(vgx_batch* b, const double* A, const double* B, const doubl
         hipLaunchKernelGGL(b_alloc_kernel, dim3(b->nblk), dim3(kAllocThreads), 0, s, a);
         hipLaunchKernelGGL(b_plan_kernel, dim3(1), dim3(1024), 0, s, a);
         hipLaunchKernelGGL(b_offsets_kernel, dim3(b->nblk), dim3(kAllocThreads), 0, s, a);
-        hipLaunchKernelGGL(b_fused_kernel, dim3(b->blocks), dim3(kFusedThreads), kBLds, s, a, it, train);
+        if (b->per_all)
+            hipLaunchKernelGGL(b_fused_kernel<true>, dim3(b->blocks), dim3(kFusedThreads), kBLds, s, a, it, train);
+        else
+            hipLaunchKernelGGL(b_fused_kernel<false>, dim3(b->blocks), dim3(kFusedThreads), kBLds, s, a, it, train);
         hipLaunchKernelGGL(strad_kernel, dim3(b->strad_blocks), dim3(kStradThreads), 0, s, b->sg, sa);
         hipLaunchKernelGGL(b_close_kernel, dim3((b->n + 3) / 4), dim3(256), 0, s, a, st, it, nitn);
         hipLaunchKernelGGL(b_adapt_kernel, dim3(b->nad), dim3(256), 0, s, a, train);

@@ -50,23 +50,13 @@ struct AxisC {
     uint32_t pad;
 };
 
-// Generates uniforms, y, x and J for point p of hcube h.
-//   R32 (24-bit uniforms): when ninc = per * nstrat the increment index and
-//       offset follow exactly from t = (c + u) * per with 24-bit integer
-//       multiplies: k = c*per + floor(u24*per / 2^24), offset = the low 24
-//       bits * 2^-24 (the map of the exact y = (c+u)/nstrat);
-//   otherwise (53-bit uniforms, or unaligned axes, or exact_div) y is formed
-//       in FP64 as the reference does, y = (c+u)/nstrat (pyx:1741), and the
-//       map computes t = y*ninc (pyx:351).
-//   DIM > 0: that dimension; DIM <= 0: runtime g.dim, at most nd_max(DIM).
-template <int DIM, bool R32>
-__device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __restrict__ ax,
-                                           const double* __restrict__ G, uint32_t h, uint32_t p,
-                                           double* x, double& J, int* kbin) {
-    const int dim = DIM > 0 ? DIM : g.dim;
+// Stratum coordinates of hcube h, axis 0 fastest (pyx:1736-1738), each
+// premultiplied by the axis's increments per stratum (the first increment
+// index of the stratum, < 2^16).
+template <int DIM>
+__device__ __forceinline__ void per_coords(const AxisC* __restrict__ ax, int dim, uint32_t h,
+                                           uint32_t* cc) {
     uint32_t t = h;
-    U4 w{0, 0, 0, 0};
-    J = 1.0;
 #pragma unroll
     for (int d = 0; d < nd_max(DIM); ++d) {
         if (DIM <= 0 && d >= dim) break;
@@ -74,6 +64,66 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
         const uint32_t q = (uint32_t)(((uint64_t)__umulhi(t, a.m) + t) >> a.l);
         const uint32_t c = t - q * a.nstrat;
         t = q;
+        cc[d] = c * a.per;
+    }
+}
+
+// The PER coordinates of one hcube, two 16-bit fields per register: a lane
+// computes them once per hcube and reuses them for its points in that hcube.
+template <int DIM>
+struct PackedCoords {
+    static constexpr int kN = (nd_max(DIM) + 1) / 2;
+    uint32_t v[kN];
+    __device__ __forceinline__ void set(const AxisC* __restrict__ ax, int dim, uint32_t h) {
+        uint32_t cc[2 * kN];
+#pragma unroll
+        for (int d = 0; d < 2 * kN; ++d) cc[d] = 0u;
+        per_coords<DIM>(ax, dim, h, cc);
+#pragma unroll
+        for (int i = 0; i < kN; ++i) v[i] = cc[2 * i] | (cc[2 * i + 1] << 16);
+    }
+    __device__ __forceinline__ uint32_t get(int d) const { return (v[d >> 1] >> (16 * (d & 1))) & 0xFFFFu; }
+};
+
+// Coordinates of hcube h decoded axis by axis as eval_point walks the axes
+// (no array of them stays live).
+struct CoordWalk {
+    uint32_t t;
+    __device__ __forceinline__ uint32_t next(const AxisC& a) {
+        const uint32_t q = (uint32_t)(((uint64_t)__umulhi(t, a.m) + t) >> a.l);
+        const uint32_t c = t - q * a.nstrat;
+        t = q;
+        return c;
+    }
+};
+
+// Generates uniforms, y, x and J for point p of hcube h (with PER, pc holds
+// h's premultiplied coordinates).
+//   PER (24-bit uniforms, every axis with ninc = per * nstrat, no exact
+//       division): the increment index and offset follow exactly from
+//       t = (c + u) * per with 24-bit integer multiplies:
+//       k = c*per + floor(u24*per / 2^24), offset = the low 24 bits * 2^-24
+//       (the map of the exact y = (c+u)/nstrat); k < ninc always;
+//   otherwise (53-bit uniforms, or unaligned axes, or exact_div) y is formed
+//       in FP64 as the reference does, y = (c+u)/nstrat (pyx:1741), and the
+//       map computes t = y*ninc (pyx:351).
+//   k = ninc (y = 1) takes x = grid[ninc] and the last increment
+//   (pyx:355-359).
+//   DIM > 0: that dimension; DIM <= 0: runtime g.dim, at most nd_max(DIM).
+template <int DIM, bool R32, bool PER>
+__device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __restrict__ ax,
+                                           const double* __restrict__ G, const PackedCoords<DIM>& pc,
+                                           uint32_t h, uint32_t p, double* x, double& J, int* kbin) {
+    static_assert(R32 || !PER, "PER needs 24-bit uniforms");
+    const int dim = DIM > 0 ? DIM : g.dim;
+    U4 w{0, 0, 0, 0};
+    J = 1.0;
+    CoordWalk walk{h};
+#pragma unroll
+    for (int d = 0; d < nd_max(DIM); ++d) {
+        if (DIM <= 0 && d >= dim) break;
+        const AxisC a = ax[d];
+        const uint32_t c = PER ? pc.get(d) : walk.next(a);
         const int ni = (int)a.ninc;
         int k, kv;
         double dl;
@@ -82,10 +132,10 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
             const int e = d & 3;
             const uint32_t wd = e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w;
             const uint32_t ui = wd >> 8;
-            if (a.per && !g.exact_div) {
+            if (PER) {
                 // both factors < 2^24: the backend selects the 24-bit multiplies
                 const uint64_t prod = (uint64_t)ui * (uint64_t)(a.per & 0xFFFFFu);
-                k = (int)(c * a.per + (uint32_t)(prod >> 24));
+                k = (int)(c + (uint32_t)(prod >> 24));
                 dl = (double)((uint32_t)prod & 0xFFFFFFu) * (1.0 / 16777216.0);
                 kv = (c == 0 && ui == 0) ? -1 : k;
             } else {
@@ -106,9 +156,10 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
             kv = (yv > 0 && yv < 1) ? (k < ni ? k : ni - 1) : -1;
         }
         const double* gd = G + d * g.ld;
-        if (k < ni) {
-            const double inc = gd[k + 1] - gd[k];
-            x[d] = gd[k] + inc * dl;
+        if (PER || k < ni) {  // PER: k < ninc always; no branch, loads can be issued early
+            const double g0 = gd[k], g1 = gd[k + 1];
+            const double inc = g1 - g0;
+            x[d] = g0 + inc * dl;
             J *= inc * (double)ni;
         } else {
             x[d] = gd[ni];
@@ -117,7 +168,6 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
         kbin[d] = kv;
     }
 }
-
 // y of point p of hcube h, formed as eval_point forms it (staged sampler).
 template <bool R32>
 __device__ inline void point_y(const TileGeom& g, const AxisC* __restrict__ ax, uint32_t h,
@@ -227,7 +277,7 @@ struct TilePos {
 
 // One wave tile (kT points, kK per lane): sample, map, integrand, training
 // histogram, and the reference's two passes per hcube as segmented scans.
-template <int DIM, class Fcn>
+template <int DIM, bool PER, class Fcn>
 __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, const Fcn& fcn,
                                           const AxisC* __restrict__ ax, const double* __restrict__ G,
                                           double* Ls, uint32_t* Lc, uint32_t* offs, const TilePos& tp,
@@ -269,6 +319,7 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
     int cur_i = -1;
     double wdiv = 0.0;
     uint32_t ncur = 1;
+    PackedCoords<DIM> pc;  // PER: coordinates of hcube cur_i
 #pragma unroll 1
     for (int j = 0; j < kK; ++j) {
         const uint32_t pj = pl0 + j;
@@ -281,13 +332,14 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
                 cur_i = li;
                 ncur = offs[li + 1] - offs[li];
                 wdiv = g.dv_y / (double)ncur;
+                if (PER) pc.set(ax, dim, hf + li - tp.hbase);
             }
             kj = li;
             nj = ncur;
             double x[ND];
             int kb[ND];
             double J;
-            eval_point<DIM, R32>(g, ax, G, hf + li - tp.hbase, pj - tp.pshift, x, J, kb);
+            eval_point<DIM, R32, PER>(g, ax, G, pc, hf + li - tp.hbase, pj - tp.pshift, x, J, kb);
             const double w = J * wdiv;
             const double f = fcn.template eval<DIM>(x, dim, pj);
             if (f != f) acc.nan_seen = 1;
@@ -492,7 +544,7 @@ __device__ inline void block_close(TileAcc& acc, double* Lred, double* out, Scal
     }
 }
 
-template <int DIM, class Fcn>
+template <int DIM, bool PER, class Fcn>
 __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedPtrs io, Fcn fcn) {
     extern __shared__ __align__(16) unsigned char smem[];
     Scalars* sc = io.sc;
@@ -521,12 +573,14 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
         for (int i = threadIdx.x; i < tot; i += blockDim.x) Ls[i] = 0.0;
         for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) Lc[i] = 0u;
     }
-    cur = reinterpret_cast<unsigned char*>(((uintptr_t)cur + 15) & ~(uintptr_t)15);
+    // offsets from smem rounded up to 16 B (pointer arithmetic on smem keeps
+    // the accesses ds_* instead of flat)
+    size_t o16 = ((size_t)(cur - smem) + 15) & ~(size_t)15;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nwave = blockDim.x >> 6;
-    uint32_t* offs = reinterpret_cast<uint32_t*>(cur) + wave * kNoff;
-    cur += (size_t)nwave * kNoff * sizeof(uint32_t);
-    double* Lred = reinterpret_cast<double*>(((uintptr_t)cur + 15) & ~(uintptr_t)15);
+    uint32_t* offs = reinterpret_cast<uint32_t*>(smem + o16) + wave * kNoff;
+    o16 = (o16 + (size_t)nwave * kNoff * sizeof(uint32_t) + 15) & ~(size_t)15;
+    double* Lred = reinterpret_cast<double*>(smem + o16);
     __syncthreads();
 
     const TileIO tio{io.off, io.sigf, io.slots, io.errv, io.sum_f, io.n_f};
@@ -542,7 +596,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
         if (t < te) {
             const uint32_t plo = p0 + t * kT;
             const TilePos tp{t, plo, min(plo + (uint32_t)kT, p1), (uint32_t)io.tile_h[t], hend, off_end, 0u, 0u};
-            tile_body<DIM>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
+            tile_body<DIM, PER>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
         }
         if (g.hist_lds && g.train_points && ((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
             __syncthreads();
@@ -566,7 +620,10 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
 template <int DIM, class Fcn>
 int launch_fused_dim(const IterGeom& g, const FusedPtrs& io, const Fcn& fcn, int blocks, size_t lds,
                      hipStream_t st) {
-    auto kern = fused_kernel<DIM, Fcn>;
+    auto kern = fused_kernel<DIM, false, Fcn>;
+    if constexpr (Fcn::kRng32) {
+        if (g.per_all) kern = fused_kernel<DIM, true, Fcn>;
+    }
     if (lds > 64 * 1024) {
         if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds) != hipSuccess)

@@ -423,6 +423,22 @@ __global__ void sigf_reset_kernel(double* sigf, int64_t n, Scalars* sc) {
         sigf[i] = 1.0;
 }
 
+// eval_point for hcube h with the stratum->bin path the fused sampler takes
+// for this geometry (staged and error-training kernels).
+template <bool R32>
+__device__ inline void eval_point_h(const IterGeom& g, const AxisC* axes, const double* grid, uint32_t h,
+                                    uint32_t p, double* x, double& J, int* kb) {
+    PackedCoords<0> pc;
+    if constexpr (R32) {
+        if (g.per_all) {
+            pc.set(axes, g.dim, h);
+            eval_point<0, true, true>(tile_geom(g), axes, grid, pc, h, p, x, J, kb);
+            return;
+        }
+    }
+    eval_point<0, R32, false>(tile_geom(g), axes, grid, pc, h, p, x, J, kb);
+}
+
 // adapt_to_errors: training entry |var_h| at the last point of every hcube
 // (pyx:2187-2193).
 template <bool R32>
@@ -435,7 +451,7 @@ __global__ __launch_bounds__(256) void errors_train_kernel(IterGeom g, const Axi
         double x[VGX_MAXDIM];
         int kb[VGX_MAXDIM];
         double J;
-        eval_point<0, R32>(tile_geom(g), axes, grid, (uint32_t)h, off[h + 1] - 1, x, J, kb);
+        eval_point_h<R32>(g, axes, grid, (uint32_t)h, off[h + 1] - 1, x, J, kb);
         for (int d = 0; d < g.dim; ++d)
             if (kb[d] >= 0) {
                 unsafeAtomicAdd(&sum_f[d * g.nb + kb[d]], errv[h]);
@@ -471,7 +487,7 @@ __global__ __launch_bounds__(256) void sample_kernel(IterGeom g, const AxisC* ax
         double xx[VGX_MAXDIM];
         int kb[VGX_MAXDIM];
         double J;
-        eval_point<0, R32>(tile_geom(g), axes, grid, h, p, xx, J, kb);
+        eval_point_h<R32>(g, axes, grid, h, p, xx, J, kb);
         const int64_t i = (int64_t)(p - p0);
         for (int d = 0; d < dim; ++d) x[i * dim + d] = xx[d];
         if (y) point_y<R32>(tile_geom(g), axes, h, p, y + i * dim);
@@ -513,6 +529,9 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
     g->weighted = c->weighted;
     g->adapt = c->adapt;
     g->fast_pow = g->adaptive_strat && !g->exact_div;
+    g->per_all = g->rng32 && !g->exact_div;
+    for (int d = 0; d < c->dim; ++d)
+        if (make_axis(*g, d).per == 0 || c->ninc[d] >= 65536) g->per_all = 0;
     const int64_t nmin = c->min_neval_hcube;
     const int64_t nmax = c->max_neval_hcube > nmin ? c->max_neval_hcube : nmin;
     const int64_t avg = (int64_t)((double)c->neval / (double)c->nhcube);

@@ -34,6 +34,8 @@ struct IterGeom {
     int adapt;
     int fast_pow;          // sigf = |var|^(beta/2) in the sampler, FP32 log2/exp2
                            // (else FP64 pow in a separate pass)
+    int per_all;           // rng32, no exact division, and ninc a multiple of
+                           // nstrat on every axis: the integer stratum->bin path
     int32_t ninc[VGX_MAXDIM];
     int32_t nstrat[VGX_MAXDIM];
     double inv_nstrat[VGX_MAXDIM];

@@ -1,5 +1,5 @@
-"""Benchmark workloads: the integrands and run parameters of the five
-measured configurations (exact integral 1 unless stated)."""
+"""Benchmark workloads: the integrands and run parameters of the measured
+workloads (exact integral 1 unless stated)."""
 from __future__ import annotations
 
 import math
@@ -39,7 +39,8 @@ def ridge_8d(diagonal=False):
 
 
 def ridge_9d(diagonal=False):
-    """Config 5's integrand: 500 diagonal Gaussians, sigma 0.01, d = 9."""
+    """The learned-map workload's integrand: mean of 500 diagonal Gaussians,
+    sigma 0.01, centres on the diagonal from 0.25*1 to 0.75*1 in [0,1]^9."""
     t = 0.25 + 0.5 * np.arange(500) / 499.0
     sigma = 0.01
     A = np.full(500, (1 / 500) * (2 * math.pi * sigma ** 2) ** -4.5)
@@ -56,7 +57,7 @@ def flops_per_eval(dim, ncomp):
 
 
 def batched_gaussians(n=4096, seed=12345):
-    """Config 4: n independent single-Gaussian integrals; dim ~ U{2..10},
+    """The batched workload: n independent single-Gaussian integrals; dim ~ U{2..10},
     centre ~ U[0.2,0.8]^dim, sigma log-uniform in [0.01, 0.1], neval
     log-uniform in [1e4, 1e6]; exact value from the erf product."""
     rng = np.random.default_rng(seed)
