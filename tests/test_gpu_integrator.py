@@ -82,6 +82,41 @@ def test_training_and_adapt_parity(va, orc):
     np.testing.assert_allclose(integ.map.grid, o.map.grid, rtol=1e-11, atol=1e-13)
 
 
+def test_y_zero_left_out_of_training(va, orc):
+    """A point with y = 0 exactly (one stratum per axis, 24-bit uniform 0 on
+    axis 0: seed 15, point 477932) stays out of axis 0's training histogram
+    (pyx:459-463) on the integer stratum-to-bin path.  Constant integrand,
+    non-uniform grid on axis 1: the adapted grid equals the oracle's to
+    rounding, far below the shift that counting the point would cause."""
+    seed, p, neval = 15, 477932, 600000
+    w = orc.philox([p, 0, 0, 0], [seed, 0])
+    assert w[0] >> 8 == 0
+    grid = [[0.0, 1.0], [0.0, 0.05, 0.2, 1.0]]
+    o = orc.OracleIntegrator(grid, neval=neval, nstrat=[1, 1], seed=seed, rng_f32=True)
+    m0 = orc.OracleMap(o.map)
+    out, sf, nf = o.iterate(orc.GaussMix([1.0], [0.0], [[0.5, 0.5]]))
+
+    def adapted(s, n):
+        m = orc.OracleMap(m0)
+        m.sum_f, m.n_f = s.copy(), n.copy()
+        m.adapt(alpha=0.5)
+        return m.grid
+
+    g_ok = adapted(sf, nf)
+    # the same histogram with point p counted on axis 0 (w f n = J there)
+    _, jp = m0.map(np.array([[0.0, (int(w[1]) >> 8) / 16777216.0]]))
+    sf2, nf2 = sf.copy(), nf.copy()
+    sf2[0, 0] += jp[0] ** 2
+    nf2[0, 0] += 1
+    gap = np.abs(adapted(sf2, nf2) - g_ok).max()
+    assert gap > 1e-9
+    f = va.GaussianMixture([1.0], [0.0], [[0.5, 0.5]])
+    integ = va.Integrator(va.AdaptiveMap(grid), neval=neval, nstrat=[1, 1], seed=seed, precision='f32')
+    integ(f, nitn=1)
+    assert integ.last_neval == neval
+    assert np.abs(integ.map.grid - g_ok).max() < gap / 100
+
+
 def test_fused_equals_staged(va, orc):
     """Same seed and precision: the device integrand and a numpy lbatch,
     rbatch, torch and non-batch integrand see the same points."""

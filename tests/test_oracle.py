@@ -289,3 +289,23 @@ def test_gaussian_4d_analytic(orc):
     assert abs(mean - 1.0) < 5 * sdev
     assert orc.gammaQ(9 / 2., chi2 / 2.) > 1e-3
     assert sdev < 2e-3
+
+
+# Seed 15, first iteration: point 477932's axis-0 uniform is 0 in its top 24
+# bits, so with one stratum per axis y0 = 0 exactly (found by a scan of the
+# Philox stream; checked here through the C restatement).
+Y0_SEED, Y0_POINT = 15, 477932
+
+
+def test_y_zero_left_out_of_training(orc):
+    """pyx:459-463 trains only on 0 < y < 1: the point with y0 = 0 is missing
+    from axis 0's counts and present on axis 1."""
+    w = orc.philox([Y0_POINT, 0, 0, 0], [Y0_SEED, 0])
+    assert w[0] >> 8 == 0 and w[1] >> 8 != 0
+    neval = 600000
+    o = orc.OracleIntegrator([[0.0, 1.0], [0.0, 0.05, 0.2, 1.0]], neval=neval, nstrat=[1, 1],
+                             seed=Y0_SEED, rng_f32=True)
+    out, sf, nf = o.iterate(orc.GaussMix([1.0], [0.0], [[0.5, 0.5]]))
+    assert out.last_neval == neval
+    assert round(nf[0].sum()) == neval - 1
+    assert round(nf[1].sum()) == neval

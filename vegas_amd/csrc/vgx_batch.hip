@@ -12,7 +12,7 @@
 //              are runs of whole tiles of one integral
 //   b_offsets  point offsets and tile -> first hcube
 //   b_fused    persistent sampler: workgroups pull work items from a queue,
-//              stage that integral's axes and grid in LDS, run the shared
+//              stage that integral's grid in LDS, run the shared
 //              tile body, and write the item's training histogram and
 //              compensated partial sums (deterministic per item)
 //   strad      cut hcubes (shared kernel), one contribution per tile slot
@@ -294,28 +294,26 @@ __global__ __launch_bounds__(kAllocThreads) void b_offsets_kernel(BArgs a) {
 }
 
 // ---------------------------------------------------------------- fused ----
-// LDS: axes | grid rows | histogram sums | u16 pair-packed counts | offsets
-// slices | reduction scratch.
-constexpr size_t kBLdsAxes = 16 * sizeof(AxisC);
+// LDS: grid rows | histogram sums | u16 pair-packed counts | offsets slices |
+// reduction scratch (the axis constants are read by scalar loads).
 constexpr size_t kBLdsGrid = (size_t)kBD * kBLd * sizeof(double);
 constexpr size_t kBLdsSums = (size_t)kBHist * sizeof(double);
 constexpr size_t kBLdsCnts = (size_t)(kBHist / 2) * sizeof(uint32_t);
 constexpr size_t kBLdsOffs = (size_t)16 * kNoff * sizeof(uint32_t);
-constexpr size_t kBLds = kBLdsAxes + kBLdsGrid + kBLdsSums + kBLdsCnts + kBLdsOffs + 16 * 6 * sizeof(double);
+constexpr size_t kBLds = kBLdsGrid + kBLdsSums + kBLdsCnts + kBLdsOffs + 16 * 6 * sizeof(double);
 
 template <bool PER>
 __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it, int train) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t s_item;
     if (a.sc->status || a.sc->overflow) return;
-    AxisC* ax = reinterpret_cast<AxisC*>(smem);
-    double* G = reinterpret_cast<double*>(smem + kBLdsAxes);
-    double* Ls = reinterpret_cast<double*>(smem + kBLdsAxes + kBLdsGrid);
-    uint32_t* Lc = reinterpret_cast<uint32_t*>(smem + kBLdsAxes + kBLdsGrid + kBLdsSums);
+    double* G = reinterpret_cast<double*>(smem);
+    double* Ls = reinterpret_cast<double*>(smem + kBLdsGrid);
+    uint32_t* Lc = reinterpret_cast<uint32_t*>(smem + kBLdsGrid + kBLdsSums);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint32_t* offs = reinterpret_cast<uint32_t*>(smem + kBLdsAxes + kBLdsGrid + kBLdsSums + kBLdsCnts) +
+    uint32_t* offs = reinterpret_cast<uint32_t*>(smem + kBLdsGrid + kBLdsSums + kBLdsCnts) +
                      wave * kNoff;
-    double* Lred = reinterpret_cast<double*>(smem + kBLdsAxes + kBLdsGrid + kBLdsSums + kBLdsCnts + kBLdsOffs);
+    double* Lred = reinterpret_cast<double*>(smem + kBLdsGrid + kBLdsSums + kBLdsCnts + kBLdsOffs);
     const uint32_t nitems = a.bs->nitems;
     const uint32_t iteration = a.bs->iter0 + (uint32_t)it;
     for (;;) {
@@ -343,7 +341,7 @@ __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it,
         tg.seed_hi = q.seed_hi;
         tg.beta_half = a.beta_half;
         tg.dv_y = q.dv_y;
-        if (threadIdx.x < (unsigned)dim) ax[threadIdx.x] = a.axes[(size_t)i * kBD + threadIdx.x];
+        const AxisC* ax = a.axes + (size_t)i * kBD;
         const double* grow = a.grid + (size_t)i * kBD * kBLd;
         for (int k = threadIdx.x; k < dim * kBLd; k += blockDim.x) G[k] = grow[k];
         const int tot = dim * kBNb;
@@ -361,6 +359,9 @@ __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it,
                 const uint32_t plo = t * kT;
                 const TilePos tp{t, plo, min(plo + (uint32_t)kT, off_end), (uint32_t)a.tile_h[t], hend, off_end,
                                  q.hbase, pl.P};
+                uint32_t ov[kOffRegs];
+                load_offsets(a.off, tp.hf, hend, off_end, lane, ov);
+                store_offsets(offs, lane, ov);
                 tile_body<-kBD, PER>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
             }
         }

@@ -42,13 +42,20 @@ __host__ __device__ inline TileGeom tile_geom(const IterGeom& g) {
     return t;
 }
 
-// Per-axis constants of the sampler, staged in LDS by the fused kernel.
+// Per-axis constants of the sampler (device memory, written before the
+// sampler runs and read with scalar loads).
 struct AxisC {
     uint32_t m, l, nstrat, ninc;  // DivMagic of nstrat, strata and increments
     double inv_s;                 // 1 / nstrat
     uint32_t per;                 // ninc / nstrat when ninc is a multiple of nstrat, else 0
     uint32_t pad;
 };
+
+__device__ __forceinline__ AxisC ldc(const AxisC* p, int i) {
+    const uint32_t* u = reinterpret_cast<const uint32_t*>(p + i);
+    const double* f = reinterpret_cast<const double*>(p + i);
+    return AxisC{ldc(u, 0), ldc(u, 1), ldc(u, 2), ldc(u, 3), ldc(f, 2), ldc(u, 6), 0u};
+}
 
 // Stratum coordinates of hcube h, axis 0 fastest (pyx:1736-1738), each
 // premultiplied by the axis's increments per stratum (the first increment
@@ -60,7 +67,7 @@ __device__ __forceinline__ void per_coords(const AxisC* __restrict__ ax, int dim
 #pragma unroll
     for (int d = 0; d < nd_max(DIM); ++d) {
         if (DIM <= 0 && d >= dim) break;
-        const AxisC a = ax[d];
+        const AxisC a = ldc(ax, d);
         const uint32_t q = (uint32_t)(((uint64_t)__umulhi(t, a.m) + t) >> a.l);
         const uint32_t c = t - q * a.nstrat;
         t = q;
@@ -83,6 +90,20 @@ struct PackedCoords {
         for (int i = 0; i < kN; ++i) v[i] = cc[2 * i] | (cc[2 * i + 1] << 16);
     }
     __device__ __forceinline__ uint32_t get(int d) const { return (v[d >> 1] >> (16 * (d & 1))) & 0xFFFFu; }
+    // h -> h + 1: axis 0 advances and carries into the next axes (the digit
+    // order of pyx:1736-1738), with no division.
+    __device__ __forceinline__ void step(const AxisC* __restrict__ ax, int dim) {
+#pragma unroll
+        for (int d = 0; d < nd_max(DIM); ++d) {
+            if (DIM <= 0 && d >= dim) break;
+            const AxisC a = ldc(ax, d);
+            const uint32_t c = get(d) + a.per;
+            const uint32_t sh = 16 * (d & 1);
+            const bool carry = c >= a.ninc;
+            v[d >> 1] = (v[d >> 1] & ~(0xFFFFu << sh)) | ((carry ? 0u : c) << sh);
+            if (!carry) break;
+        }
+    }
 };
 
 // Coordinates of hcube h decoded axis by axis as eval_point walks the axes
@@ -109,11 +130,16 @@ struct CoordWalk {
 //       map computes t = y*ninc (pyx:351).
 //   k = ninc (y = 1) takes x = grid[ninc] and the last increment
 //   (pyx:355-359).
+//   kbin[d]: the training bin of axis d, -1 where y is not inside (0,1)
+//   (pyx:459-463); with PER the bins are always given and zmin (min over the
+//   axes of c|u, to be initialised by the caller) is 0 iff some axis has
+//   y = 0 exactly -- per_zero_axes() then names the axes to skip.
 //   DIM > 0: that dimension; DIM <= 0: runtime g.dim, at most nd_max(DIM).
 template <int DIM, bool R32, bool PER>
 __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __restrict__ ax,
                                            const double* __restrict__ G, const PackedCoords<DIM>& pc,
-                                           uint32_t h, uint32_t p, double* x, double& J, int* kbin) {
+                                           uint32_t h, uint32_t p, double* x, double& J, int* kbin,
+                                           uint32_t& zmin) {
     static_assert(R32 || !PER, "PER needs 24-bit uniforms");
     const int dim = DIM > 0 ? DIM : g.dim;
     U4 w{0, 0, 0, 0};
@@ -122,7 +148,7 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
 #pragma unroll
     for (int d = 0; d < nd_max(DIM); ++d) {
         if (DIM <= 0 && d >= dim) break;
-        const AxisC a = ax[d];
+        const AxisC a = ldc(ax, d);
         const uint32_t c = PER ? pc.get(d) : walk.next(a);
         const int ni = (int)a.ninc;
         int k, kv;
@@ -137,7 +163,8 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
                 const uint64_t prod = (uint64_t)ui * (uint64_t)(a.per & 0xFFFFFu);
                 k = (int)(c + (uint32_t)(prod >> 24));
                 dl = (double)((uint32_t)prod & 0xFFFFFFu) * (1.0 / 16777216.0);
-                kv = (c == 0 && ui == 0) ? -1 : k;
+                kv = k;  // y = 0 (c = 0, u = 0) is flagged through zmin
+                zmin = min(zmin, c | ui);
             } else {
                 const double yv = g.exact_div ? ((double)c + u24(wd)) / (double)a.nstrat
                                               : ((double)c + u24(wd)) * a.inv_s;
@@ -168,6 +195,25 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
         kbin[d] = kv;
     }
 }
+// PER: bit d set where point p of the hcube with coordinates pc has y = 0
+// exactly on axis d (stratum 0 and a zero uniform; probability 2^-24 per
+// axis and point, so this regenerates the uniforms instead of carrying a
+// mask through the point loop).
+template <int DIM>
+__device__ __forceinline__ uint64_t per_zero_axes(const TileGeom& g, const PackedCoords<DIM>& pc, uint32_t p) {
+    const int dim = DIM > 0 ? DIM : g.dim;
+    uint64_t m = 0;
+    U4 w{0, 0, 0, 0};
+    for (int d = 0; d < nd_max(DIM); ++d) {
+        if (DIM <= 0 && d >= dim) break;
+        if ((d & 3) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
+        const int e = d & 3;
+        const uint32_t wd = e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w;
+        if ((pc.get(d) | (wd >> 8)) == 0) m |= 1ull << d;
+    }
+    return m;
+}
+
 // y of point p of hcube h, formed as eval_point forms it (staged sampler).
 template <bool R32>
 __device__ inline void point_y(const TileGeom& g, const AxisC* __restrict__ ax, uint32_t h,
@@ -275,6 +321,23 @@ struct TilePos {
     uint32_t t, plo, phi, hf, hend, off_end, hbase, pshift;
 };
 
+// Offsets of the hcubes from hf on, as a wave tile needs them: entry i of
+// lane i + kWave*q (past hend: the end offset).
+constexpr int kOffRegs = (kNoff + kWave - 1) / kWave;
+__device__ __forceinline__ void load_offsets(const uint32_t* __restrict__ off, uint32_t hf, uint32_t hend,
+                                             uint32_t off_end, int lane, uint32_t (&v)[kOffRegs]) {
+#pragma unroll
+    for (int q = 0; q < kOffRegs; ++q) {
+        const uint32_t hh = hf + (uint32_t)(lane + kWave * q);
+        v[q] = hh < hend ? off[hh] : off_end;
+    }
+}
+__device__ __forceinline__ void store_offsets(uint32_t* offs, int lane, const uint32_t (&v)[kOffRegs]) {
+#pragma unroll
+    for (int q = 0; q < kOffRegs; ++q)
+        if (lane + kWave * q < kNoff) offs[lane + kWave * q] = v[q];
+}
+
 // One wave tile (kT points, kK per lane): sample, map, integrand, training
 // histogram, and the reference's two passes per hcube as segmented scans.
 template <int DIM, bool PER, class Fcn>
@@ -287,10 +350,7 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
     const int dim = DIM > 0 ? DIM : g.dim;
     const double eps = kEpsilon;
     const uint32_t t = tp.t, plo = tp.plo, phi = tp.phi, hf = tp.hf;
-    for (int i = lane; i < kNoff; i += kWave) {
-        const uint32_t hh = hf + i;
-        offs[i] = hh < tp.hend ? io.off[hh] : tp.off_end;
-    }
+    // offs[] (the offsets of hcubes hf, hf+1, ...) is staged by the caller
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // hcube (local index) of this lane's first point
@@ -316,10 +376,19 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
         wf[j] = 0.0;
         nn[j] = 1;
     }
-    int cur_i = -1;
+    // hcube cur_i (local index) of the lane's current point: its end offset
+    // nxt, its point count, dv_y / n_h (pyx:1749) and, PER, its coordinates
+    int cur_i = li;
+    uint32_t nxt = 0xFFFFFFFFu;
     double wdiv = 0.0;
     uint32_t ncur = 1;
-    PackedCoords<DIM> pc;  // PER: coordinates of hcube cur_i
+    PackedCoords<DIM> pc;
+    if (pl0 < phi) {
+        nxt = offs[li + 1];
+        ncur = nxt - offs[li];
+        wdiv = g.dv_y / (double)ncur;
+        if (PER) pc.set(ax, dim, hf + li - tp.hbase);
+    }
 #pragma unroll 1
     for (int j = 0; j < kK; ++j) {
         const uint32_t pj = pl0 + j;
@@ -327,32 +396,45 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
         double vj = 0.0;
         uint32_t nj = 1;
         if (pj < phi) {
-            while (li < kNoff - 2 && offs[li + 1] <= pj) ++li;
-            if (li != cur_i) {
-                cur_i = li;
-                ncur = offs[li + 1] - offs[li];
+            while (pj >= nxt && cur_i < kNoff - 2) {  // next hcube (one step: hcubes have >= 2 points)
+                ++cur_i;
+                const uint32_t o0 = nxt;
+                nxt = offs[cur_i + 1];
+                ncur = nxt - o0;
                 wdiv = g.dv_y / (double)ncur;
-                if (PER) pc.set(ax, dim, hf + li - tp.hbase);
+                if (PER) pc.step(ax, dim);
             }
-            kj = li;
+            kj = cur_i;
             nj = ncur;
             double x[ND];
             int kb[ND];
             double J;
-            eval_point<DIM, R32, PER>(g, ax, G, pc, hf + li - tp.hbase, pj - tp.pshift, x, J, kb);
+            uint32_t zmin = 0xFFFFFFFFu;
+            eval_point<DIM, R32, PER>(g, ax, G, pc, hf + cur_i - tp.hbase, pj - tp.pshift, x, J, kb, zmin);
             const double w = J * wdiv;
             const double f = fcn.template eval<DIM>(x, dim, pj);
             if (f != f) acc.nan_seen = 1;
             vj = w * f;
             const double tn = vj * (double)ncur;
             acc.wmax = fmax(acc.wmax, fabs(tn));
-            if (g.train_points) {
+            if (g.train_points && PER && g.hist_lds && zmin != 0) {
+                // every axis inside (0,1): no per-axis test
                 const double c2 = tn * tn;
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
                     if (DIM <= 0 && d >= dim) break;
+                    const int idx = d * g.nb + kb[d];
+                    unsafeAtomicAdd(&Ls[idx], c2);
+                    atomicAdd(&Lc[idx >> 1], 1u + (uint32_t)(idx & 1) * 65535u);
+                }
+            } else if (g.train_points) {
+                const double c2 = tn * tn;
+                const uint64_t zm = (PER && zmin == 0) ? per_zero_axes<DIM>(g, pc, pj - tp.pshift) : 0ull;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    if (DIM <= 0 && d >= dim) break;
                     const int k = kb[d];
-                    if (k < 0) continue;
+                    if (k < 0 || ((zm >> d) & 1)) continue;
                     const int idx = d * g.nb + k;
                     if (g.hist_lds) {
                         unsafeAtomicAdd(&Ls[idx], c2);
@@ -556,9 +638,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
 
     // ---- LDS carve-up ----
     unsigned char* cur = smem;
-    AxisC* ax = reinterpret_cast<AxisC*>(cur);
-    if (threadIdx.x < (unsigned)dim) ax[threadIdx.x] = io.axes[threadIdx.x];
-    cur += (size_t)VGX_MAXDIM * sizeof(AxisC);
+    const AxisC* ax = io.axes;  // read by scalar loads (ldc), not staged
     double* Lg = reinterpret_cast<double*>(cur);
     for (int i = threadIdx.x; i < dim * g.ld; i += blockDim.x) Lg[i] = io.grid[i];
     const double* G = Lg;
@@ -596,6 +676,9 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
         if (t < te) {
             const uint32_t plo = p0 + t * kT;
             const TilePos tp{t, plo, min(plo + (uint32_t)kT, p1), (uint32_t)io.tile_h[t], hend, off_end, 0u, 0u};
+            uint32_t ov[kOffRegs];
+            load_offsets(io.off, tp.hf, hend, off_end, lane, ov);
+            store_offsets(offs, lane, ov);
             tile_body<DIM, PER>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
         }
         if (g.hist_lds && g.train_points && ((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
@@ -624,10 +707,19 @@ int launch_fused_dim(const IterGeom& g, const FusedPtrs& io, const Fcn& fcn, int
     if constexpr (Fcn::kRng32) {
         if (g.per_all) kern = fused_kernel<DIM, true, Fcn>;
     }
-    if (lds > 64 * 1024) {
-        if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds) != hipSuccess)
+    // the LDS limit of this instantiation is raised once, to the largest
+    // request seen (a host-side attribute, not a per-launch cost)
+    static size_t lds_set = 64 * 1024;
+    if (lds > lds_set) {
+        if (hipFuncSetAttribute((const void*)fused_kernel<DIM, false, Fcn>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
             return VGX_EHIP;
+        if constexpr (Fcn::kRng32) {
+            if (hipFuncSetAttribute((const void*)fused_kernel<DIM, true, Fcn>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+                return VGX_EHIP;
+        }
+        lds_set = lds;
     }
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kFusedThreads), lds, st, g, io, fcn);
     return check_launch();

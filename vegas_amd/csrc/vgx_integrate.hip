@@ -324,90 +324,130 @@ __global__ __launch_bounds__(kReduceThreads) void finalize_kernel(IterGeom g, Sc
                                                                   int nsigf, const double* blk_part, int nblk,
                                                                   const double* strad_part, int nstrad) {
     __shared__ double red[kReduceThreads * 6];
+    __shared__ double mv[2];
     if (nblk + nstrad > 0 && !(sc->status || sc->overflow))
         reduce_parts(blk_part, nblk, strad_part, nstrad, sc, red);
-    DD t{0, 0};
-    if (g.adaptive_strat && !g.fast_pow)
+    const bool sig_parts = g.adaptive_strat && !g.fast_pow;
+    if (sig_parts) {
+        DD t{0, 0};
         for (int k = threadIdx.x; k < nsigf; k += blockDim.x)
             t = dd_sum(t, DD{sigf_part[2 * k], sigf_part[2 * k + 1]});
-    red[2 * threadIdx.x] = t.hi;
-    red[2 * threadIdx.x + 1] = t.lo;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < (unsigned)o) {
-            DD r = dd_sum(DD{red[2 * threadIdx.x], red[2 * threadIdx.x + 1]},
-                          DD{red[2 * (threadIdx.x + o)], red[2 * (threadIdx.x + o) + 1]});
-            red[2 * threadIdx.x] = r.hi;
-            red[2 * threadIdx.x + 1] = r.lo;
-        }
+        red[2 * threadIdx.x] = t.hi;
+        red[2 * threadIdx.x + 1] = t.lo;
         __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (threadIdx.x < (unsigned)o) {
+                DD r = dd_sum(DD{red[2 * threadIdx.x], red[2 * threadIdx.x + 1]},
+                              DD{red[2 * (threadIdx.x + o)], red[2 * (threadIdx.x + o) + 1]});
+                red[2 * threadIdx.x] = r.hi;
+                red[2 * threadIdx.x + 1] = r.lo;
+            }
+            __syncthreads();
+        }
     }
-    if (threadIdx.x != 0) return;
-    if (sc->overflow) {
-        if (st.status) *st.status = VGX_ERANGE;
-        sc->status = VGX_ERANGE;
-        return;
-    }
-    if (sc->status) {
-        if (st.status) *st.status = sc->status;
+    const double ssig_parts = sig_parts ? red[0] + red[1] : 0.0;
+    __syncthreads();
+    // The workgroup stops here on an error; every thread reads the same flags.
+    const int ovf = sc->overflow, bad = sc->status;
+    if (ovf || bad) {
+        if (threadIdx.x == 0) {
+            if (st.status) *st.status = ovf ? VGX_ERANGE : bad;
+            if (ovf) sc->status = VGX_ERANGE;
+        }
         return;
     }
     const int it = (int)g.itn;
-    const double mean = sc->acc[0] + sc->acc[1];
-    const double var = sc->acc[2] + sc->acc[3];
-    const double ssig = g.fast_pow ? sc->acc[4] + sc->acc[5] : red[0] + red[1];
-    st.itn_mean[it] = mean;
-    st.itn_var[it] = var;
-    st.itn_neval[it] = (int64_t)sc->total;
-    st.itn_nhrange[2 * it] = g.adaptive_sampler ? sc->nh_min : g.nmin;
-    st.itn_nhrange[2 * it + 1] = g.adaptive_sampler ? sc->nh_max : g.nmin;
-    sc->reset_sigf = 0;
-    // sum_sigf (pyx:2209-2215); sigf[:] = 1 is applied lazily by the next
-    // allocation, and by sigf_reset_kernel at the end of the run
-    if (g.beta > 0 && g.adapt && !g.train_errors) {
-        if (ssig > 0) {
-            *st.sum_sigf = ssig;
-        } else {
-            sc->reset_sigf = has_sigf;
-            *st.sum_sigf = has_sigf ? (double)g.nhcube : 0.0;
+    if (threadIdx.x == 0) {
+        const double mean = sc->acc[0] + sc->acc[1];
+        const double var = sc->acc[2] + sc->acc[3];
+        mv[0] = mean;
+        mv[1] = var;
+        const double ssig = g.fast_pow ? sc->acc[4] + sc->acc[5] : ssig_parts;
+        st.itn_mean[it] = mean;
+        st.itn_var[it] = var;
+        st.itn_neval[it] = (int64_t)sc->total;
+        st.itn_nhrange[2 * it] = g.adaptive_sampler ? sc->nh_min : g.nmin;
+        st.itn_nhrange[2 * it + 1] = g.adaptive_sampler ? sc->nh_max : g.nmin;
+        sc->reset_sigf = 0;
+        // sum_sigf (pyx:2209-2215); sigf[:] = 1 is applied lazily by the next
+        // allocation, and by sigf_reset_kernel at the end of the run
+        if (g.beta > 0 && g.adapt && !g.train_errors) {
+            if (ssig > 0) {
+                *st.sum_sigf = ssig;
+            } else {
+                sc->reset_sigf = has_sigf;
+                *st.sum_sigf = has_sigf ? (double)g.nhcube : 0.0;
+            }
         }
     }
-    // running average over iterations 0..it (RAvg.add, pyx:2392-2410)
+    // running average over iterations 0..it (RAvg.add, pyx:2392-2410): the
+    // earlier iterations' (mean, var) are staged through LDS in chunks (one
+    // load latency per chunk) and summed by thread 0 in iteration order
     const int n = it + 1;
-    double rm, rs, chi2 = 0.0;
+    constexpr int kChunk = kReduceThreads * 3;  // (mean, var) pairs in red[]
+    auto stage = [&](int k0) {
+        __syncthreads();
+        for (int k = k0 + (int)threadIdx.x; k < min(n, k0 + kChunk); k += blockDim.x) {
+            red[2 * (k - k0)] = k == it ? mv[0] : st.itn_mean[k];
+            red[2 * (k - k0) + 1] = k == it ? mv[1] : st.itn_var[k];
+        }
+        __syncthreads();
+    };
+    double rm = 0.0, rs = 0.0, chi2 = 0.0;
     if (g.weighted) {
         double sw = 0.0, swm = 0.0;
-        for (int k = 0; k < n; ++k) {
-            const double s = sqrt(st.itn_var[k]);
-            const double vk = s * s;
-            const double w = 1. / (vk > kTiny ? vk : kTiny);
-            sw += w;
-            swm += w * st.itn_mean[k];
+        for (int k0 = 0; k0 < n; k0 += kChunk) {
+            stage(k0);
+            if (threadIdx.x == 0)
+                for (int k = k0; k < min(n, k0 + kChunk); ++k) {
+                    const double sd = sqrt(red[2 * (k - k0) + 1]);
+                    const double vk = sd * sd;
+                    const double w = 1. / (vk > kTiny ? vk : kTiny);
+                    sw += w;
+                    swm += w * red[2 * (k - k0)];
+                }
         }
         const double v = 1. / sw;
         rm = swm * v;
         rs = sqrt(v);
         if (n > 1)
-            for (int k = 0; k < n; ++k) {
-                const double s = sqrt(st.itn_var[k]);
-                const double vk = s * s;
-                const double w = 1. / (vk > kTiny ? vk : kTiny);
-                chi2 += (rm - st.itn_mean[k]) * (rm - st.itn_mean[k]) * w;
+            for (int k0 = 0; k0 < n; k0 += kChunk) {
+                if (n > kChunk) stage(k0);
+                if (threadIdx.x == 0)
+                    for (int k = k0; k < min(n, k0 + kChunk); ++k) {
+                        const double sd = sqrt(red[2 * (k - k0) + 1]);
+                        const double vk = sd * sd;
+                        const double w = 1. / (vk > kTiny ? vk : kTiny);
+                        const double dm = rm - red[2 * (k - k0)];
+                        chi2 += dm * dm * w;
+                    }
             }
     } else {
         double ms = 0.0, vs = 0.0;
-        for (int k = 0; k < n; ++k) {
-            const double s = sqrt(st.itn_var[k]);
-            ms += st.itn_mean[k];
-            vs += s * s;
+        for (int k0 = 0; k0 < n; k0 += kChunk) {
+            stage(k0);
+            if (threadIdx.x == 0)
+                for (int k = k0; k < min(n, k0 + kChunk); ++k) {
+                    const double sd = sqrt(red[2 * (k - k0) + 1]);
+                    ms += red[2 * (k - k0)];
+                    vs += sd * sd;
+                }
         }
         rm = ms / n;
         rs = sqrt(vs / ((double)n * (double)n));
         if (n > 1) {
-            for (int k = 0; k < n; ++k) chi2 += (st.itn_mean[k] - rm) * (st.itn_mean[k] - rm);
+            for (int k0 = 0; k0 < n; k0 += kChunk) {
+                if (n > kChunk) stage(k0);
+                if (threadIdx.x == 0)
+                    for (int k = k0; k < min(n, k0 + kChunk); ++k) {
+                        const double dm = red[2 * (k - k0)] - rm;
+                        chi2 += dm * dm;
+                    }
+            }
             chi2 /= (vs / n);
         }
     }
+    if (threadIdx.x != 0) return;
     st.ravg[0] = rm;
     st.ravg[1] = rs;
     st.ravg[2] = chi2;
@@ -429,14 +469,20 @@ template <bool R32>
 __device__ inline void eval_point_h(const IterGeom& g, const AxisC* axes, const double* grid, uint32_t h,
                                     uint32_t p, double* x, double& J, int* kb) {
     PackedCoords<0> pc;
+    uint32_t zmin = 0xFFFFFFFFu;
     if constexpr (R32) {
         if (g.per_all) {
             pc.set(axes, g.dim, h);
-            eval_point<0, true, true>(tile_geom(g), axes, grid, pc, h, p, x, J, kb);
+            eval_point<0, true, true>(tile_geom(g), axes, grid, pc, h, p, x, J, kb, zmin);
+            if (zmin == 0) {
+                const uint64_t zm = per_zero_axes<0>(tile_geom(g), pc, p);
+                for (int d = 0; d < g.dim; ++d)
+                    if ((zm >> d) & 1) kb[d] = -1;
+            }
             return;
         }
     }
-    eval_point<0, R32, false>(tile_geom(g), axes, grid, pc, h, p, x, J, kb);
+    eval_point<0, R32, false>(tile_geom(g), axes, grid, pc, h, p, x, J, kb, zmin);
 }
 
 // adapt_to_errors: training entry |var_h| at the last point of every hcube
@@ -545,8 +591,7 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
     // LDS plan of the fused kernel: axis table + FP64 grid always, the
     // privatised histogram when it also fits.
     const size_t tot = (size_t)c->dim * g->nb;
-    const size_t fixed = (size_t)VGX_MAXDIM * sizeof(AxisC) + (size_t)16 * kNoff * sizeof(uint32_t) +
-                         16 * 6 * sizeof(double) + 64;
+    const size_t fixed = (size_t)16 * kNoff * sizeof(uint32_t) + 16 * 6 * sizeof(double) + 64;
     const size_t gbytes = (size_t)c->dim * c->ld * sizeof(double);
     const size_t hbytes = tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t) + 16;
     g->grid_lds = 1;
@@ -557,8 +602,7 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
 
 static size_t fused_lds(const IterGeom& g) {
     const size_t tot = (size_t)g.dim * g.nb;
-    size_t b = (size_t)VGX_MAXDIM * sizeof(AxisC);
-    b += (size_t)g.dim * g.ld * sizeof(double);
+    size_t b = (size_t)g.dim * g.ld * sizeof(double);
     if (g.hist_lds) b += tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t);
     b = (b + 15) & ~(size_t)15;
     b += (size_t)16 * kNoff * sizeof(uint32_t);

@@ -98,6 +98,15 @@ struct FusedPtrs {
 // Integrands (device functors).  eval() receives x in FP64 and returns f in
 // FP64; the arithmetic precision is the functor's.
 
+// Reads a parameter that no kernel writes while the integrand runs, through
+// the constant address space: at a wave-uniform index (the component loop)
+// this is a scalar load into SGPRs rather than a vector load with its full
+// memory latency per component.
+template <class T>
+__device__ __forceinline__ T ldc(const T* p, int i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 // f(x) = sum_k A_k exp(-B_k |x - c_k|^2)
 struct GaussMix64 {
     static constexpr bool kRng32 = false;
@@ -114,10 +123,10 @@ struct GaussMix64 {
 #pragma unroll
             for (int d = 0; d < nd_max(DIM); ++d) {
                 if (DIM <= 0 && d >= nd) break;
-                double t = x[d] - c[k * nd + d];
+                double t = x[d] - ldc(c, k * nd + d);
                 r2 += t * t;
             }
-            acc += A[k] * exp(-B[k] * r2);
+            acc += ldc(A, k) * exp(-ldc(B, k) * r2);
         }
         return acc;
     }
@@ -144,10 +153,10 @@ struct GaussMix32 {
 #pragma unroll
             for (int d = 0; d < nd_max(DIM); ++d) {
                 if (DIM <= 0 && d >= nd) break;
-                float t = xf[d] - c[k * nd + d];
+                float t = xf[d] - ldc(c, k * nd + d);
                 r2 = __builtin_fmaf(t, t, r2);
             }
-            acc = __builtin_fmaf(A[k], __expf(-B[k] * r2), acc);
+            acc = __builtin_fmaf(ldc(A, k), __expf(-ldc(B, k) * r2), acc);
         }
         return (double)acc;
     }
@@ -183,9 +192,9 @@ struct DiagRidge32 {
         const float fd = (float)nd;
         float acc = 0.0f;
         for (int k = 0; k < ncomp; ++k) {
-            float t = xbar - c[k];
+            float t = xbar - ldc(c, k);
             float r2 = __builtin_fmaf(fd * t, t, P);
-            acc = __builtin_fmaf(A[k], __expf(-B[k] * r2), acc);
+            acc = __builtin_fmaf(ldc(A, k), __expf(-ldc(B, k) * r2), acc);
         }
         return (double)acc;
     }
@@ -216,8 +225,8 @@ struct DiagRidge64 {
         }
         double acc = 0.0;
         for (int k = 0; k < ncomp; ++k) {
-            double t = xbar - c[k];
-            acc += A[k] * exp(-B[k] * (P + (double)nd * t * t));
+            double t = xbar - ldc(c, k);
+            acc += ldc(A, k) * exp(-ldc(B, k) * (P + (double)nd * t * t));
         }
         return acc;
     }
