@@ -242,3 +242,42 @@ def test_seed_determinism(va):
     b = va.Integrator([[0, 1]] * 4, neval=2e5, seed=42)(f, nitn=4)
     for x, y in zip(a.itn_results[:1], b.itn_results[:1]):
         assert x.mean == y.mean and x.sdev == y.sdev
+
+
+def test_diagonal_ridge_first_iteration_parity(va, orc):
+    """The diagonal-ridge functor (|x - t 1|^2 = P + d (xbar - t)^2) against
+    the oracle's plain Gaussian mixture with centres t 1: FP64, same uniforms,
+    so only the integrand's algebra and exp rounding differ."""
+    dim, nc, sigma = 6, 20, 0.05
+    t = 0.3 + 0.4 * np.arange(nc) / (nc - 1)
+    A = np.full(nc, (2 * math.pi * sigma ** 2) ** (-dim / 2) / nc)
+    B = np.full(nc, 1 / (2 * sigma ** 2))
+    f = va.DiagonalRidge(A, B, t, dim)
+    fo = orc.GaussMix(A, B, t[:, None] * np.ones(dim))
+    integ = va.Integrator([[0, 1]] * dim, neval=2e5, seed=21, precision='f64', exact=True, alpha=0.0)
+    r = integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * dim, neval=2e5, seed=21, alpha=0.0)
+    out, sf, nf = o.iterate(fo)
+    assert integ.last_neval == out.last_neval
+    m = r.itn_results[0]
+    assert abs(m.mean - out.mean) <= 1e-10 * abs(out.mean)
+    assert abs(m.var - out.var) <= 1e-8 * out.var
+
+
+def test_ridge_8d_workload(va):
+    """The 8-D ridge workload (1000 Gaussians, sigma 0.005, neval 2e7):
+    the reference rule's geometry, and the diagonal and generic functors
+    giving the same first iteration on the same points (FP32 rounding)."""
+    from vegas_amd import workloads
+    res = {}
+    for diag in (True, False):
+        wl = workloads.ridge_8d(diagonal=diag)
+        integ = va.Integrator([[0, 1]] * 8, neval=wl["neval"], seed=2026, precision='f32')
+        assert list(integ.nstrat) == [7, 7, 6, 6, 6, 6, 6, 6]
+        assert integ.nhcube == 2286144
+        assert list(integ.map.ninc) == [994, 994] + [996] * 6
+        r = integ(wl["f"], nitn=1)
+        res[diag] = r.itn_results[0]
+        assert math.isfinite(res[diag].mean) and res[diag].sdev > 0
+    assert abs(res[True].mean - res[False].mean) <= 1e-4 * abs(res[False].mean)
+    assert abs(res[True].sdev - res[False].sdev) <= 1e-3 * res[False].sdev

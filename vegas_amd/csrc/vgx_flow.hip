@@ -664,39 +664,94 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
 // virtual all-ones B row tile for the biases.  8 waves: wave w accumulates
 // dW2 rows 16w.. (8 tiles + bias), dW1 rows 16w.. (1 tile) and, for w < 2,
 // dW3 rows 16w.. (8 tiles + bias).  Partial sums per workgroup.
+// The operand tiles (kRows x 16 bf16, 17.5 KiB) stream from HBM: the
+// workgroup copies each tile into one of two LDS buffers with 16-byte loads
+// that are issued kWgAhead tiles ahead (a register ring), so several tiles
+// are in flight while the MFMAs run on the current one.
 constexpr int kWgThreads = 512;
 constexpr int kWgChunk = 256;   // sample tiles per workgroup
+constexpr int kWgAhead = 4;     // tiles in flight per workgroup
+constexpr int kTileChunks = kRows * 16 * 2 / 16;                          // 16-byte pieces per tile
+constexpr int kChunksPerThread = (kTileChunks + kWgThreads - 1) / kWgThreads;
+
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 __device__ inline bf16x4_t frag_op(const uint16_t* tileops, int row0, int lane) {
     return *reinterpret_cast<const bf16x4_t*>(tileops + (row0 + (lane & 15)) * 16 + (lane >> 4) * 4);
 }
 
+struct WgRegs {
+    u32x4_t v[kChunksPerThread];
+};
+
+__device__ inline void wg_fetch(const uint16_t* ops, uint64_t t, uint64_t t1, WgRegs& r) {
+    if (t >= t1) return;
+    const u32x4_t* src = reinterpret_cast<const u32x4_t*>(ops + (size_t)t * kRows * 16);
+#pragma unroll
+    for (int j = 0; j < kChunksPerThread; ++j) {
+        const int c = threadIdx.x + kWgThreads * j;
+        if (c < kTileChunks) r.v[j] = __builtin_nontemporal_load(src + c);
+    }
+}
+
+__device__ inline void wg_put(uint16_t* buf, const WgRegs& r) {
+    u32x4_t* dst = reinterpret_cast<u32x4_t*>(buf);
+#pragma unroll
+    for (int j = 0; j < kChunksPerThread; ++j) {
+        const int c = threadIdx.x + kWgThreads * j;
+        if (c < kTileChunks) dst[c] = r.v[j];
+    }
+}
+
+struct WgAcc {
+    f32x4_t acc2[9], acc3[9], acc1;
+};
+
+__device__ inline void wg_mfma(const uint16_t* to, int w, int lane, const bf16x4_t ones, WgAcc& A) {
+    const bf16x4_t a2 = frag_op(to, kRdh2 + 16 * w, lane);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) A.acc2[q] = mfma16(a2, frag_op(to, kRh1 + 16 * q, lane), A.acc2[q]);
+    A.acc2[8] = mfma16(a2, ones, A.acc2[8]);
+    A.acc1 = mfma16(frag_op(to, kRdh1 + 16 * w, lane), frag_op(to, kRa, lane), A.acc1);
+    if (w < 2) {
+        const bf16x4_t a3 = frag_op(to, kRdo + 16 * w, lane);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) A.acc3[q] = mfma16(a3, frag_op(to, kRh2 + 16 * q, lane), A.acc3[q]);
+        A.acc3[8] = mfma16(a3, ones, A.acc3[8]);
+    }
+}
+
 __global__ __launch_bounds__(kWgThreads) void flow_wgrad_kernel(const uint16_t* ops, uint64_t ntiles, float* part) {
+    __shared__ __align__(16) uint16_t Lt[2][kRows * 16];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
-    f32x4_t acc2[9], acc1, acc3[9];
+    WgAcc A;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-        acc2[q] = zero;
-        acc3[q] = zero;
+        A.acc2[q] = zero;
+        A.acc3[q] = zero;
     }
-    acc1 = zero;
+    A.acc1 = zero;
     bf16x4_t ones;
 #pragma unroll
     for (int j = 0; j < 4; ++j) ones[j] = (lane & 15) == 0 ? (short)0x3F80 : (short)0;
     const uint64_t t0 = (uint64_t)blockIdx.x * kWgChunk, t1 = min(t0 + kWgChunk, ntiles);
-    for (uint64_t t = t0; t < t1; ++t) {
-        const uint16_t* to = ops + (size_t)t * kRows * 16;
-        const bf16x4_t a2 = frag_op(to, kRdh2 + 16 * w, lane);
+    WgRegs R[kWgAhead];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) acc2[q] = mfma16(a2, frag_op(to, kRh1 + 16 * q, lane), acc2[q]);
-        acc2[8] = mfma16(a2, ones, acc2[8]);
-        acc1 = mfma16(frag_op(to, kRdh1 + 16 * w, lane), frag_op(to, kRa, lane), acc1);
-        if (w < 2) {
-            const bf16x4_t a3 = frag_op(to, kRdo + 16 * w, lane);
+    for (int k = 0; k < kWgAhead; ++k) wg_fetch(ops, t0 + k, t1, R[k]);
+    // tile t: its registers R[(t - t0) % kWgAhead] go to Lt[(t - t0) & 1];
+    // one barrier per tile (the other buffer's readers finished before it)
+    for (uint64_t tb = t0; tb < t1; tb += kWgAhead) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) acc3[q] = mfma16(a3, frag_op(to, kRh2 + 16 * q, lane), acc3[q]);
-            acc3[8] = mfma16(a3, ones, acc3[8]);
+        for (int k = 0; k < kWgAhead; ++k) {
+            const uint64_t t = tb + k;
+            if (t < t1) {
+                uint16_t* buf = Lt[k & 1];
+                wg_put(buf, R[k]);
+                __syncthreads();
+                wg_fetch(ops, t + kWgAhead, t1, R[k]);
+                wg_mfma(buf, w, lane, ones, A);
+            }
         }
     }
     // C layout: lane l, i -> dW[16 w + (l/16)*4 + i][16 q + l%16]
@@ -705,22 +760,31 @@ __global__ __launch_bounds__(kWgThreads) void flow_wgrad_kernel(const uint16_t* 
     for (int i = 0; i < 4; ++i) {
         const int r = 16 * w + (lane >> 4) * 4 + i;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) o[kPW2 + r * kFH + 16 * q + (lane & 15)] = acc2[q][i];
-        if ((lane & 15) == 0) o[kPB2 + r] = acc2[8][i];
-        o[kPW1 + r * kFI + (lane & 15)] = acc1[i];
+        for (int q = 0; q < 8; ++q) o[kPW2 + r * kFH + 16 * q + (lane & 15)] = A.acc2[q][i];
+        if ((lane & 15) == 0) o[kPB2 + r] = A.acc2[8][i];
+        o[kPW1 + r * kFI + (lane & 15)] = A.acc1[i];
         if (w < 2) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) o[kPW3 + r * kFH + 16 * q + (lane & 15)] = acc3[q][i];
-            if ((lane & 15) == 0) o[kPB3 + r] = acc3[8][i];
+            for (int q = 0; q < 8; ++q) o[kPW3 + r * kFH + 16 * q + (lane & 15)] = A.acc3[q][i];
+            if ((lane & 15) == 0) o[kPB3 + r] = A.acc3[8][i];
         }
     }
 }
 
-// Sum of the workgroup partials (fixed order) into grad [kPLayer].
+// Sum of the workgroup partials (fixed order: part 0, 1, ...) into grad
+// [kPLayer]; the loads of eight parts are issued before their adds.
 __global__ void flow_wgrad_reduce_kernel(const float* part, int nparts, float* grad) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kPLayer; i += gridDim.x * blockDim.x) {
         double s = 0.0;
-        for (int p = 0; p < nparts; ++p) s += part[(size_t)p * kPLayer + i];
+        int p = 0;
+        for (; p + 8 <= nparts; p += 8) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = part[(size_t)(p + j) * kPLayer + i];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s += v[j];
+        }
+        for (; p < nparts; ++p) s += part[(size_t)p * kPLayer + i];
         grad[i] = (float)s;
     }
 }
@@ -1054,7 +1118,8 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
         hipLaunchKernelGGL(flow_backward_kernel, dim3(bb), dim3(kBwdThreads), blds, s, flow_args(fl), b);
         hipLaunchKernelGGL(flow_wgrad_kernel, dim3(nparts), dim3(kWgThreads), 0, s, (const uint16_t*)fl->ops.ptr,
                            ntiles, (float*)fl->wpart.ptr);
-        hipLaunchKernelGGL(flow_wgrad_reduce_kernel, dim3(64), dim3(256), 0, s, (const float*)fl->wpart.ptr, nparts,
+        hipLaunchKernelGGL(flow_wgrad_reduce_kernel, dim3((kPLayer + 255) / 256), dim3(256), 0, s,
+                           (const float*)fl->wpart.ptr, nparts,
                            fl->grad + (size_t)k * kPLayer);
         if ((rc = check_launch())) return rc;
     }
