@@ -221,7 +221,7 @@ def flow_ridge(a):
         "scaling": "strong", "vs_baseline": None, "dtype": "bf16 (flow MLP), f32 (integrand), f64 (sums)",
         "data": "synthetic (device Philox samples)",
         "config": {"workload": "9-D ridge: 500 diagonal Gaussians sigma 0.01, [0,1]^9; 6-layer affine coupling "
-                               "flow, MLP 9-128-128 per layer, trained on 2e7 vegas samples in 1e6 batches",
+                               "flow, MLP 9-128-128 per layer, trained on 2e7 vegas samples in 2.5e5 batches",
                    "parallelism": "single GPU"},
         "result": {"flow": {"mean": est.mean, "sdev": est.sdev, "samples": int(nflow), "train_s": t_train,
                             "reached_target": bool(flow_ok)},

@@ -27,6 +27,7 @@ extern "C" {
 #endif
 
 #define VGX_MAXDIM 64
+#define VGX_MAXCOMP 64  /* components of a multi-component integrand */
 
 enum {
     VGX_OK = 0,
@@ -179,6 +180,21 @@ int vgx_iter_accumulate(vgx_ctx* ctx, const vgx_config* cfg, vgx_state* st, int 
                         const double* f, void* stream);
 int vgx_iter_end(vgx_ctx* ctx, const vgx_config* cfg, vgx_state* st, int it,
                  void* stream);
+/* Multi-component integrands (fcn_size m > 1; pyx:2136-2183 and its
+ * correlate_integrals switch): for the batch [h0,h1) / [p0,p1) of the
+ * iteration opened by vgx_iter_begin, f is device double[n][m] (n = p1-p0)
+ * and wgt the weights vgx_iter_sample gave for the same points.  Adds the
+ * batch's sums to acc (device double, double-double (hi, lo) pairs, zeroed
+ * by the caller per iteration): mean[m], then the covariance lower triangle
+ * row by row ((0,0) (1,0) (1,1) (2,0) ...: m(m+1)/2 pairs) when correlate,
+ * else the m variances.  Component 0 alone steers the sampling: the caller
+ * passes f[:,0] to vgx_iter_accumulate for sigf and the training data.
+ * status (device int32, optional): VGX_ERANGE when an hcube's points are
+ * not inside [p0, p1) (those hcubes are skipped, nothing is read).
+ * Replaces the fcn_size loops of Integrator.__call__ (pyx:2146-2183).   */
+int vgx_iter_accumulate_cov(vgx_ctx* ctx, const vgx_config* cfg, int64_t h0, int64_t h1,
+                            int64_t p0, int64_t p1, const double* f, const double* wgt,
+                            int m, int correlate, double* acc, int32_t* status, void* stream);
 
 /* Timing of the fused sampler kernel: vgx_set_timing(ctx, 1) clears the
  * record and brackets every following fused launch (up to 256) with HIP
@@ -266,7 +282,9 @@ int vgx_flow_set_params(vgx_flow* fl, const float* host, void* stream);
 int vgx_flow_get_params(const vgx_flow* fl, float* host, float* grad_host, void* stream);
 /* Importance-sampling estimate: n points z (Philox: key seed, counter
  * (base + i, iteration, lane group)), x = F(z), f(x) J(z) summed:
- * sums[0] = sum f J, sums[1] = sum (f J)^2 (host).  fcn: GAUSSMIX or
+ * sums[0] = sum f J, sums[1] = sum (f J)^2 (host; the call then waits for
+ * the stream).  sums may be NULL when xout and fout are given: nothing is
+ * read back and the call returns without waiting.  fcn: GAUSSMIX or
  * DIAGRIDGE with device FP64 parameters (evaluated in FP32).  xout, fout
  * (device, optional): the points and f J.  status: device int32, set to
  * VGX_ENAN when f is NaN.                                                 */

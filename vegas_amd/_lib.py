@@ -80,6 +80,8 @@ def lib():
     L.vgx_iter_accumulate.argtypes = [vp, C.POINTER(VgxConfig), C.POINTER(VgxState), C.c_int,
                                       C.c_int64, C.c_int64, C.c_int64, C.c_int64, vp, vp]
     L.vgx_iter_end.argtypes = [vp, C.POINTER(VgxConfig), C.POINTER(VgxState), C.c_int, vp]
+    L.vgx_iter_accumulate_cov.argtypes = [vp, C.POINTER(VgxConfig), C.c_int64, C.c_int64, C.c_int64,
+                                          C.c_int64, vp, vp, C.c_int, C.c_int, vp, vp, vp]
     L.vgx_set_timing.argtypes = [vp, C.c_int]
     L.vgx_fused_times.argtypes = [vp, C.POINTER(C.c_float), C.c_int]
     _lib = L

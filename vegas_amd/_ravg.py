@@ -14,6 +14,7 @@ import math
 import numpy as np
 
 TINY = 10 ** (-307 + 50)  # pyx:34, sys.float_info.min_10_exp + 50
+EPSILON = 2.220446049250313e-16 * 1e4  # pyx:36, sys.float_info.epsilon * 1e4
 
 
 def gammaQ(a, x):
@@ -222,3 +223,240 @@ class RAvg(GVar):
 def _rebuild_ravg(weighted, results, sum_neval):
     return RAvg(weighted=weighted, itn_results=[GVar(m, s) for m, s in results],
                 sum_neval=sum_neval)
+
+
+# ------------------------------------------------------------------ arrays --
+def _inv_sqrt(matrix, svdcut):
+    """Rows w_i = v_i / sqrt(lambda_i) of the eigen-decomposition of the
+    symmetric ``matrix`` (so that sum_i w_i w_i^T is its inverse on the kept
+    modes).  svdcut < 0 drops the modes with lambda < |svdcut| max(lambda),
+    as gvar.SVD(matrix, svdcut, rescale=False).decomp(-1) does for the
+    reference's RAvgArray._w (pyx:2715-2721); svdcut <= 0 here."""
+    matrix = np.asarray(matrix, np.float64)
+    vals, vecs = np.linalg.eigh(matrix)
+    top = vals.max() if vals.size else 0.0
+    keep = (vals > abs(svdcut) * top) & (vals > 0)
+    return (vecs[:, keep] / np.sqrt(vals[keep])).T
+
+
+class GVarArray:
+    """Estimates of several integrals: ``mean``, ``cov`` (covariance matrix)
+    and ``sdev`` arrays of shape ``shape``; ``self[i]`` is a GVar."""
+
+    def __init__(self, mean, cov, shape=None):
+        mean = np.asarray(mean, np.float64).reshape(-1)
+        self._m = mean
+        self._cov = np.asarray(cov, np.float64).reshape(len(mean), len(mean))
+        self.shape = tuple(shape) if shape is not None else (len(mean),)
+
+    @property
+    def mean(self):
+        return self._m.reshape(self.shape)
+
+    @property
+    def cov(self):
+        return self._cov
+
+    @property
+    def sdev(self):
+        return np.sqrt(np.maximum(np.diag(self._cov), 0.0)).reshape(self.shape)
+
+    @property
+    def var(self):
+        return self.sdev ** 2
+
+    @property
+    def size(self):
+        return len(self._m)
+
+    def flat_gvars(self):
+        sd = np.sqrt(np.maximum(np.diag(self._cov), 0.0))
+        return [GVar(m, s) for m, s in zip(self._m, sd)]
+
+    def __getitem__(self, idx):
+        g = np.empty(self.size, dtype=object)
+        g[:] = self.flat_gvars()
+        return g.reshape(self.shape)[idx]
+
+    def __len__(self):
+        return self.shape[0]
+
+    def __iter__(self):
+        for i in range(len(self)):
+            yield self[i]
+
+    def __str__(self):
+        g = np.empty(self.size, dtype=object)
+        g[:] = [str(x) for x in self.flat_gvars()]
+        return str(g.reshape(self.shape).tolist()).replace("'", "")
+
+    __repr__ = __str__
+
+
+class RAvgArray(GVarArray):
+    """Running average of array-valued estimates (pyx:2453-2831).  weighted:
+    the covariance-weighted average, each iteration's estimate rescaled by
+    |mean| (or sdev where that is larger) of the first one and decomposed
+    with the svdcut -EPSILON*len*1e4 of the reference; otherwise the plain
+    average with covariance sum(cov)/n^2.  The per-iteration estimates
+    ``itn_results`` are GVarArrays."""
+
+    def __init__(self, shape, weighted=True, itn_results=None, sum_neval=0, rescale=True):
+        n = int(np.prod(shape)) if len(shape) else 1
+        super().__init__(np.zeros(n), np.zeros((n, n)), shape)
+        self.weighted = weighted
+        self.rescale = rescale
+        self._mlist, self._wlist = [], []
+        self._msum, self._covsum, self._n = 0.0, 0.0, 0
+        self._invw = None
+        self.itn_results = []
+        self.sum_neval = sum_neval
+        for r in itn_results or []:
+            self.add(r)
+
+    def _w(self, matrix):
+        return _inv_sqrt(matrix, -EPSILON * len(matrix) * 1e4)
+
+    def add(self, g):
+        """Add the estimate ``g`` (a GVarArray, or (mean, cov)) (pyx:2775-2811)."""
+        if not isinstance(g, GVarArray):
+            g = GVarArray(g[0], g[1], self.shape)
+        self.itn_results.append(g)
+        gm = g._m.copy()
+        gc = g._cov.copy()
+        if self.weighted:
+            if not hasattr(self, '_rescale'):
+                if self.rescale is not None:
+                    rs = np.fabs(gm)
+                    gsd = np.sqrt(np.maximum(np.diag(gc), 0.0))
+                    idx = gsd > rs
+                    rs[idx] = gsd[idx]
+                    rs[rs <= 0] = 1.
+                    self._rescale = rs
+                else:
+                    self._rescale = np.ones(len(gm))
+            rs = self._rescale
+            gm = gm / rs
+            gc = gc / np.outer(rs, rs)
+            d = np.diag(gc).copy()
+            d[d <= 0] = TINY
+            gc[np.diag_indices_from(gc)] = d
+            self._mlist.append(gm)
+            self._wlist.append(self._w(gc))
+            invcov = np.sum([w.T.dot(w) for w in self._wlist], axis=0)
+            invw = self._w(invcov)
+            cov = invw.T.dot(invw)
+            mean = np.zeros(len(gm))
+            for m, w in zip(self._mlist, self._wlist):
+                for wj in w:
+                    wj_m = wj.dot(m)
+                    for invwi in invw:
+                        mean += invwi * invwi.dot(wj) * wj_m
+            self._m = mean * rs
+            self._cov = cov * np.outer(rs, rs)
+        else:
+            self._mlist.append(gm)
+            self._msum = self._msum + gm
+            self._covsum = self._covsum + gc
+            self._invw = None
+            self._n += 1
+            self._m = self._msum / self._n
+            self._cov = self._covsum / self._n ** 2
+
+    def extend(self, ravg):
+        for r in ravg.itn_results:
+            self.add(r)
+        self.sum_neval += ravg.sum_neval
+
+    @property
+    def chi2(self):
+        """pyx:2727-2748"""
+        if len(self.itn_results) <= 1:
+            return 0.0
+        if self.weighted:
+            wavg = self._m / self._rescale
+            ans = 0.0
+            for w, m in zip(self._wlist, self._mlist):
+                for wi in w:
+                    ans += wi.dot(m - wavg) ** 2
+            return float(ans)
+        if self._invw is None:
+            self._invw = self._w(self._covsum / self._n)
+        ans = 0.0
+        for m in self._mlist:
+            delta = self._m - m
+            for invwi in self._invw:
+                ans += invwi.dot(delta) ** 2
+        return float(ans)
+
+    @property
+    def dof(self):
+        """pyx:2751-2758"""
+        if len(self.itn_results) <= 1:
+            return 0
+        if not self.weighted:
+            if self._invw is None:
+                self._invw = self._w(self._covsum / self._n)
+            return (len(self.itn_results) - 1) * len(self._invw)
+        return int(np.sum([len(w) for w in self._wlist]) - self.size)
+
+    @property
+    def nitn(self):
+        return len(self.itn_results)
+
+    @property
+    def Q(self):
+        if self.dof <= 0 or self.chi2 < 0:
+            return float('nan')
+        return gammaQ(self.dof / 2., self.chi2 / 2.)
+
+    @property
+    def avg_neval(self):
+        return self.sum_neval / self.nitn if self.nitn > 0 else 0
+
+    def converged(self, rtol, atol):
+        """pyx:2723-2726"""
+        return bool(np.all(self.sdev < atol + rtol * np.abs(self.mean)))
+
+
+class RAvgDict(dict):
+    """Running average of dictionary-valued estimates: one RAvgArray over the
+    flattened values (keys in the integrand's order), viewed per key."""
+
+    def __init__(self, layout, weighted=True):
+        super().__init__()
+        self.layout = layout  # [(key, shape, start, stop)]
+        n = layout[-1][3] if layout else 0
+        self.rarray = RAvgArray((n,), weighted=weighted)
+        self._refresh()
+
+    def _refresh(self):
+        for k, shape, a, b in self.layout:
+            if shape == ():
+                self[k] = GVar(self.rarray._m[a], math.sqrt(max(self.rarray._cov[a, a], 0.0)))
+            else:
+                self[k] = GVarArray(self.rarray._m[a:b], self.rarray._cov[a:b, a:b], shape)
+
+    def add(self, g):
+        self.rarray.add(g)
+        self._refresh()
+
+    @property
+    def itn_results(self):
+        return self.rarray.itn_results
+
+    @property
+    def sum_neval(self):
+        return self.rarray.sum_neval
+
+    @sum_neval.setter
+    def sum_neval(self, v):
+        self.rarray.sum_neval = v
+
+    chi2 = property(lambda self: self.rarray.chi2)
+    dof = property(lambda self: self.rarray.dof)
+    Q = property(lambda self: self.rarray.Q)
+    nitn = property(lambda self: self.rarray.nitn)
+
+    def converged(self, rtol, atol):
+        return self.rarray.converged(rtol, atol)

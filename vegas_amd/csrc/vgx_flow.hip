@@ -564,23 +564,36 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
     const uint32_t am = f.amask[k];
     const uint64_t ntiles = (a.n + 15) / 16;
     const int nw = blockDim.x >> 6;
-    for (uint64_t tile = (uint64_t)blockIdx.x * nw + (threadIdx.x >> 6); tile < ntiles;
-         tile += (uint64_t)gridDim.x * nw) {
+    // The inputs of a wave's next tile (saved u, c, and G or u0) are loaded
+    // while the current tile computes.
+    const f32x4_t zero4 = {0.f, 0.f, 0.f, 0.f};
+    const uint64_t stride = (uint64_t)gridDim.x * nw;
+    auto fetch = [&](uint64_t tl, f32x4_t& u, f32x4_t& gr, float& c) {
+        const uint64_t s2 = tl * 16 + (lane & 15);
+        u = zero4;
+        gr = zero4;
+        c = 0.f;
+        if (tl < ntiles && s2 < a.n) {
+            u = *reinterpret_cast<const f32x4_t*>(a.saved + ((size_t)k * a.n + s2) * 16 + 4 * g);
+            c = a.c[s2];
+            gr = *reinterpret_cast<const f32x4_t*>((k == 0 ? a.u0 : a.G) + s2 * 16 + 4 * g);
+        }
+    };
+    uint64_t tile = (uint64_t)blockIdx.x * nw + (threadIdx.x >> 6);
+    f32x4_t nx_u, nx_g;
+    float nx_c;
+    fetch(tile, nx_u, nx_g, nx_c);
+    for (; tile < ntiles; tile += stride) {
         const uint64_t sl = tile * 16 + (lane & 15);
         const bool live = sl < a.n;
-        f32x4_t uin = {0.f, 0.f, 0.f, 0.f}, G = {0.f, 0.f, 0.f, 0.f};
-        float cw = 0.f;
-        if (live) {
-            uin = *reinterpret_cast<const f32x4_t*>(a.saved + ((size_t)k * a.n + sl) * 16 + 4 * g);
-            cw = a.c[sl] * inv_c;
-            if (k == 0) {
-                const f32x4_t u0 = *reinterpret_cast<const f32x4_t*>(a.u0 + sl * 16 + 4 * g);
+        const f32x4_t uin = nx_u, graw = nx_g;
+        const float cw = nx_c * inv_c;
+        fetch(tile + stride, nx_u, nx_g, nx_c);
+        f32x4_t G = graw;
+        if (k == 0) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    G[i] = (4 * g + i < f.D) ? -cw * (1.0f - 2.0f / (1.0f + __expf(-u0[i]))) : 0.f;
-            } else {
-                G = *reinterpret_cast<const f32x4_t*>(a.G + sl * 16 + 4 * g);
-            }
+            for (int i = 0; i < 4; ++i)
+                G[i] = (live && 4 * g + i < f.D) ? -cw * (1.0f - 2.0f / (1.0f + __expf(-graw[i]))) : 0.f;
         }
         const bf16x4_t ain = cond_input(uin, am, f.D, lane);
         f32x4_t o[2];
@@ -1010,7 +1023,7 @@ static int flow_fcn_params(vgx_flow* fl, const vgx_integrand* fcn, FlowFcn* F, h
 extern "C" int vgx_flow_sample(vgx_flow* fl, const vgx_integrand* fcn, int64_t n, uint64_t base, uint64_t seed,
                                uint32_t iteration, double* sums, float* xout, float* fout, int32_t* status,
                                void* stream) {
-    if (!fl || !sums || n < 1) return VGX_EINVAL;
+    if (!fl || n < 1 || (!sums && (!xout || !fout))) return VGX_EINVAL;
     hipStream_t s = (hipStream_t)stream;
     FlowFcn F{};
     int rc = flow_fcn_params(fl, fcn, &F, s);
@@ -1036,6 +1049,7 @@ extern "C" int vgx_flow_sample(vgx_flow* fl, const vgx_integrand* fcn, int64_t n
         return VGX_EHIP;
     hipLaunchKernelGGL(flow_sample_kernel, dim3(blocks), dim3(kFlowThreads), lds, s, flow_args(fl), F, a);
     if ((rc = check_launch())) return rc;
+    if (!sums) return VGX_OK;
     std::vector<double> h((size_t)blocks * 4);
     VGX_HIP(hipMemcpyAsync(h.data(), a.part, h.size() * 8, hipMemcpyDeviceToHost, s));
     VGX_HIP(hipStreamSynchronize(s));

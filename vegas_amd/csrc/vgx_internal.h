@@ -60,6 +60,8 @@ struct vgx_ctx {
     vgx::Scratch hist_cnt, hist_sorted, hist_start;
     // integrator
     vgx::Scratch nh, off, tile_h, blocksum, tile_slots, blk_part, strad_part, misc, errv;
+    // multi-component integrands: per-workgroup partials
+    vgx::Scratch cov_part;
     // adapt
     vgx::Scratch adapt_tmp;
     static constexpr int kMaxTimed = 256;

@@ -27,7 +27,7 @@ import warnings
 import numpy as np
 
 from . import _geometry, _lib
-from ._ravg import RAvg, GVar
+from ._ravg import RAvg, RAvgArray, RAvgDict, GVar, GVarArray
 from .adaptive_map import AdaptiveMap
 from .integrands import DeviceIntegrand
 
@@ -49,6 +49,31 @@ def _region_map(region):
     else:
         grid = [list(np.asarray(g, float).reshape(-1)) for g in arr.reshape(-1)]
     return AdaptiveMap(grid)
+
+
+def _as_device(torch, fx):
+    if isinstance(fx, torch.Tensor):
+        return fx.to(device='cuda', dtype=torch.float64)
+    return torch.as_tensor(np.asarray(fx, dtype=float), dtype=torch.float64).cuda()
+
+
+def _flatten_dict(torch, fx, n, kind, call):
+    """A dictionary of batch arrays (the reference's dict-valued batch
+    integrands) as one (n, m) array, keys in the integrand's order; the
+    layout (key, shape, first, last column) is recorded on the first call."""
+    cols, layout, a = [], [], 0
+    for k, v in fx.items():
+        v = _as_device(torch, v)
+        shape = tuple(v.shape[1:]) if kind != 'rbatch' else tuple(v.shape[:-1])
+        size = int(np.prod(shape)) if shape else 1
+        cols.append(v.reshape(-1, n).T if kind == 'rbatch' else v.reshape(n, -1))
+        layout.append((k, shape, a, a + size))
+        a += size
+    if call.layout is None:
+        call.layout = layout
+    elif [(k, sh) for k, sh, _, _ in call.layout] != [(k, sh) for k, sh, _, _ in layout]:
+        raise ValueError('integrand output changed structure')
+    return torch.cat(cols, dim=1)
 
 
 class Integrator:
@@ -309,16 +334,28 @@ class Integrator:
         if kargs:
             self.set(kargs)
         torch = _lib.torch_cuda()
-        result = RAvg(weighted=self.adapt)
         device_fcn = isinstance(fcn, DeviceIntegrand)
         if device_fcn and fcn.dim != self.dim:
             raise ValueError('integrand dimension %d != integrator dimension %d' % (fcn.dim, self.dim))
         prec = self._precision(device_fcn)
+        std = None if device_fcn else self._std_integrand(fcn)
+        layout = None if device_fcn else self._probe_layout(torch, std)
+        # scalar integrands give an RAvg, arrays an RAvgArray of their shape,
+        # dictionaries an RAvgDict (pyx:2058-2064)
+        if layout is None or (len(layout) == 1 and layout[0][0] is None and layout[0][1] == ()):
+            result = RAvg(weighted=self.adapt)
+            if std is not None:
+                std.multi = False
+        elif len(layout) == 1 and layout[0][0] is None:
+            result = RAvgArray(layout[0][1], weighted=self.adapt)
+            std.multi = True
+        else:
+            result = RAvgDict(layout, weighted=self.adapt)
+            std.multi = True
         per_itn = bool(self.rtol or self.atol or self.analyzer is not None or save or saveall)
         if device_fcn and not per_itn:
             self._run_device(torch, fcn, self.nitn, prec, result)
         else:
-            std = None if device_fcn else self._std_integrand(fcn)
             for itn in range(self.nitn):
                 if self.analyzer is not None:
                     self.analyzer.begin(itn, self)
@@ -379,28 +416,41 @@ class Integrator:
                 xa, ja = x, jac
             if on_gpu:
                 fx = fcn(xa) if ja is None else fcn(xa, jac=ja)
-                fx = torch.as_tensor(fx, dtype=torch.float64, device='cuda')
             else:
                 xn = xa.cpu().numpy()
                 jn = None if ja is None else ja.cpu().numpy()
                 if kind in ('lbatch', 'rbatch'):
                     fx = fcn(xn) if jn is None else fcn(xn, jac=jn)
                 else:
-                    fx = np.array([fcn(xi) if jn is None else fcn(xi, jac=jn[i])
-                                   for i, xi in enumerate(xn)], dtype=float)
-                fx = np.asarray(fx, dtype=float)
-                fx = torch.as_tensor(fx, dtype=torch.float64).cuda()
+                    vals = [fcn(xi) if jn is None else fcn(xi, jac=jn[i]) for i, xi in enumerate(xn)]
+                    if vals and isinstance(vals[0], dict):
+                        fx = {k: np.array([v[k] for v in vals], dtype=float) for k in vals[0]}
+                    else:
+                        fx = np.array(vals, dtype=float)
             n = x.shape[0]
-            if kind == 'rbatch':
-                fx = fx.reshape(-1, n).T
+            if isinstance(fx, dict):
+                fx = _flatten_dict(torch, fx, n, kind, call)
             else:
-                fx = fx.reshape(n, -1)
-            if fx.shape[1] != 1:
-                raise NotImplementedError('multi-component integrands are not supported yet')
-            return fx[:, 0].contiguous()
+                fx = _as_device(torch, fx)
+                if call.layout is None:
+                    shape = tuple(fx.shape[1:]) if kind != 'rbatch' else tuple(fx.shape[:-1])
+                    call.layout = [(None, shape, 0, int(np.prod(shape)) if shape else 1)]
+                fx = fx.reshape(-1, n).T if kind == 'rbatch' else fx.reshape(n, -1)
+            return fx.contiguous()
 
         call.uses_jac = uses_jac
+        call.layout = None
         return call
+
+    def _probe_layout(self, torch, std):
+        """Output structure of a Python integrand: one call on a batch of one
+        point (the centre of the y box) with jac = 1, as the reference's
+        VegasIntegrand evaluates fcn at xsample to learn its shape and size
+        (pyx:3010-3087)."""
+        y = torch.full((1, self.dim), 0.5, dtype=torch.float64, device='cuda')
+        x, _ = self.map.map(y)
+        std(x, torch.ones_like(y) if std.uses_jac else None)
+        return std.layout
 
     def _run_staged(self, torch, std, prec, result):
         L = _lib.lib()
@@ -409,24 +459,61 @@ class Integrator:
         nh = torch.empty(self.nhcube, dtype=torch.int32, device='cuda')
         total = C.c_int64()
         cx, sp = _lib.ctx(), _lib.stream_ptr()
+        m = std.layout[-1][3] if std.layout else 1
+        multi = bool(getattr(std, 'multi', m > 1))
+        corr = 1 if self.correlate_integrals else 0
+        if multi:
+            # per iteration: mean[m] and the covariance lower triangle (or
+            # the variances), double-double pairs
+            npairs = m * (m + 1) // 2 if corr else m
+            acc = torch.zeros(2 * (m + npairs), dtype=torch.float64, device='cuda')
+            cstat = torch.zeros(1, dtype=torch.int32, device='cuda')
         _lib.check(L.vgx_iter_begin(cx, C.byref(cfg), C.byref(st), 0, _lib.ptr(nh), C.byref(total),
                                     sp), 'iter_begin')
         for h0, h1, p0, p1 in _geometry.batches(nh.cpu().numpy(), self.min_neval_batch):
             n = p1 - p0
             x = torch.empty((n, self.dim), dtype=torch.float64, device='cuda')
             y = torch.empty((n, self.dim), dtype=torch.float64, device='cuda') if std.uses_jac else None
+            w = torch.empty(n, dtype=torch.float64, device='cuda') if multi else None
             _lib.check(L.vgx_iter_sample(cx, C.byref(cfg), C.byref(st), 0, h0, h1, p0, p1, _lib.ptr(x),
-                                         _lib.ptr(y), None, None, sp), 'iter_sample')
+                                         _lib.ptr(y), _lib.ptr(w), None, sp), 'iter_sample')
             jac = self.map.jac1d(y) if y is not None else None
             f = std(x, jac)
+            if f.shape[1] != m:
+                raise ValueError('integrand output changed shape (%d values, not %d)' % (f.shape[1], m))
             if bool(torch.isnan(f).any()):
                 raise ValueError('integrand evaluates to nan')
+            # component 0 steers sigf and the training data (pyx:2173, 2185)
+            f0 = f[:, 0].contiguous()
             _lib.check(L.vgx_iter_accumulate(cx, C.byref(cfg), C.byref(st), 0, h0, h1, p0, p1,
-                                             _lib.ptr(f), sp), 'iter_accumulate')
+                                             _lib.ptr(f0), sp), 'iter_accumulate')
+            if multi:
+                _lib.check(L.vgx_iter_accumulate_cov(cx, C.byref(cfg), h0, h1, p0, p1, _lib.ptr(f), _lib.ptr(w),
+                                                     m, corr, _lib.ptr(acc), _lib.ptr(cstat), sp),
+                           'iter_accumulate_cov')
         _lib.check(L.vgx_iter_end(cx, C.byref(cfg), C.byref(st), 0, sp), 'iter_end')
         status, res = self._read_back(bufs)
         self._rng_iter += 1
-        self._finish(status, res, 1, result)
+        if not multi:
+            self._finish(status, res, 1, result)
+            return
+        if int(cstat.item()):
+            _lib.check(int(cstat.item()), 'iter_accumulate_cov')
+        scalar = RAvg(weighted=self.adapt)
+        self._finish(status, res, 1, scalar)
+        a = acc.cpu().numpy()
+        mean = a[0:2 * m:2] + a[1:2 * m:2]
+        cv = a[2 * m::2] + a[2 * m + 1::2]
+        cov = np.zeros((m, m))
+        if corr:
+            il = np.tril_indices(m)
+            cov[il] = cv
+            cov = cov + np.tril(cov, -1).T
+        else:
+            cov[np.diag_indices(m)] = cv
+        result.add(GVarArray(mean, cov, (m,)))
+        result.sum_neval += int(res['itn_neval'][0])
+        self._last_scalar = scalar
 
     # -------------------------------------------------------- sampler APIs
     def _device_batches(self, yield_hcube=False, yield_y=False):
