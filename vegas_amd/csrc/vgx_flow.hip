@@ -682,7 +682,7 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
 // that are issued kWgAhead tiles ahead (a register ring), so several tiles
 // are in flight while the MFMAs run on the current one.
 constexpr int kWgThreads = 512;
-constexpr int kWgChunk = 256;   // sample tiles per workgroup
+constexpr int kWgChunkMax = 256;  // sample tiles per workgroup (at most; fewer for small batches)
 constexpr int kWgAhead = 4;     // tiles in flight per workgroup
 constexpr int kTileChunks = kRows * 16 * 2 / 16;                          // 16-byte pieces per tile
 constexpr int kChunksPerThread = (kTileChunks + kWgThreads - 1) / kWgThreads;
@@ -734,7 +734,8 @@ __device__ inline void wg_mfma(const uint16_t* to, int w, int lane, const bf16x4
     }
 }
 
-__global__ __launch_bounds__(kWgThreads) void flow_wgrad_kernel(const uint16_t* ops, uint64_t ntiles, float* part) {
+__global__ __launch_bounds__(kWgThreads) void flow_wgrad_kernel(const uint16_t* ops, uint64_t ntiles, int chunk,
+                                                                 float* part) {
     __shared__ __align__(16) uint16_t Lt[2][kRows * 16];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
@@ -748,7 +749,7 @@ __global__ __launch_bounds__(kWgThreads) void flow_wgrad_kernel(const uint16_t* 
     bf16x4_t ones;
 #pragma unroll
     for (int j = 0; j < 4; ++j) ones[j] = (lane & 15) == 0 ? (short)0x3F80 : (short)0;
-    const uint64_t t0 = (uint64_t)blockIdx.x * kWgChunk, t1 = min(t0 + kWgChunk, ntiles);
+    const uint64_t t0 = (uint64_t)blockIdx.x * chunk, t1 = min(t0 + (uint64_t)chunk, ntiles);
     WgRegs R[kWgAhead];
 #pragma unroll
     for (int k = 0; k < kWgAhead; ++k) wg_fetch(ops, t0 + k, t1, R[k]);
@@ -1104,7 +1105,12 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
     if ((rc = fl->ops.ensure((size_t)ntiles * kRows * 16 * 2))) return rc;
     if ((rc = fl->cpart.ensure(nc * 8))) return rc;
     if ((rc = fl->misc.ensure(256))) return rc;
-    const int nparts = (int)((ntiles + kWgChunk - 1) / kWgChunk);
+    // tiles per workgroup: two workgroups per CU, between 16 and 256 tiles
+    DeviceInfo dinfo;
+    if ((rc = device_info(&dinfo))) return rc;
+    const int wg_chunk = (int)std::max<uint64_t>(
+        16, std::min<uint64_t>(kWgChunkMax, (ntiles + 2 * (uint64_t)dinfo.cus - 1) / (2 * (uint64_t)dinfo.cus)));
+    const int nparts = (int)((ntiles + wg_chunk - 1) / wg_chunk);
     if ((rc = fl->wpart.ensure((size_t)nparts * kPLayer * 4))) return rc;
     uint32_t* mx = (uint32_t*)fl->misc.ptr;
     hipLaunchKernelGGL(flow_zero_u32_kernel, dim3(1), dim3(64), 0, s, mx, 1);
@@ -1131,7 +1137,7 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
         const int bb = (int)std::min<uint64_t>((uint64_t)info.cus * 2, (ntiles + 7) / 8);
         hipLaunchKernelGGL(flow_backward_kernel, dim3(bb), dim3(kBwdThreads), blds, s, flow_args(fl), b);
         hipLaunchKernelGGL(flow_wgrad_kernel, dim3(nparts), dim3(kWgThreads), 0, s, (const uint16_t*)fl->ops.ptr,
-                           ntiles, (float*)fl->wpart.ptr);
+                           ntiles, wg_chunk, (float*)fl->wpart.ptr);
         hipLaunchKernelGGL(flow_wgrad_reduce_kernel, dim3((kPLayer + 255) / 256), dim3(256), 0, s,
                            (const float*)fl->wpart.ptr, nparts,
                            fl->grad + (size_t)k * kPLayer);
