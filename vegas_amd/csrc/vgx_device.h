@@ -137,21 +137,69 @@ __device__ inline double wave_max(double v) {
     return v;
 }
 
-// Segmented inclusive scan across the 64 lanes.  head = 1 when the lane
-// starts a new segment.  Order of additions is fixed (Hillis-Steele).
-__device__ inline double wave_segscan(double v, int head) {
-    int lane = lane_id();
-    int hflag = head;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        double pv = __shfl_up(v, o, kWave);
-        int ph = __shfl_up(hflag, o, kWave);
-        if (lane >= o && !hflag) {
-            v = pv + v;
-        }
-        if (lane >= o) hflag |= ph;
+// ---- DPP lane movement (gfx9 encodings): another lane's value through the
+// VALU's data-parallel-primitive path instead of an LDS permute.
+constexpr int kDppRowShl = 0x100;     // + n: row_shl:n (within 16-lane rows)
+constexpr int kDppRowShr = 0x110;     // + n: row_shr:n
+constexpr int kDppWaveShl1 = 0x130;   // lane i reads lane i+1
+constexpr int kDppWaveShr1 = 0x138;   // lane i reads lane i-1
+constexpr int kDppRowBcast15 = 0x142; // lane 15 of row r to the lanes of row r+1
+constexpr int kDppRowBcast31 = 0x143; // lane 31 to rows 2 and 3
+// Lanes with no source (or outside ROWS) keep `old`.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ int dpp_i(int old, int x) {
+    return __builtin_amdgcn_update_dpp(old, x, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_d(double old, double x) {
+    const uint64_t o = (uint64_t)__double_as_longlong(old), v = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)dpp_i<CTRL, ROWS>((int)(uint32_t)o, (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)dpp_i<CTRL, ROWS>((int)(uint32_t)(o >> 32), (int)(uint32_t)(v >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// One step of a segmented inclusive scan (v += v of the source lane unless a
+// segment starts at or before this lane since the source), for 1 or 2 values.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void segscan_step(double& v, int& f) {
+    const double pv = dpp_d<CTRL, ROWS>(0.0, v);
+    const int pf = dpp_i<CTRL, ROWS>(0, f);
+    if (!f) v = pv + v;
+    f |= pf;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void segscan_step2(double& a, double& b, int& f) {
+    const double pa = dpp_d<CTRL, ROWS>(0.0, a), pb = dpp_d<CTRL, ROWS>(0.0, b);
+    const int pf = dpp_i<CTRL, ROWS>(0, f);
+    if (!f) {
+        a = pa + a;
+        b = pb + b;
     }
+    f |= pf;
+}
+
+// Segmented inclusive scan across the 64 lanes.  head = 1 when the lane
+// starts a new segment.  Fixed order of additions: within each 16-lane row
+// by shifts of 1, 2, 4, 8, then the carry of row 0 into row 1 and of row 2
+// into row 3, then that of rows 0-1 into rows 2-3.
+__device__ inline double wave_segscan(double v, int head) {
+    int f = head;
+    segscan_step<kDppRowShr + 1, 0xF>(v, f);
+    segscan_step<kDppRowShr + 2, 0xF>(v, f);
+    segscan_step<kDppRowShr + 4, 0xF>(v, f);
+    segscan_step<kDppRowShr + 8, 0xF>(v, f);
+    segscan_step<kDppRowBcast15, 0xA>(v, f);
+    segscan_step<kDppRowBcast31, 0xC>(v, f);
     return v;
+}
+__device__ inline void wave_segscan2(double& a, double& b, int head) {
+    int f = head;
+    segscan_step2<kDppRowShr + 1, 0xF>(a, b, f);
+    segscan_step2<kDppRowShr + 2, 0xF>(a, b, f);
+    segscan_step2<kDppRowShr + 4, 0xF>(a, b, f);
+    segscan_step2<kDppRowShr + 8, 0xF>(a, b, f);
+    segscan_step2<kDppRowBcast15, 0xA>(a, b, f);
+    segscan_step2<kDppRowBcast31, 0xC>(a, b, f);
 }
 
 }  // namespace vgx

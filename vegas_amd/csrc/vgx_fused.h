@@ -253,22 +253,10 @@ __device__ __forceinline__ void seg_prefix2(const int (&key)[kK], double (&a)[kK
             a[j] += a[j - 1];
             b[j] += b[j - 1];
         }
-    int prev_last = __shfl_up(key[kK - 1], 1, kWave);
-    if (lane == 0) prev_last = -1;
-    int hflag = key[kK - 1] != prev_last;
+    const int prev_last = dpp_i<kDppWaveShr1>(-1, key[kK - 1]);  // lane 0: -1
     double va = a[kK - 1], vb = b[kK - 1];
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const double pa = __shfl_up(va, o, kWave);
-        const double pb = __shfl_up(vb, o, kWave);
-        const int ph = __shfl_up(hflag, o, kWave);
-        if (lane >= o && !hflag) {
-            va = pa + va;
-            vb = pb + vb;
-        }
-        if (lane >= o) hflag |= ph;
-    }
-    const double ca = __shfl_up(va, 1, kWave), cb = __shfl_up(vb, 1, kWave);
+    wave_segscan2(va, vb, key[kK - 1] != prev_last);
+    const double ca = dpp_d<kDppWaveShr1>(0.0, va), cb = dpp_d<kDppWaveShr1>(0.0, vb);
     const bool carry = lane > 0 && key[0] == prev_last;
 #pragma unroll
     for (int j = 0; j < kK; ++j)
@@ -285,11 +273,10 @@ __device__ __forceinline__ void seg_prefix(const int (&key)[kK], const double (&
     P[0] = v[0];
 #pragma unroll
     for (int j = 1; j < kK; ++j) P[j] = (key[j] == key[j - 1]) ? P[j - 1] + v[j] : v[j];
-    int prev_last = __shfl_up(key[kK - 1], 1, kWave);
-    if (lane == 0) prev_last = -1;
+    const int prev_last = dpp_i<kDppWaveShr1>(-1, key[kK - 1]);  // lane 0: -1
     const int head = key[kK - 1] != prev_last;
     const double incl = wave_segscan(P[kK - 1], head);
-    double prev_incl = __shfl_up(incl, 1, kWave);
+    const double prev_incl = dpp_d<kDppWaveShr1>(0.0, incl);
     const double cin = (lane > 0 && key[0] == prev_last) ? prev_incl : 0.0;
 #pragma unroll
     for (int j = 0; j < kK; ++j)
@@ -461,7 +448,7 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
     double P[kK];
     seg_prefix(key, wf, P, lane);
     int endf[kK];
-    const int next_first = __shfl_down(key[0], 1, kWave);
+    const int next_first = dpp_i<kDppWaveShl1>(kKeySentinel, key[0]);  // lane 63: unused
 #pragma unroll
     for (int j = 0; j < kK - 1; ++j) endf[j] = key[j + 1] != key[j];
     endf[kK - 1] = (lane == kWave - 1) || (next_first != key[kK - 1]);
@@ -475,16 +462,28 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
         if (key[j] == key[0]) fv = P[j];
     double rv = fv;
     int rf = (key[0] != key[kK - 1]) || endf[kK - 1];
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const double nv = __shfl_down(rv, o, kWave);
-        const int nf = __shfl_down(rf, o, kWave);
-        if (!rf && lane + o < kWave) {
+    // Within each 16-lane row by DPP (a lane with no source keeps its own
+    // value, which leaves it unchanged): lane i then holds the first flagged
+    // lane of [i, end of its row].  Across rows by permutes from the first
+    // lane of the row 1, then 2 rows ahead, which extends that to [i, 63].
+    auto back_step = [&](double nv, int nf) {
+        if (!rf) {
             rv = nv;
             rf = nf;
         }
+    };
+    back_step(dpp_d<kDppRowShl + 1>(rv, rv), dpp_i<kDppRowShl + 1>(rf, rf));
+    back_step(dpp_d<kDppRowShl + 2>(rv, rv), dpp_i<kDppRowShl + 2>(rf, rf));
+    back_step(dpp_d<kDppRowShl + 4>(rv, rv), dpp_i<kDppRowShl + 4>(rf, rf));
+    back_step(dpp_d<kDppRowShl + 8>(rv, rv), dpp_i<kDppRowShl + 8>(rf, rf));
+#pragma unroll
+    for (int hop = 1; hop <= 2; hop <<= 1) {
+        const int src = ((lane >> 4) + hop) << 4;
+        const double nv = __shfl(rv, src & (kWave - 1), kWave);
+        const int nf = __shfl(rf, src & (kWave - 1), kWave);
+        if (src < kWave) back_step(nv, nf);
     }
-    const double next_rv = __shfl_down(rv, 1, kWave);
+    const double next_rv = dpp_d<kDppWaveShl1>(0.0, rv);
     const double vlast = endf[kK - 1] ? P[kK - 1] : next_rv;
     double S[kK];
     {
