@@ -39,7 +39,7 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 512 MiB device to device 4 times (HBM counter calibration)")
     p.add_argument("--workload", default="double_peak_8d",
-                   choices=["double_peak_8d", "ridge_8d", "batched_4096", "flow_ridge_9d"],
+                   choices=["double_peak_8d", "ridge_8d", "batched_4096", "flow_ridge_9d", "moments_8d"],
                    help="double_peak_8d (default): the headline; ridge_8d: 1000-Gaussian diagonal "
                         "ridge, neval 2e7 (diagonal-ridge and generic mixture functors); batched_4096: "
                         "4096 independent Gaussian integrals in one graph, nitn 8 per call (--steps "
@@ -178,6 +178,49 @@ def ridge(a):
     print(json.dumps(out))
 
 
+def moments(a):
+    """Multi-component integrand through the staged path: the 8-D double peak
+    g and its moments x0 g, x0^2 g, x0 x1 g (4 components, full covariance)
+    evaluated by a torch integrand on the device, neval 1e7 per iteration.
+    value = evaluations/s of the whole staged iteration (device sampling,
+    the torch integrand, the scalar and covariance accumulations, adapt)."""
+    import torch
+    import vegas_amd as va
+    from vegas_amd import workloads
+    wl = workloads.double_peak_8d()
+    f = wl["f"]
+    A = torch.as_tensor(f.A, device="cuda")
+    B = torch.as_tensor(f.B, device="cuda")
+    c = torch.as_tensor(f.centers, device="cuda")
+
+    @va.gpu_lbatchintegrand
+    def g(x):
+        r2 = ((x[:, None, :] - c[None, :, :]) ** 2).sum(-1)
+        v = (A[None, :] * torch.exp(-B[None, :] * r2)).sum(-1)
+        return torch.stack([v, x[:, 0] * v, x[:, 0] ** 2 * v, x[:, 0] * x[:, 1] * v], dim=1)
+
+    integ = va.Integrator([[0, 1]] * 8, neval=wl["neval"], seed=2026, min_neval_batch=2000000)
+    if a.warmup > 0:
+        integ(g, nitn=a.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = integ(g, nitn=a.steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {
+        "metric": METRIC, "value": float(r.sum_neval) / dt, "unit": "evals/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": dt * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (device Philox samples)",
+        "config": {"workload": "8-D double peak and 3 moments (4-component integrand, correlated), torch "
+                               "integrand on the device, staged batches of >= 2e6, neval 1e7/iteration",
+                   "parallelism": "single GPU"},
+        "result": {"mean": [float(v) for v in r.mean], "sdev": [float(v) for v in r.sdev],
+                   "chi2_dof": r.chi2 / max(r.dof, 1), "Q": r.Q},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def flow_ridge(a):
     """GPU wall-time to a relative error rtol on the 9-D ridge (500 diagonal
     Gaussians, sigma 0.01): the learned map (vegas warm-up, 2e7 vegas
@@ -253,6 +296,9 @@ def main():
         return
     if a.workload == "ridge_8d":
         ridge(a)
+        return
+    if a.workload == "moments_8d":
+        moments(a)
         return
     import torch
 

@@ -123,3 +123,25 @@ def test_ravg_array_correlated_weighted_closed_form():
     np.testing.assert_allclose(r.cov, C / 2, rtol=1e-10)
     d = np.array([0.2, -0.3])
     assert abs(r.chi2 - 0.5 * d @ np.linalg.solve(C, d)) <= 1e-10
+
+
+def test_ravg_array_pickle_and_summary():
+    import pickle
+    from vegas_amd._ravg import RAvgArray, RAvgDict
+    rng = np.random.default_rng(6)
+    r = RAvgArray((2,), weighted=True)
+    for _ in range(3):
+        m = rng.normal(size=2) + 3
+        r.add((m, np.array([[0.04, 0.01], [0.01, 0.09]])))
+    r.sum_neval = 300
+    r2 = pickle.loads(pickle.dumps(r))
+    np.testing.assert_allclose(r2.mean, r.mean, rtol=0, atol=0)
+    np.testing.assert_allclose(r2.cov, r.cov, rtol=0, atol=0)
+    assert r2.sum_neval == 300 and r2.nitn == 3
+    text = r.summary(extended=True)
+    assert text.splitlines()[0].split()[:2] == ['itn', 'integral'] and len(text.splitlines()) == 2 + 3 + 1 + 2
+    d = RAvgDict([('a', (), 0, 1), ('b', (2,), 1, 3)])
+    d.add((np.array([1.0, 2.0, 3.0]), np.diag([0.1, 0.2, 0.3])))
+    d2 = pickle.loads(pickle.dumps(d))
+    assert list(d2.keys()) == ['a', 'b'] and d2['b'].shape == (2,)
+    assert abs(d2['a'].mean - 1.0) < 1e-15

@@ -418,6 +418,50 @@ class RAvgArray(GVarArray):
         """pyx:2723-2726"""
         return bool(np.all(self.sdev < atol + rtol * np.abs(self.mean)))
 
+    def summary(self, extended=False, weighted=None, rescale=None):
+        """Iteration-by-iteration table of the first component and the
+        chi2 of the whole array (pyx:2833-2875); with ``extended`` a table of
+        every component's average follows."""
+        if weighted is None:
+            weighted = self.weighted
+        if rescale is None:
+            rescale = self.rescale
+        acc = RAvgArray(self.shape, weighted=weighted, rescale=rescale)
+        linedata = []
+        for i, res in enumerate(self.itn_results):
+            acc.add(res)
+            itn = '%3d' % (i + 1)
+            integral = '%-15s' % res.flat_gvars()[0]
+            wgtavg = '%-15s' % acc.flat_gvars()[0]
+            chi2dof = '%8.2f' % (acc.chi2 / acc.dof if i != 0 else 0.0)
+            Q = '%8.2f' % (acc.Q if i != 0 else 1.0)
+            linedata.append((itn, integral, wgtavg, chi2dof, Q))
+        nchar = 5 * [0]
+        for data in linedata:
+            for i, d in enumerate(data):
+                nchar[i] = max(nchar[i], len(d))
+        fmt = '%%%ds   %%-%ds %%-%ds %%%ds %%%ds\n' % tuple(nchar)
+        ans = fmt % ('itn', 'integral', 'wgt average' if weighted else 'average', 'chi2/dof', 'Q')
+        ans += len(ans[:-1]) * '-' + '\n'
+        for data in linedata:
+            ans += fmt % data
+        if extended and self.size > 1:
+            ans += '\n' + '\n'.join('%8s  %s' % (str(idx), g)
+                                    for idx, g in zip(np.ndindex(self.shape), self.flat_gvars())) + '\n'
+        return ans
+
+    def __reduce__(self):
+        return (_rebuild_ravg_array, (self.shape, self.weighted, self.rescale,
+                                      [(r._m, r._cov) for r in self.itn_results], self.sum_neval))
+
+
+def _rebuild_ravg_array(shape, weighted, rescale, results, sum_neval):
+    r = RAvgArray(shape, weighted=weighted, rescale=rescale)
+    for m, c in results:
+        r.add(GVarArray(m, c, shape))
+    r.sum_neval = sum_neval
+    return r
+
 
 class RAvgDict(dict):
     """Running average of dictionary-valued estimates: one RAvgArray over the
@@ -460,3 +504,16 @@ class RAvgDict(dict):
 
     def converged(self, rtol, atol):
         return self.rarray.converged(rtol, atol)
+
+    def summary(self, extended=False, weighted=None):
+        return self.rarray.summary(extended=extended, weighted=weighted)
+
+    def __reduce__(self):
+        return (_rebuild_ravg_dict, (self.layout, self.rarray))
+
+
+def _rebuild_ravg_dict(layout, rarray):
+    r = RAvgDict(layout, weighted=rarray.weighted)
+    r.rarray = rarray
+    r._refresh()
+    return r
