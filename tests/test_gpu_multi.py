@@ -33,8 +33,8 @@ def _moments(va):
     return f
 
 
-def _first_iteration_points(va, dim, neval, seed):
-    twin = va.Integrator([[0, 1]] * dim, neval=neval, seed=seed)
+def _first_iteration_points(va, dim, neval, seed, **kw):
+    twin = va.Integrator([[0, 1]] * dim, neval=neval, seed=seed, **kw)
     xs, ws, hs = [], [], []
     for x, w, h in twin._device_batches(yield_hcube=True):
         xs.append(x.cpu().numpy())
@@ -59,6 +59,37 @@ def test_device_sums_match_oracle(va, correlate):
     else:
         np.testing.assert_allclose(np.diag(g.cov), cov, rtol=1e-10, atol=1e-300)
         assert np.count_nonzero(g.cov - np.diag(np.diag(g.cov))) == 0
+
+
+@pytest.mark.parametrize("m", [4, 6])
+@pytest.mark.parametrize("correlate", [True, False])
+def test_device_sums_mixed_hcube_sizes(va, m, correlate):
+    """Hcubes from 2 to several hundred points in one batch (a lognormal
+    sigf steers the allocation): short hcubes summed per thread, long ones
+    by the whole workgroup; m = 4 takes the all-pairs kernel, m = 6 the
+    per-pair kernel."""
+    dim, neval, seed = 3, 40000, 29
+    rng = np.random.default_rng(5)
+    kw = dict(nstrat=[8, 8, 4], neval_frac=0.95, sigf=np.exp(2.5 * rng.standard_normal(256)))
+
+    @va.lbatchintegrand
+    def f(x):
+        g = _gauss(x)
+        cols = [g, x[:, 0] * g, np.ones(len(x)), x[:, 1] * x[:, 1] * g, x[:, 2] * g, 1.0 + x[:, 0]]
+        return np.stack(cols[:m], axis=1)
+
+    integ = va.Integrator([[0, 1]] * dim, neval=neval, seed=seed, correlate_integrals=correlate, **kw)
+    r = integ(f, nitn=1, adapt=False)
+    x, w, h = _first_iteration_points(va, dim, neval, seed, **kw)
+    n_h = np.bincount(h, minlength=256)
+    assert n_h.min() <= 32 < n_h.max()  # both paths taken
+    mean, cov = mo.iteration_sums(h, w, f(x), correlate=correlate)
+    g = r.itn_results[0]
+    np.testing.assert_allclose(g.mean, mean, rtol=1e-12, atol=1e-15)
+    if correlate:
+        np.testing.assert_allclose(g.cov, cov, rtol=1e-10, atol=1e-300)
+    else:
+        np.testing.assert_allclose(np.diag(g.cov), cov, rtol=1e-10, atol=1e-300)
 
 
 def test_component_zero_steers_sampling(va):
