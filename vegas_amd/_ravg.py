@@ -142,19 +142,29 @@ class RAvg(GVar):
     def add(self, g):
         """Add estimate ``g`` (pyx:2392-2410).  The variance re-read from g is
         sdev**2, as the reference re-reads it from its gvar."""
-        if not isinstance(g, GVar):
-            g = GVar(*g)
-        self.itn_results.append(g)
-        self._mlist.append(g.mean)
+        self.add_all([g])
+
+    def add_all(self, gs):
+        """Add estimates in order; the same result as add() on each in turn
+        (the weighted sums are formed once, over the whole list)."""
+        for g in gs:
+            if not isinstance(g, GVar):
+                g = GVar(*g)
+            self.itn_results.append(g)
+            self._mlist.append(g.mean)
+            if self.weighted:
+                self._wlist.append(1 / (g.var if g.var > TINY else TINY))
+            else:
+                self._msum += g.mean
+                self._varsum += g.var
+                self._n += 1
+        if not self.itn_results:
+            return
         if self.weighted:
-            self._wlist.append(1 / (g.var if g.var > TINY else TINY))
             var = 1. / np.sum(self._wlist)
             sdev = np.sqrt(var)
             mean = np.sum([w * m for w, m in zip(self._wlist, self._mlist)]) * var
         else:
-            self._msum += g.mean
-            self._varsum += g.var
-            self._n += 1
             mean = self._msum / self._n
             sdev = np.sqrt(self._varsum / self._n ** 2)
         self._mean = float(mean)
@@ -272,6 +282,13 @@ class GVarArray:
     def flat_gvars(self):
         sd = np.sqrt(np.maximum(np.diag(self._cov), 0.0))
         return [GVar(m, s) for m, s in zip(self._m, sd)]
+
+    @property
+    def flat(self):
+        """The estimates in C order, as a 1-d object array of GVars."""
+        g = np.empty(self.size, dtype=object)
+        g[:] = self.flat_gvars()
+        return g
 
     def __getitem__(self, idx):
         g = np.empty(self.size, dtype=object)
@@ -484,6 +501,16 @@ class RAvgDict(dict):
     def add(self, g):
         self.rarray.add(g)
         self._refresh()
+
+    def extend(self, ravg):
+        """Merge results from ``ravg`` after those in ``self`` (pyx:2314-2318)."""
+        self.rarray.extend(ravg.rarray)
+        self._refresh()
+
+    @property
+    def flat(self):
+        """Every key's estimates, keys in order, as a 1-d object array."""
+        return self.rarray.flat
 
     @property
     def itn_results(self):

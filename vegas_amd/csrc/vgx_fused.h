@@ -701,7 +701,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
 
 template <int DIM, class Fcn>
 int launch_fused_dim(const IterGeom& g, const FusedPtrs& io, const Fcn& fcn, int blocks, size_t lds,
-                     hipStream_t st) {
+                     hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     auto kern = fused_kernel<DIM, false, Fcn>;
     if constexpr (Fcn::kRng32) {
         if (g.per_all) kern = fused_kernel<DIM, true, Fcn>;
@@ -720,7 +720,10 @@ int launch_fused_dim(const IterGeom& g, const FusedPtrs& io, const Fcn& fcn, int
         }
         lds_set = lds;
     }
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kFusedThreads), lds, st, g, io, fcn);
+    if (ev0)
+        hipExtLaunchKernelGGL(kern, dim3(blocks), dim3(kFusedThreads), (uint32_t)lds, st, ev0, ev1, 0, g, io, fcn);
+    else
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kFusedThreads), lds, st, g, io, fcn);
     return check_launch();
 }
 

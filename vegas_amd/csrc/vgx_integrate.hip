@@ -728,13 +728,11 @@ template <class Fcn>
 static int accumulate_kernels(vgx_ctx* ctx, Run* R, const Fcn& fcn, hipStream_t s, int timed, int reduce) {
     const IterGeom& g = R->g;
     const int rec = timed && ctx->timing_on && ctx->ntimed < vgx_ctx::kMaxTimed;
-    if (rec) VGX_HIP(hipEventRecord(ctx->ev[2 * ctx->ntimed], s));
-    int rc = launch_fused(g, R->io, fcn, R->fused_blocks, R->lds, s);
+    int rc = rec ? launch_fused(g, R->io, fcn, R->fused_blocks, R->lds, s, ctx->ev[2 * ctx->ntimed],
+                                ctx->ev[2 * ctx->ntimed + 1])
+                 : launch_fused(g, R->io, fcn, R->fused_blocks, R->lds, s);
     if (rc) return rc;
-    if (rec) {
-        VGX_HIP(hipEventRecord(ctx->ev[2 * ctx->ntimed + 1], s));
-        ctx->ntimed++;
-    }
+    if (rec) ctx->ntimed++;
     hipLaunchKernelGGL(strad_kernel, dim3(R->strad_blocks), dim3(kStradThreads), 0, s, g, R->sa);
     if (reduce)
         hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(kReduceThreads), 0, s,

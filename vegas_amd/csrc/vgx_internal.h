@@ -1,5 +1,6 @@
 // vgx_internal.h -- host-side plumbing shared by the libvgx translation units.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>

@@ -251,8 +251,10 @@ int adapt_in_place(vgx_ctx* ctx, int dim, const int32_t* ninc, double* grid, int
                    hipStream_t st);
 
 // Launchers, one per (integrand, dim) instantiation (vgx_fused_*.hip).
+// ev0/ev1 (may be null): events that take the launch's own start and end
+// timestamps (hipExtLaunchKernel), so timing adds no packets to the stream.
 template <class Fcn>
 int launch_fused(const IterGeom& g, const FusedPtrs& io, const Fcn& fcn, int grid_blocks,
-                 size_t lds, hipStream_t st);
+                 size_t lds, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 }  // namespace vgx

@@ -136,13 +136,26 @@ class Integrator:
         odict = {k: getattr(self, k) for k in Integrator.defaults if k not in ('map',)}
         odict['nstrat'] = np.asarray(self.nstrat)
         odict['sigf'] = self.sigf
+        # beyond the reference's state: the random stream position and the
+        # device's own sum of sigf, so that a restored integrator continues
+        # with exactly the points and allocation of the original
+        # (the constructor's default settings re-grid the map in passing; the
+        # nodes are put back as they were)
         odict['_rng_iter'] = self._rng_iter
+        odict['_sum_sigf'] = self.sum_sigf
+        odict['_grid'] = self.map.extract_grid()
         return (Integrator, (self.map,), odict)
 
     def __setstate__(self, odict):
         rng_iter = odict.pop('_rng_iter', 0)
+        sum_sigf = odict.pop('_sum_sigf', None)
+        grid = odict.pop('_grid', None)
         self.set(odict)
         self._rng_iter = rng_iter
+        if sum_sigf is not None and self._sigf is not None:
+            self.sum_sigf = sum_sigf
+        if grid is not None and [len(g) - 1 for g in grid] == [int(n) for n in self.map.ninc]:
+            self.map = AdaptiveMap(grid)
 
     def set(self, ka={}, **kargs):
         """Reset parameters; returns the old values (pyx:1256-1446)."""
@@ -298,34 +311,50 @@ class Integrator:
         return c
 
     def _state(self, torch, nitn):
+        """Device state of one call: the training histogram (initialised by
+        the library at the start of a run) and one small float64 block,
+        filled from the host in one copy and read back in one copy, that
+        holds sum_sigf, the RAvg summary, the status word and the
+        per-iteration results (int64 counts as views of the same block)."""
         dev = 'cuda'
         f64 = torch.float64
         if self._sigf is None:
             self._sigf = torch.ones(max(self.nhcube, 1), dtype=f64, device=dev)
-        nb = (self.dim, self.map.ld - 1)
+        nb = self.dim * (self.map.ld - 1)
+        hist = torch.empty(2 * nb, dtype=f64, device=dev)
+        h = np.zeros(6 + 5 * nitn)
+        h[0] = self.sum_sigf
+        head = torch.from_numpy(h).to(dev)
+        o = 6 + 2 * nitn
         bufs = dict(
-            sum_sigf=torch.tensor([self.sum_sigf], dtype=f64, device=dev),
-            sum_f=torch.zeros(nb, dtype=f64, device=dev),
-            n_f=torch.zeros(nb, dtype=f64, device=dev),
-            itn_mean=torch.zeros(nitn, dtype=f64, device=dev),
-            itn_var=torch.zeros(nitn, dtype=f64, device=dev),
-            ravg=torch.zeros(4, dtype=f64, device=dev),
-            itn_neval=torch.zeros(nitn, dtype=torch.int64, device=dev),
-            itn_nhrange=torch.zeros(2 * nitn, dtype=torch.int64, device=dev),
-            status=torch.zeros(1, dtype=torch.int32, device=dev),
+            head=head,
+            hist=hist,
+            sum_sigf=head[0:1],
+            ravg=head[1:5],
+            status=head[5:6].view(torch.int32),
+            itn_mean=head[6:6 + nitn],
+            itn_var=head[6 + nitn:o],
+            itn_neval=head[o:o + nitn].view(torch.int64),
+            itn_nhrange=head[o + nitn:o + 3 * nitn].view(torch.int64),
+            sum_f=hist[:nb],
+            n_f=hist[nb:],
         )
         st = _lib.VgxState()
         st.grid = self.map._grid.data_ptr()
         st.sigf = self._sigf.data_ptr()
-        for k, v in bufs.items():
-            setattr(st, k, v.data_ptr())
+        for k in ('sum_sigf', 'ravg', 'status', 'itn_mean', 'itn_var', 'itn_neval', 'itn_nhrange',
+                  'sum_f', 'n_f'):
+            setattr(st, k, bufs[k].data_ptr())
         return st, bufs
 
     def _read_back(self, bufs):
         # one synchronising copy of everything the host needs
-        status = int(bufs['status'].item())
-        res = {k: bufs[k].cpu().numpy() for k in ('itn_mean', 'itn_var', 'itn_neval',
-                                                   'itn_nhrange', 'ravg', 'sum_sigf')}
+        h = bufs['head'].cpu().numpy()
+        nitn = (len(h) - 6) // 5
+        o = 6 + 2 * nitn
+        status = int(h[5:6].view(np.int32)[0])
+        res = dict(sum_sigf=h[0:1], ravg=h[1:5], itn_mean=h[6:6 + nitn], itn_var=h[6 + nitn:o],
+                   itn_neval=h[o:o + nitn].view(np.int64), itn_nhrange=h[o + nitn:o + 3 * nitn].view(np.int64))
         return status, res
 
     # ---------------------------------------------------------- integration
@@ -379,10 +408,9 @@ class Integrator:
             raise ValueError('integrand evaluates to nan')
         if status:
             _lib.check(status, 'integrate')
-        for k in range(n):
-            result.add(GVar(res['itn_mean'][k], math.sqrt(max(res['itn_var'][k], 0.0))
-                            if res['itn_var'][k] >= 0 else float('nan')))
-            result.sum_neval += int(res['itn_neval'][k])
+        result.add_all([GVar(float(res['itn_mean'][k]), math.sqrt(max(float(res['itn_var'][k]), 0.0))
+                             if res['itn_var'][k] >= 0 else float('nan')) for k in range(n)])
+        result.sum_neval += int(np.sum(res['itn_neval'][:n]))
         self.last_neval = int(res['itn_neval'][n - 1])
         self.neval_hcube_range = np.array(res['itn_nhrange'][2 * (n - 1):2 * n])
         if self.beta > 0 and not self.adapt_to_errors and self.adapt:
