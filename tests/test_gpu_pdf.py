@@ -1,0 +1,126 @@
+"""PDFIntegrator and restratify on the GPU (src/vegas/__init__.py:373-1188,
+1314-1590).  The reference's own tests need gvar (absent here), so these
+pin the results by closed forms: the moments of a Gaussian and of a
+two-bump mixture (the reference's docstring example, __init__.py:1055-1088,
+which prints <p> = 0.40(17)), the PDF's normalisation, weighted samples,
+and the strata that restratify moves onto the one axis an integrand varies
+along."""
+import math
+import pickle
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def va():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import vegas_amd
+    return vegas_amd
+
+
+MEAN = np.array([0.5, -1.0])
+COV = np.array([[0.25, 0.1], [0.1, 0.16]])
+
+
+def _bimodal(va):
+    @va.rbatchintegrand
+    def g_pdf(p):
+        ans = 0
+        h = 1.
+        for p0 in [0.3, 0.6]:
+            ans += h * np.exp(-np.sum((p - p0) ** 2, axis=0) / 2 / .01)
+            h /= 2
+        return ans
+    return g_pdf
+
+
+def test_gaussian_moments(va):
+    """pdf=None: the PDF integrates to 1 and E[p], Cov[p] are the
+    parameters' own."""
+    itg = va.PDFIntegrator((MEAN, COV), neval=20000, nitn=8, seed=3)
+    itg(neval=20000, nitn=5)  # adapt
+    norm = itg()
+    assert abs(norm.pdfnorm.mean - 1.0) < 5 * norm.pdfnorm.sdev + 1e-12
+    ev = itg(lambda p: p)
+    assert ev.shape == (2,)
+    pull = (ev.mean - MEAN) / ev.sdev
+    assert np.all(np.abs(pull) < 5), pull
+    st = itg.stats()
+    np.testing.assert_allclose(st.mean, MEAN, atol=0.01)
+    np.testing.assert_allclose(st.cov, COV, rtol=0.03, atol=0.003)
+
+
+def test_bimodal_expectations_and_stats(va):
+    """E[p] = (1 * 0.3 + 0.5 * 0.6) / 1.5 = 0.4 per axis; variance
+    0.01 + 0.02 = 0.03 (sdev 0.173, the reference's 0.40(17)), covariance
+    between the axes 0.02."""
+    itg = va.PDFIntegrator(([0.5, 0.5], [[.25, .2], [.2, .25]]), pdf=_bimodal(va), seed=5)
+    itg(neval=20000, nitn=10)
+    ev = itg(lambda p: p, neval=20000, nitn=10)
+    assert np.all(np.abs(ev.mean - 0.4) < 5 * ev.sdev + 1e-4), (ev.mean, ev.sdev)
+    st = itg.stats(neval=20000, nitn=10)
+    np.testing.assert_allclose(st.mean, [0.4, 0.4], atol=0.01)
+    np.testing.assert_allclose(st.cov, [[0.03, 0.02], [0.02, 0.03]], atol=0.002)
+    assert ev.Q > 1e-4  # forwarded to the running average
+
+
+def test_dict_valued_f_and_adapt_to_f(va):
+    itg = va.PDFIntegrator((MEAN, COV), neval=20000, seed=9)
+    itg(nitn=5)
+
+    @va.lbatchintegrand
+    def f(p):
+        return dict(a=p[:, 0], b=p ** 2)
+
+    ev = itg(f, nitn=6)
+    assert set(ev.keys()) == {'a', 'b'}
+    assert abs(ev['a'].mean - 0.5) < 5 * ev['a'].sdev
+    eb = np.diag(COV) + MEAN ** 2
+    assert np.all(np.abs(ev['b'].mean - eb) < 5 * ev['b'].sdev)
+    assert ev.cov.shape == (3, 3)
+    ev2 = itg(f, nitn=6, adapt_to_pdf=False)
+    assert abs(ev2['a'].mean - 0.5) < 5 * ev2['a'].sdev
+
+
+def test_sample(va):
+    itg = va.PDFIntegrator(([0.5, 0.5], [[.25, .2], [.2, .25]]), pdf=_bimodal(va), seed=7)
+    itg(neval=4000, nitn=10)
+    wgts, p = itg.sample(nbatch=40000)
+    assert p.shape[0] == 2 and p.shape[1] == len(wgts) >= 40000
+    assert abs(np.sum(wgts) - 1) < 1e-12
+    np.testing.assert_allclose(np.sum(wgts * p, axis=1), [0.4, 0.4], atol=0.01)
+    w2, p2 = itg.sample(nbatch=4000, mode='lbatch')
+    assert p2.shape == (len(w2), 2)
+
+
+def test_pickle(va):
+    itg = va.PDFIntegrator((MEAN, COV), neval=5000, seed=4)
+    itg(nitn=3)
+    it2 = pickle.loads(pickle.dumps(itg))
+    assert it2.map.extract_grid() == itg.map.extract_grid()
+    np.testing.assert_array_equal(it2.param_pdf.L, itg.param_pdf.L)
+    r1 = itg(lambda p: p[0], nitn=1)
+    r2 = it2(lambda p: p[0], nitn=1)
+    assert r1.mean == r2.mean and r1.sdev == r2.sdev
+
+
+def test_restratify_moves_strata_to_the_varying_axis(va):
+    """An integrand that varies only along x0: restratify gives axis 0 the
+    strata; the restratified integrator still integrates it correctly."""
+    @va.lbatchintegrand
+    def f(x):
+        return np.exp(-100. * (x[:, 0] - 0.5) ** 2) * math.sqrt(100. / math.pi) / math.erf(5.)
+
+    itg = va.Integrator(4 * [[0, 1]], neval=40000, seed=21)
+    itg(f, nitn=5)
+    old = np.array(itg.nstrat)
+    new = va.restratify(itg, f, below_avg_nstrat=1)
+    assert new.restratify_weight[0] > 100 * max(new.restratify_weight[1:])
+    assert new.nstrat[0] > old[0] and np.all(new.nstrat[1:] == 1)
+    r = new(f, nitn=5)
+    assert abs(r.mean - 1.0) < 5 * r.sdev

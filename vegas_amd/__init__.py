@@ -4,7 +4,7 @@ The call surface is gplepage/vegas' (``Integrator``, ``AdaptiveMap``,
 ``RAvg``, ``lbatchintegrand``/``rbatchintegrand``); the hot path runs as
 hand-written HIP kernels for gfx950 in ``libvgx.so`` (C-ABI in include/vgx.h).
 """
-from ._ravg import GVar, RAvg, gammaQ
+from ._ravg import GVar, GVarArray, RAvg, RAvgArray, RAvgDict, gammaQ
 from .batchintegrand import (BatchIntegrand, LBatchIntegrand, MPIintegrand, RBatchIntegrand,
                              VecIntegrand, batchintegrand, gpu_lbatchintegrand,
                              gpu_rbatchintegrand, lbatchintegrand, rbatchintegrand, vecintegrand)
@@ -21,4 +21,7 @@ def __getattr__(name):
     if name in ('Integrator', 'reporter'):
         from . import integrator
         return getattr(integrator, name)
+    if name in ('PDFIntegrator', 'PDFEV', 'PDFEVArray', 'PDFEVDict', 'restratify'):
+        from . import pdf
+        return getattr(pdf, name)
     raise AttributeError(name)

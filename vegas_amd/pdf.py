@@ -1,0 +1,424 @@
+"""Expectation values over a probability density, and y-space
+restratification, on top of the device :class:`Integrator`.
+
+``PDFIntegrator`` follows src/vegas/__init__.py:373-1188: the parameters p of
+a Gaussian (mean, covariance) are written p = mean + L chiv with L from the
+covariance's eigen-decomposition, each chiv axis is mapped to
+theta = arctan(chiv / scale) in [-arctan(limit / scale), arctan(limit / scale)]
+with a vegas grid pre-trained on the Gaussian (``_make_map``), and vegas
+integrates pdf(p) and f(p) pdf(p) over theta together; E[f] is their ratio,
+with the error propagated from the two integrals' covariance.  The reference
+takes gvar objects for the parameters; gvar is not a dependency here, so the
+parameters are given as ``(mean, cov)`` (or anything with ``.mean`` and
+``.cov``, such as an ``RAvgArray`` result), and results are plain
+``GVar``/``GVarArray`` objects carrying the covariance.
+
+``restratify`` follows __init__.py:1314-1590 for plain integrators: it
+integrates f and its distribution over ndy slices of every y axis, weights
+each axis by the variance of dI/dy there, and re-divides the hypercubes
+among the axes by those weights.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ._ravg import GVar, GVarArray
+from .integrator import Integrator
+from .adaptive_map import AdaptiveMap
+from .batchintegrand import lbatchintegrand
+
+__all__ = ['PDFIntegrator', 'PDFEV', 'PDFEVArray', 'PDFEVDict', 'restratify']
+
+
+class _GaussPDF:
+    """p = mean + L chiv for a Gaussian with covariance cov (the role of
+    gvar.PDF in the reference): L = D V sqrt(lambda), with D the standard
+    deviations and (lambda, V) the eigen-decomposition of the correlation
+    matrix, eigenvalues below svdcut * max(lambda) raised to that floor;
+    dp_dchiv = |det L|."""
+
+    def __init__(self, mean, cov, svdcut=1e-15):
+        mean = np.asarray(mean, float)
+        self.shape = mean.shape
+        self.meanflat = mean.reshape(-1).copy()
+        n = self.meanflat.size
+        cov = np.asarray(cov, float).reshape(n, n)
+        d = np.sqrt(np.diag(cov))
+        if np.any(d <= 0):
+            raise ValueError('parameter variances must be positive')
+        corr = cov / np.outer(d, d)
+        lam, vec = np.linalg.eigh(0.5 * (corr + corr.T))
+        lam = np.maximum(lam, abs(svdcut) * lam.max())
+        self.L = d[:, None] * vec * np.sqrt(lam)[None, :]
+        self.dp_dchiv = float(np.prod(d) * np.prod(np.sqrt(lam)))
+        self.size = n
+        self.cov = cov
+
+    def pflat(self, chiv):
+        """Flat parameters (n, size) for chiv (n, size)."""
+        return self.meanflat[None, :] + chiv @ self.L.T
+
+
+def _param(param):
+    if hasattr(param, 'mean') and hasattr(param, 'cov'):
+        return np.asarray(param.mean, float), np.asarray(param.cov, float)
+    mean, cov = param
+    return np.asarray(mean, float), np.asarray(cov, float)
+
+
+def _call_user(fn, p):
+    """fn at parameter points p (n, *shape) in fn's batch convention;
+    returns an (n, ...) array or a dict of them."""
+    kind = getattr(fn, 'fcntype', 'scalar')
+    if kind == 'lbatch':
+        out = fn(p)
+        if hasattr(out, 'keys'):
+            return {k: np.asarray(v, float) for k, v in out.items()}
+        return np.asarray(out, float)
+    if kind == 'rbatch':
+        out = fn(np.moveaxis(p, 0, -1))
+        if hasattr(out, 'keys'):
+            return {k: np.moveaxis(np.asarray(v, float), -1, 0) for k, v in out.items()}
+        return np.moveaxis(np.asarray(out, float), -1, 0)
+    outs = [fn(pi) for pi in p]
+    if hasattr(outs[0], 'keys'):
+        return {k: np.array([o[k] for o in outs], float) for k in outs[0]}
+    return np.array(outs, float)
+
+
+def _flatten_f(fp, n):
+    """(n, m) columns and a rebuild layout for an array or dict output."""
+    if isinstance(fp, dict):
+        cols, layout, a = [], [], 0
+        for k, v in fp.items():
+            shape = v.shape[1:]
+            m = int(np.prod(shape)) if shape else 1
+            cols.append(v.reshape(n, m))
+            layout.append((k, shape, a, a + m))
+            a += m
+        return np.concatenate(cols, axis=1), ('dict', layout)
+    shape = fp.shape[1:]
+    return fp.reshape(n, -1), ('array', shape)
+
+
+class PDFEV(GVar):
+    """Expectation value of a scalar f(p) (__init__.py:81-176): a GVar with
+    ``pdfnorm`` (the integral of the PDF) and ``results`` (the running
+    average of the two integrals), whose attributes (summary, chi2, Q, ...)
+    it forwards."""
+
+    def __init__(self, mean, sdev, results=None, pdfnorm=None):
+        super().__init__(mean, sdev)
+        self.results = results
+        self.pdfnorm = pdfnorm
+
+    def __getattr__(self, k):
+        if k in ('results', 'pdfnorm') or k.startswith('__'):
+            raise AttributeError(k)
+        return getattr(self.results, k)
+
+
+class PDFEVArray(GVarArray):
+    """Expectation values of an array-valued f(p) (__init__.py:178-272)."""
+
+    def __init__(self, mean, cov, shape, results=None, pdfnorm=None):
+        super().__init__(mean, cov, shape)
+        self.results = results
+        self.pdfnorm = pdfnorm
+
+    def __getattr__(self, k):
+        if k in ('results', 'pdfnorm') or k.startswith('_'):
+            raise AttributeError(k)
+        return getattr(self.results, k)
+
+
+class PDFEVDict(dict):
+    """Expectation values of a dictionary-valued f(p) (__init__.py:274-371):
+    one GVar or GVarArray per key; ``cov`` is the covariance of all values,
+    keys in order."""
+
+    def __init__(self, items, cov, results=None, pdfnorm=None):
+        super().__init__(items)
+        self.cov = cov
+        self.results = results
+        self.pdfnorm = pdfnorm
+
+    def __getattr__(self, k):
+        if k in ('results', 'pdfnorm', 'cov') or k.startswith('_'):
+            raise AttributeError(k)
+        return getattr(self.results, k)
+
+
+def _ratio(mean, cov, ipdf):
+    """E = F / P and its covariance to first order, from the means and
+    covariance of (P, F_1..F_m) with P at index ipdf."""
+    m = len(mean)
+    fi = [i for i in range(m) if i != ipdf]
+    P = mean[ipdf]
+    F = mean[fi]
+    J = np.zeros((len(fi), m))
+    J[np.arange(len(fi)), fi] = 1.0 / P
+    J[:, ipdf] = -F / P ** 2
+    return F / P, J @ cov @ J.T
+
+
+class PDFIntegrator(Integrator):
+    """Expectation values with respect to a PDF (__init__.py:373-1188).
+
+    ``param``: ``(mean, cov)`` of the Gaussian that shapes the integration
+    variables (or an object with ``.mean`` and ``.cov``); ``pdf``: the
+    (unnormalised) density, a function of p in any batch convention, or
+    None for the Gaussian itself; ``adapt_to_pdf``: vegas adapts to the
+    PDF (True) or to f(p) pdf(p); ``limit``, ``scale``, ``svdcut``: as in
+    the reference.  Other keywords go to :class:`Integrator`."""
+
+    def __init__(self, param, pdf=None, adapt_to_pdf=True, limit=100., scale=1., svdcut=1e-15, **kargs):
+        if isinstance(param, PDFIntegrator):
+            super().__init__(param)
+            for k in ('param_pdf', 'pdf', 'adapt_to_pdf', 'limit', 'scale', 'svdcut', '_param'):
+                setattr(self, k, getattr(param, k))
+            if kargs:
+                self.set(kargs)
+            return
+        self._param = _param(param)
+        self.param_pdf = _GaussPDF(*self._param, svdcut=svdcut)
+        self.limit = abs(limit)
+        self.scale = abs(scale)
+        self.svdcut = svdcut
+        self.pdf = pdf
+        self.adapt_to_pdf = adapt_to_pdf
+        grid = self._make_map(self.limit / self.scale)
+        kargs = {k: v for k, v in kargs.items() if k != 'uses_jac'}
+        super().__init__(AdaptiveMap(self.param_pdf.size * [list(grid)]), **kargs)
+
+    def __reduce__(self):
+        _, _, odict = Integrator.__reduce__(self)
+        return (PDFIntegrator, (self._param, self.pdf, self.adapt_to_pdf, self.limit, self.scale, self.svdcut),
+                odict)
+
+    def set(self, ka={}, **kargs):
+        kargs = dict(ka, **kargs) if kargs else dict(ka)
+        if 'param' in kargs:
+            raise ValueError("Can't reset param.")
+        old = {}
+        for k in ('pdf', 'adapt_to_pdf'):
+            if k in kargs:
+                old[k] = getattr(self, k, None)
+                setattr(self, k, kargs.pop(k))
+        kargs.pop('uses_jac', None)
+        if kargs:
+            old.update(super().set(kargs))
+        return old
+
+    @staticmethod
+    def _make_map(limit):
+        """Grid nodes in theta = arctan(chiv) adapted to the unit Gaussian
+        (__init__.py:576-592: 2000 points, 100 increments, 10 passes of
+        alpha 1.5)."""
+        ny = 2000
+        y = np.random.default_rng(20240601).random((ny, 1))
+        lim = math.atan(limit)
+        m = AdaptiveMap([[-lim, lim]], ninc=100)
+        for _ in range(10):
+            theta, jac = m.map(y)
+            theta = theta.cpu().numpy()[:, 0]
+            jac = jac.cpu().numpy()
+            t = np.tan(theta)
+            fx = (t ** 2 + 1) * np.exp(-(t ** 2) / 2.)
+            m.add_training_data(y, (jac * fx) ** 2)
+            m.adapt(alpha=1.5)
+        return np.array(m.extract_grid()[0])
+
+    def _p_and_weight(self, theta):
+        """Parameters p (n, *shape) and dp/dtheta * pdf(p) at theta (n, size)
+        (__init__.py:594-630)."""
+        t = np.tan(theta)
+        chiv = self.scale * t
+        dp_dtheta = np.prod(self.scale * (t ** 2 + 1.), axis=1) * self.param_pdf.dp_dchiv
+        pflat = self.param_pdf.pflat(chiv)
+        p = pflat.reshape((len(theta),) + self.param_pdf.shape)
+        if self.pdf is None:
+            pdfv = np.prod(np.exp(-(chiv ** 2) / 2.) / math.sqrt(2 * math.pi), axis=1) / self.param_pdf.dp_dchiv
+        else:
+            pdfv = np.asarray(_call_user(self.pdf, p), float).reshape(len(theta), -1)
+            pdfv = np.prod(pdfv, axis=1)
+        return p, dp_dtheta * pdfv
+
+    def __call__(self, f=None, save=None, saveall=None, **kargs):
+        """E[f(p)] over the PDF (__init__.py:632-718): a PDFEV, PDFEVArray or
+        PDFEVDict; with f=None the integral of the PDF alone (an RAvgArray
+        with ``pdfnorm``)."""
+        kargs.pop('uses_jac', None)
+        if kargs:
+            self.set(kargs)
+        layout = {}
+        pdf_first = self.adapt_to_pdf or f is None
+
+        @lbatchintegrand
+        def integrand(theta):
+            p, w = self._p_and_weight(theta)
+            n = len(theta)
+            if f is None:
+                return w[:, None]
+            cols, layout['l'] = _flatten_f(_call_user(f, p), n)
+            fw = cols * w[:, None]
+            return np.concatenate([w[:, None], fw] if pdf_first else [fw, w[:, None]], axis=1)
+
+        results = Integrator.__call__(self, integrand, save=save, saveall=saveall)
+        ipdf = 0 if pdf_first else len(results.mean) - 1
+        pdfnorm = GVar(results.mean[ipdf], math.sqrt(max(results.cov[ipdf, ipdf], 0.0)))
+        if pdfnorm.mean == 0:
+            raise RuntimeError('Integral of PDF vanishes; increase neval?')
+        if f is None:
+            results.pdfnorm = pdfnorm
+            return results
+        mean, cov = _ratio(np.asarray(results.mean, float).reshape(-1), results.cov, ipdf)
+        kind, lay = layout['l']
+        if kind == 'array':
+            if lay == ():
+                return PDFEV(mean[0], math.sqrt(max(cov[0, 0], 0.0)), results, pdfnorm)
+            return PDFEVArray(mean, cov, lay, results, pdfnorm)
+        items = {}
+        for k, shape, a, b in lay:
+            if shape == ():
+                items[k] = GVar(mean[a], math.sqrt(max(cov[a, a], 0.0)))
+            else:
+                items[k] = GVarArray(mean[a:b], cov[a:b, a:b], shape)
+        return PDFEVDict(items, cov, results, pdfnorm)
+
+    def stats(self, f=None, moments=False, histograms=False, **kargs):
+        """Mean and covariance of the distribution of f(p) under the PDF
+        (__init__.py:729-946): E[f] and E[f f^T] - E[f] E[f]^T from one
+        integration.  f=None means f(p) = p.  ``moments`` and
+        ``histograms`` need gvar and are not provided."""
+        if moments or histograms:
+            raise NotImplementedError('stats(moments=..., histograms=...) needs gvar')
+        shape_box = {}
+
+        @lbatchintegrand
+        def fff(p):
+            fp = p if f is None else _call_user(f, p)
+            if isinstance(fp, dict):
+                raise NotImplementedError('stats of dictionary-valued f')
+            fp = np.asarray(fp, float)
+            n = fp.shape[0]
+            shape_box['s'] = fp.shape[1:]
+            v = fp.reshape(n, -1)
+            il = np.tril_indices(v.shape[1])
+            return np.concatenate([v, (v[:, :, None] * v[:, None, :])[:, il[0], il[1]]], axis=1)
+
+        ev = self(fff, **kargs)
+        m = ev.mean.reshape(-1)
+        k = int(round((math.sqrt(8 * len(m) + 9) - 3) / 2))  # k + k(k+1)/2 = len(m)
+        mu = m[:k]
+        second = np.zeros((k, k))
+        il = np.tril_indices(k)
+        second[il] = m[k:]
+        second = second + np.tril(second, -1).T
+        cov = second - np.outer(mu, mu)
+        shape = shape_box['s']
+        if shape == ():
+            return GVar(mu[0], math.sqrt(max(cov[0, 0], 0.0)))
+        return GVarArray(mu, cov, shape)
+
+    def sample(self, nbatch, mode='rbatch'):
+        """Weighted samples of p from the PDF (__init__.py:1049-1188): at
+        least nbatch samples (whole iterations), weights summing to one;
+        the batch index last (``'rbatch'``) or first (``'lbatch'``)."""
+        neval = self.last_neval if self.last_neval > 0 else self.neval
+        nit = 1 if nbatch is None else int(nbatch) // neval
+        if nbatch is not None and nit * neval < nbatch:
+            nit += 1
+        samples, wgts = [], []
+        for _ in range(nit):
+            for theta, wgt in self.random_batch():
+                p, w = self._p_and_weight(theta)
+                wgts.append(wgt * w)
+                samples.append(p)
+        samples = np.concatenate(samples, axis=0)
+        wgts = np.concatenate(wgts)
+        wgts /= np.sum(wgts)
+        if mode == 'rbatch':
+            samples = np.moveaxis(samples, 0, -1)
+        return wgts, samples
+
+
+def restratify(integ, f=None, nitn=1, ndy=5, below_avg_nstrat=None, verbose=False, gamma=1., **vargs):
+    """A copy of ``integ`` whose strata per axis follow how much the
+    integral of f varies along each y axis (__init__.py:1314-1590): weight
+    w_mu = ndy * sum_i (dI_mu,i - I / ndy)^2 over ndy equal slices of y_mu;
+    nstrat_mu scaled by ((w_mu / geometric mean of w) * geometric mean of
+    nstrat / nstrat_mu) ** gamma, smallest first, the total kept by
+    rescaling the remaining axes; axes of below-average weight get
+    ``below_avg_nstrat`` strata when that is set.  The copy is then trained
+    on f for nitn iterations."""
+    if isinstance(integ, PDFIntegrator):
+        raise NotImplementedError('restratify of a PDFIntegrator')
+    if f is None:
+        raise ValueError('restratify needs the integrand f')
+    new = Integrator(integ)
+    if vargs:
+        new.set(vargs)
+    dim = new.dim
+    yst = np.linspace(0, 1, ndy + 1)
+    old_nstrat = np.array(new.nstrat)
+
+    @lbatchintegrand
+    def I_dI(x):
+        n = x.shape[0]
+        y = np.zeros(x.shape, float)
+        jac = np.zeros(n, float)
+        new.map.invmap(x, y, jac)
+        I = np.asarray(_call_user(f, x), float).reshape(n, -1)[:, 0]
+        dI = np.zeros((n, dim, ndy), float)
+        for mu in range(dim):
+            for i in range(ndy):
+                idx = (yst[i] <= y[:, mu]) & (y[:, mu] <= yst[i + 1])
+                dI[idx, mu, i] += I[idx]
+        return dict(I=I, dI=dI)
+
+    save = new.set(correlate_integrals=False, nitn=nitn, alpha=0, adapt=True)
+    result = new(I_dI)
+    new.set(save)
+    if verbose:
+        print('\n==================== restratify')
+        print('BEFORE:')
+        print(result.summary()[:-1])
+        print('nstrat =', np.array2string(old_nstrat, max_line_width=60, prefix=9 * ' '))
+        print('nhcube =', np.prod(old_nstrat), '\n')
+    # first-order gvar arithmetic takes means through: the weights are
+    # functions of the means
+    Im = float(np.asarray(result['I'].mean))
+    dIm = np.asarray(result['dI'].mean).reshape(dim, ndy)
+    weight = np.array([np.sum((dIm[mu] - Im / ndy) ** 2) * ndy for mu in range(dim)])
+    if verbose:
+        print('WEIGHTS:', weight)
+    w_avg = np.average(weight)
+    w_gm = np.prod(weight) ** (1 / dim)
+    nstrat_gm = np.prod(old_nstrat) ** (1 / dim)
+    nstrat = old_nstrat * ((weight / w_gm) * nstrat_gm / old_nstrat) ** gamma
+    nleft = dim
+    musort = np.array(np.argsort(nstrat))
+    new_nstrat = np.array(new.nstrat)
+    for mu in musort:
+        new_nstrat[mu] = nstrat[mu] if nstrat[mu] > 1 else 1
+        if below_avg_nstrat and weight[mu] < w_avg:
+            new_nstrat[mu] = below_avg_nstrat
+        nleft -= 1
+        if nleft > 0:
+            nstrat[musort[-nleft:]] *= (nstrat[mu] / new_nstrat[mu]) ** (1 / nleft)
+            nstrat[mu] = new_nstrat[mu]
+    new.set(nstrat=new_nstrat, neval=new.neval)
+    save = new.set(nitn=nitn, adapt=True)
+    training = new(f)
+    new.set(save)
+    new.restratify_weight = weight
+    if verbose:
+        print('\nAFTER:')
+        print(training.summary()[:-1])
+        print('nstrat =', np.array2string(np.asarray(new.nstrat), max_line_width=60, prefix=9 * ' '))
+        print('nhcube =', np.prod(new.nstrat))
+        print(20 * '=')
+    return new
