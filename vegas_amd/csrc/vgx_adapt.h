@@ -98,4 +98,66 @@ __device__ inline void adapt_axis_block(double* av, double* P, double* red, cons
     }
 }
 
+constexpr int kAdaptLdsMax = 4000;  // old increments handled in LDS (2 x 31.25 KB)
+
+struct AdaptArgs {
+    int dim;
+    int ld;       // old grid stride
+    int nb;       // ld - 1, stride of sum_f / n_f
+    int new_ld;
+    int has_training;
+    int chain;    // exact mode: one lane for all axes, buffer shared (pyx:534)
+    double alpha;
+    int32_t ninc[VGX_MAXDIM];
+    int32_t new_ninc[VGX_MAXDIM];
+    const double* grid;
+    const double* sum_f;
+    const double* n_f;
+    double* new_grid;
+    double* buf;  // exact mode scratch: avg and tmp, 2 x nb per axis (chain: one pair)
+    // in-place use by the integrator (all null for vgx_adapt):
+    double* old_rows;      // dim x ld copy of the old grid rows
+    double* clear_sum_f;   // training data to clear once read
+    double* clear_n_f;
+    const int32_t* status; // skip everything when nonzero (NaN seen)
+};
+
+// In-place mode: copy the axis' old row aside before it is overwritten and
+// return the row to read from.
+__device__ inline const double* source_row(const AdaptArgs& a, int d, int lanes) {
+    const double* g = a.grid + (int64_t)d * a.ld;
+    if (!a.old_rows) return g;
+    double* o = a.old_rows + (int64_t)d * a.ld;
+    for (int i = threadIdx.x; i < a.ld; i += lanes) o[i] = g[i];
+    return o;
+}
+
+__device__ inline void clear_row(const AdaptArgs& a, int d, int lanes) {
+    if (!a.clear_sum_f) return;
+    for (int i = threadIdx.x; i < a.nb; i += lanes) {
+        a.clear_sum_f[(int64_t)d * a.nb + i] = 0.0;
+        a.clear_n_f[(int64_t)d * a.nb + i] = kTiny;
+    }
+}
+
+// The parallel adapt of axis d by the calling workgroup, in place when
+// a.old_rows is set (av, P: kAdaptLdsMax doubles of LDS each; red: 40).
+__device__ inline void adapt_parallel_axis(const AdaptArgs& a, int d, double* av, double* P, double* red) {
+    const int old = a.ninc[d], nn = a.new_ninc[d];
+    double* ng = a.new_grid + (int64_t)d * a.new_ld;
+    const int T = blockDim.x;
+    const double* g = source_row(a, d, T);
+    for (int i = threadIdx.x; i < old; i += T) {
+        double v = 1.0;
+        if (a.alpha != 0 && old > 1 && a.has_training) {
+            double nf = a.n_f[(int64_t)d * a.nb + i];
+            v = nf > 0 ? a.sum_f[(int64_t)d * a.nb + i] / nf : 0.;
+        }
+        av[i] = v;
+    }
+    __syncthreads();
+    clear_row(a, d, T);
+    adapt_axis_block(av, P, red, g, ng, old, nn, a.alpha, a.new_ld);
+}
+
 }  // namespace vgx

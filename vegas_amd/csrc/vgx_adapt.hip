@@ -14,46 +14,6 @@
 
 namespace vgx {
 
-struct AdaptArgs {
-    int dim;
-    int ld;       // old grid stride
-    int nb;       // ld - 1, stride of sum_f / n_f
-    int new_ld;
-    int has_training;
-    int chain;    // exact mode: one lane for all axes, buffer shared (pyx:534)
-    double alpha;
-    int32_t ninc[VGX_MAXDIM];
-    int32_t new_ninc[VGX_MAXDIM];
-    const double* grid;
-    const double* sum_f;
-    const double* n_f;
-    double* new_grid;
-    double* buf;  // exact mode scratch: avg and tmp, 2 x nb per axis (chain: one pair)
-    // in-place use by the integrator (all null for vgx_adapt):
-    double* old_rows;      // dim x ld copy of the old grid rows
-    double* clear_sum_f;   // training data to clear once read
-    double* clear_n_f;
-    const int32_t* status; // skip everything when nonzero (NaN seen)
-};
-
-// In-place mode: copy the axis' old row aside before it is overwritten and
-// return the row to read from.
-__device__ inline const double* source_row(const AdaptArgs& a, int d, int lanes) {
-    const double* g = a.grid + (int64_t)d * a.ld;
-    if (!a.old_rows) return g;
-    double* o = a.old_rows + (int64_t)d * a.ld;
-    for (int i = threadIdx.x; i < a.ld; i += lanes) o[i] = g[i];
-    return o;
-}
-
-__device__ inline void clear_row(const AdaptArgs& a, int d, int lanes) {
-    if (!a.clear_sum_f) return;
-    for (int i = threadIdx.x; i < a.nb; i += lanes) {
-        a.clear_sum_f[(int64_t)d * a.nb + i] = 0.0;
-        a.clear_n_f[(int64_t)d * a.nb + i] = kTiny;
-    }
-}
-
 // Fill the padding of one output row and nothing else.
 __device__ inline void pad_row(double* ng, int nn, int new_ld) {
     for (int i = nn + 1; i < new_ld; ++i) ng[i] = ng[nn];
@@ -141,38 +101,21 @@ __global__ void adapt_exact_kernel(AdaptArgs a) {
     }
 }
 
-constexpr int kAdaptLdsMax = 4000;  // old increments handled in LDS (2 x 31.25 KB)
-
 __global__ __launch_bounds__(1024) void adapt_parallel_kernel(AdaptArgs a) {
     __shared__ double av[kAdaptLdsMax];
     __shared__ double P[kAdaptLdsMax];
     __shared__ double red[40];
     if (a.status && *a.status) return;
-    const int d = blockIdx.x;
-    const int old = a.ninc[d], nn = a.new_ninc[d];
-    double* ng = a.new_grid + (int64_t)d * a.new_ld;
-    const int T = blockDim.x;
-    const double* g = source_row(a, d, T);
-
-    for (int i = threadIdx.x; i < old; i += T) {
-        double v = 1.0;
-        if (a.alpha != 0 && old > 1 && a.has_training) {
-            double nf = a.n_f[(int64_t)d * a.nb + i];
-            v = nf > 0 ? a.sum_f[(int64_t)d * a.nb + i] / nf : 0.;
-        }
-        av[i] = v;
-    }
-    __syncthreads();
-    clear_row(a, d, T);
-    adapt_axis_block(av, P, red, g, ng, old, nn, a.alpha, a.new_ld);
+    adapt_parallel_axis(a, blockIdx.x, av, P, red);
 }
 
 // The integrator's adapt: in place on `grid`, skipped when *status != 0,
 // clearing the training data after reading it.
-int adapt_in_place(vgx_ctx* ctx, int dim, const int32_t* ninc, double* grid, int ld, double* sum_f,
-                   double* n_f, int has_training, double alpha, int exact, const int32_t* status,
-                   hipStream_t st) {
-    AdaptArgs a{};
+int adapt_in_place_setup(vgx_ctx* ctx, int dim, const int32_t* ninc, double* grid, int ld, double* sum_f,
+                         double* n_f, int has_training, double alpha, int exact, const int32_t* status,
+                         AdaptArgs* out, int* parallel) {
+    AdaptArgs& a = *out;
+    a = AdaptArgs{};
     a.dim = dim;
     a.ld = ld;
     a.nb = ld - 1;
@@ -200,10 +143,22 @@ int adapt_in_place(vgx_ctx* ctx, int dim, const int32_t* ninc, double* grid, int
     if (rc) return rc;
     a.old_rows = (double*)ctx->adapt_tmp.ptr;
     a.buf = a.old_rows + rows;
-    if (exact || a.chain)
-        hipLaunchKernelGGL(adapt_exact_kernel, dim3(a.chain ? 1 : dim), dim3(64), 0, st, a);
-    else
+    *parallel = !(exact || a.chain);
+    return VGX_OK;
+}
+
+int adapt_in_place(vgx_ctx* ctx, int dim, const int32_t* ninc, double* grid, int ld, double* sum_f,
+                   double* n_f, int has_training, double alpha, int exact, const int32_t* status,
+                   hipStream_t st) {
+    AdaptArgs a;
+    int parallel = 0;
+    int rc = adapt_in_place_setup(ctx, dim, ninc, grid, ld, sum_f, n_f, has_training, alpha, exact, status,
+                                  &a, &parallel);
+    if (rc) return rc;
+    if (parallel)
         hipLaunchKernelGGL(adapt_parallel_kernel, dim3(dim), dim3(1024), 0, st, a);
+    else
+        hipLaunchKernelGGL(adapt_exact_kernel, dim3(a.chain ? 1 : dim), dim3(64), 0, st, a);
     return check_launch();
 }
 

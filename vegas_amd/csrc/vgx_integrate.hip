@@ -19,6 +19,7 @@
 
 #include "vgx_fused.h"
 #include "vgx_strad.h"
+#include "vgx_adapt.h"
 
 namespace vgx {
 
@@ -232,9 +233,11 @@ __global__ void batch_tiles_kernel(const uint32_t* off, int32_t* tile_h, Scalars
     }
 }
 
+constexpr int kReduceThreads = 256;  // threads of the reduction / finalize code
+
 // Fixed-order tree over DD triples: red[k*6 .. k*6+5] for k < blockDim.x.
 __device__ void tree_dd3(double* red) {
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    for (int o = kReduceThreads / 2; o > 0; o >>= 1) {
         if (threadIdx.x < (unsigned)o) {
             const int i = threadIdx.x, j = threadIdx.x + o;
             for (int q = 0; q < 3; ++q) {
@@ -251,12 +254,11 @@ __device__ void tree_dd3(double* red) {
 // Adds partial sums (fused workgroups, then straddler workgroups, fixed
 // order) into the iteration accumulators sc->acc (DD mean, var, sigf sum).
 // One workgroup of kReduceThreads.
-constexpr int kReduceThreads = 256;
 __device__ void reduce_parts(const double* blk_part, int nblk, const double* strad_part, int nstrad,
                              Scalars* sc, double* red) {
     DD m{0, 0}, v{0, 0}, t{0, 0};
     const int tot = nblk + nstrad;
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    for (int i = threadIdx.x; i < tot; i += kReduceThreads) {
         const double* p = i < nblk ? blk_part + 6 * (size_t)i : strad_part + 6 * (size_t)(i - nblk);
         m = dd_sum(m, DD{p[0], p[1]});
         v = dd_sum(v, DD{p[2], p[3]});
@@ -319,18 +321,17 @@ __global__ __launch_bounds__(256) void sigf_pow_kernel(IterGeom g, double* sigf,
 // (single-batch iterations), the sum of sigf (accumulated with the sums, or
 // from the sigf_pow partials in exact mode), the estimate, sum_sigf bookkeeping and the FP64 running average; resets the
 // per-iteration scalars for the next iteration of the run.
-__global__ __launch_bounds__(kReduceThreads) void finalize_kernel(IterGeom g, Scalars* sc, vgx_state st,
-                                                                  int has_sigf, const double* sigf_part,
-                                                                  int nsigf, const double* blk_part, int nblk,
-                                                                  const double* strad_part, int nstrad) {
-    __shared__ double red[kReduceThreads * 6];
-    __shared__ double mv[2];
+// Runs on threads 0..kReduceThreads-1 of the calling workgroup (the others
+// must have returned); red: kReduceThreads * 6 doubles of LDS, mv: 2.
+__device__ void finalize_body(const IterGeom& g, Scalars* sc, const vgx_state& st, int has_sigf,
+                              const double* sigf_part, int nsigf, const double* blk_part, int nblk,
+                              const double* strad_part, int nstrad, double* red, double* mv) {
     if (nblk + nstrad > 0 && !(sc->status || sc->overflow))
         reduce_parts(blk_part, nblk, strad_part, nstrad, sc, red);
     const bool sig_parts = g.adaptive_strat && !g.fast_pow;
     if (sig_parts) {
         DD t{0, 0};
-        for (int k = threadIdx.x; k < nsigf; k += blockDim.x)
+        for (int k = threadIdx.x; k < nsigf; k += kReduceThreads)
             t = dd_sum(t, DD{sigf_part[2 * k], sigf_part[2 * k + 1]});
         red[2 * threadIdx.x] = t.hi;
         red[2 * threadIdx.x + 1] = t.lo;
@@ -387,7 +388,7 @@ __global__ __launch_bounds__(kReduceThreads) void finalize_kernel(IterGeom g, Sc
     constexpr int kChunk = kReduceThreads * 3;  // (mean, var) pairs in red[]
     auto stage = [&](int k0) {
         __syncthreads();
-        for (int k = k0 + (int)threadIdx.x; k < min(n, k0 + kChunk); k += blockDim.x) {
+        for (int k = k0 + (int)threadIdx.x; k < min(n, k0 + kChunk); k += kReduceThreads) {
             red[2 * (k - k0)] = k == it ? mv[0] : st.itn_mean[k];
             red[2 * (k - k0) + 1] = k == it ? mv[1] : st.itn_var[k];
         }
@@ -455,6 +456,39 @@ __global__ __launch_bounds__(kReduceThreads) void finalize_kernel(IterGeom g, Sc
     // next iteration of this run
     sc->wfn_max_bits = 0;
     for (int k = 0; k < 6; ++k) sc->acc[k] = 0.0;
+}
+
+__global__ __launch_bounds__(kReduceThreads) void finalize_kernel(IterGeom g, Scalars* sc, vgx_state st,
+                                                                  int has_sigf, const double* sigf_part,
+                                                                  int nsigf, const double* blk_part, int nblk,
+                                                                  const double* strad_part, int nstrad) {
+    __shared__ double red[kReduceThreads * 6];
+    __shared__ double mv[2];
+    finalize_body(g, sc, st, has_sigf, sigf_part, nsigf, blk_part, nblk, strad_part, nstrad, red, mv);
+}
+
+// Iteration close with the parallel adapt in one launch: workgroups
+// 0..dim-1 adapt one axis each (they read only the training histogram and
+// the grid), workgroup dim runs finalize_body on its first kReduceThreads
+// threads (it touches neither).  Waves that return early leave the
+// workgroup's barriers to the others.
+__global__ __launch_bounds__(1024) void close_kernel(IterGeom g, Scalars* sc, vgx_state st, int has_sigf,
+                                                     const double* sigf_part, int nsigf, const double* blk_part,
+                                                     int nblk, const double* strad_part, int nstrad,
+                                                     AdaptArgs a) {
+    if ((int)blockIdx.x == a.dim) {
+        __shared__ double red[kReduceThreads * 6];
+        __shared__ double mv[2];
+        if (threadIdx.x >= (unsigned)kReduceThreads) return;
+        finalize_body(g, sc, st, has_sigf, sigf_part, nsigf, blk_part, nblk, strad_part, nstrad, red, mv);
+        return;
+    }
+    __shared__ double av[kAdaptLdsMax];
+    __shared__ double P[kAdaptLdsMax];
+    __shared__ double ared[40];
+    // the flags finalize_body may raise are set before this launch
+    if (sc->status || sc->overflow) return;
+    adapt_parallel_axis(a, blockIdx.x, av, P, ared);
 }
 
 __global__ void sigf_reset_kernel(double* sigf, int64_t n, Scalars* sc) {
@@ -761,17 +795,33 @@ static int iter_end_kernels(vgx_ctx* ctx, Run* R, const vgx_config* c, vgx_state
                                (const double*)st->grid, (const uint32_t*)ctx->off.ptr, (const double*)R->errv,
                                st->sum_f, st->n_f, (const Scalars*)R->sc);
     }
+    const int do_adapt = c->alpha > 0 && c->adapt;
+    const int has_training = g.train_points || g.train_errors;
+    const int exact = (c->flags & VGX_FLAG_EXACT_ADAPT) ? 1 : 0;
+    int rc;
+    if (do_adapt) {
+        AdaptArgs a;
+        int parallel = 0;
+        rc = adapt_in_place_setup(ctx, c->dim, c->ninc, st->grid, c->ld, st->sum_f, st->n_f, has_training,
+                                  c->alpha, exact, &R->sc->status, &a, &parallel);
+        if (rc) return rc;
+        if (parallel) {
+            hipLaunchKernelGGL(close_kernel, dim3(c->dim + 1), dim3(1024), 0, s, g, R->sc, *st, has_sigf,
+                               (const double*)R->sigf_part, 256, (const double*)R->io.blk_part,
+                               parts ? R->fused_blocks : 0, (const double*)R->sa.strad_part,
+                               parts ? R->strad_blocks : 0, a);
+            return check_launch();
+        }
+    }
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kReduceThreads), 0, s, g, R->sc, *st, has_sigf,
                        (const double*)R->sigf_part, 256, (const double*)R->io.blk_part,
                        parts ? R->fused_blocks : 0, (const double*)R->sa.strad_part,
                        parts ? R->strad_blocks : 0);
-    int rc = check_launch();
+    rc = check_launch();
     if (rc) return rc;
-    if (c->alpha > 0 && c->adapt) {
-        const int has_training = g.train_points || g.train_errors;
+    if (do_adapt)
         rc = adapt_in_place(ctx, c->dim, c->ninc, st->grid, c->ld, st->sum_f, st->n_f, has_training,
-                            c->alpha, (c->flags & VGX_FLAG_EXACT_ADAPT) ? 1 : 0, &R->sc->status, s);
-    }
+                            c->alpha, exact, &R->sc->status, s);
     return rc;
 }
 

@@ -249,6 +249,12 @@ struct BufferFcn {
 int adapt_in_place(vgx_ctx* ctx, int dim, const int32_t* ninc, double* grid, int ld, double* sum_f,
                    double* n_f, int has_training, double alpha, int exact, const int32_t* status,
                    hipStream_t st);
+// Its set-up alone: fills *args and sets *parallel when the workgroup-per-
+// axis kernel applies (the caller then runs adapt_parallel_axis itself).
+struct AdaptArgs;
+int adapt_in_place_setup(vgx_ctx* ctx, int dim, const int32_t* ninc, double* grid, int ld, double* sum_f,
+                         double* n_f, int has_training, double alpha, int exact, const int32_t* status,
+                         AdaptArgs* args, int* parallel);
 
 // Launchers, one per (integrand, dim) instantiation (vgx_fused_*.hip).
 // ev0/ev1 (may be null): events that take the launch's own start and end
