@@ -21,6 +21,7 @@ struct TileGeom {
     int exact_div, hist_lds, train_points, adaptive_strat, fast_pow, train_errors;
     uint32_t iteration, seed_lo, seed_hi;
     double beta_half, dv_y;
+    double jn;  // product of ninc over the axes (the integer stratum path's Jacobian factor)
 };
 
 __host__ __device__ inline TileGeom tile_geom(const IterGeom& g) {
@@ -39,6 +40,7 @@ __host__ __device__ inline TileGeom tile_geom(const IterGeom& g) {
     t.seed_hi = g.seed_hi;
     t.beta_half = g.beta_half;
     t.dv_y = g.dv_y;
+    t.jn = g.jn;
     return t;
 }
 
@@ -183,7 +185,15 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
             kv = (yv > 0 && yv < 1) ? (k < ni ? k : ni - 1) : -1;
         }
         const double* gd = G + d * g.ld;
-        if (PER || k < ni) {  // PER: k < ninc always; no branch, loads can be issued early
+        if (PER) {
+            // k < ninc always: no branch, the loads can be issued early; x by
+            // one FMA, and the ninc factors of J applied once (g.jn) after
+            // the axes (the FP64 parity mode takes the branch below)
+            const double g0 = gd[k], g1 = gd[k + 1];
+            const double inc = g1 - g0;
+            x[d] = __builtin_fma(inc, dl, g0);
+            J *= inc;
+        } else if (k < ni) {
             const double g0 = gd[k], g1 = gd[k + 1];
             const double inc = g1 - g0;
             x[d] = g0 + inc * dl;
@@ -194,6 +204,7 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
         }
         kbin[d] = kv;
     }
+    if (PER) J *= g.jn;
 }
 // PER: bit d set where point p of the hcube with coordinates pc has y = 0
 // exactly on axis d (stratum 0 and a zero uniform; probability 2^-24 per

@@ -612,6 +612,8 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
     g->per_all = g->rng32 && !g->exact_div;
     for (int d = 0; d < c->dim; ++d)
         if (make_axis(*g, d).per == 0 || c->ninc[d] >= 65536) g->per_all = 0;
+    g->jn = 1.0;
+    for (int d = 0; d < c->dim; ++d) g->jn *= (double)c->ninc[d];
     const int64_t nmin = c->min_neval_hcube;
     const int64_t nmax = c->max_neval_hcube > nmin ? c->max_neval_hcube : nmin;
     const int64_t avg = (int64_t)((double)c->neval / (double)c->nhcube);

@@ -46,6 +46,7 @@ struct IterGeom {
     double beta_half;
     double beta;
     double dv_y;
+    double jn;             // product over the axes of (double)ninc
     int32_t nmin, nmax, avg_nh, pad2;
     uint32_t seed_lo, seed_hi;
     uint32_t iteration;  // RNG stream of this iteration
