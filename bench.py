@@ -112,17 +112,41 @@ def copy_calibration():
 
 def cpu_baseline(wl, iters):
     """Reference-structured FP64 oracle on `iters` iterations of the workload
-    (single thread, integrand inlined in C)."""
+    (integrand inlined in C): one thread alone, then one independent run per
+    host core (up to 16, the box's CPU share) side by side; the C call runs
+    without the GIL.  value is the all-core rate."""
+    import threading
     from oracle import oracle as orc
     f = wl["f"]
     g = orc.GaussMix(f.A, f.B, f.centers)
-    I = orc.OracleIntegrator([(0, 1)] * wl["dim"], neval=wl["neval"], seed=1)
+
+    def one(seed, out):
+        I = orc.OracleIntegrator([(0, 1)] * wl["dim"], neval=wl["neval"], seed=seed)
+        _, _, nev = I.run(g, iters)
+        out.append(float(nev.sum()))
+
+    solo = []
     t0 = time.perf_counter()
-    _, _, nev = I.run(g, iters)
-    dt = time.perf_counter() - t0
-    return dict(value=float(nev.sum()) / dt, unit="evals/s", cores=1, kind="port",
-                sample="%d iteration(s) of the 8-D double peak, neval 1e7 (%d evals) through "
-                       "the FP64 C oracle, 1 thread, %.1f s" % (iters, int(nev.sum()), dt))
+    one(1, solo)
+    dt1 = time.perf_counter() - t0
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    ncores = max(1, min(ncores, 16))
+    outs = []
+    ths = [threading.Thread(target=one, args=(s + 1, outs)) for s in range(ncores)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dtn = time.perf_counter() - t0
+    return dict(value=sum(outs) / dtn, unit="evals/s", cores=ncores, kind="port",
+                value_1core=solo[0] / dt1,
+                sample="%d iteration(s) of the 8-D double peak, neval 1e7, through the FP64 C oracle: "
+                       "%d independent runs on %d threads in %.1f s (%.3g evals); one thread alone "
+                       "%.3g evals/s" % (iters, ncores, ncores, dtn, sum(outs), solo[0] / dt1))
 
 
 def ridge(a):
@@ -352,7 +376,10 @@ def main():
                      "unit": "TFLOP/s", "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
                      "traffic": traffic, "kernel": "fused_kernel<8, true, GaussMix32>",
                      "kernel_ms_avg": avg_ms, "flops_per_eval": fpe,
-                     "note": "VALU-bound (no MFMA in this kernel); peak is the FP32 vector peak"},
+                     "hbm_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if (traffic and kern_ms) else None,
+                     "hbm_frac": (traffic / (avg_ms * 1e-3) / 8.0e12) if (traffic and kern_ms) else None,
+                     "note": "VALU-bound (no MFMA in this kernel); peak is the FP32 vector peak; "
+                             "hbm_GBps = measured HBM bytes per launch / launch time, against 8 TB/s"},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_iters)
