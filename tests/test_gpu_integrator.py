@@ -49,6 +49,48 @@ def test_first_iteration_parity(va, orc, neval, nstrat):
     assert abs(integ.sum_sigf - o.sigf.sum()) <= 1e-10 * o.sigf.sum()
 
 
+DIMS = [1, 2, 3, 5, 7, 9, 12, 13, 17]
+
+
+@pytest.mark.parametrize("dim", DIMS)
+def test_first_iteration_parity_dims(va, orc, dim):
+    """Every dimension instantiation of the fused sampler (1-12 unrolled,
+    13+ the runtime-dim path): FP64 parity mode against the oracle."""
+    c = [0.5 + 0.02 * ((d % 3) - 1) for d in range(dim)]
+    f = va.GaussianMixture([1.0], [8.0], [c])
+    fo = orc.GaussMix([1.0], [8.0], [c])
+    integ = va.Integrator([[0, 1]] * dim, neval=20000, seed=23, precision='f64', exact=True, alpha=0.0)
+    r = integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * dim, neval=20000, seed=23, alpha=0.0)
+    out, sf, nf = o.iterate(fo)
+    assert integ.last_neval == out.last_neval
+    m = r.itn_results[0]
+    assert abs(m.mean - out.mean) <= 1e-12 * abs(out.mean)
+    assert abs(m.var - out.var) <= 1e-10 * out.var
+    np.testing.assert_allclose(integ.sigf, o.sigf, rtol=1e-11)
+
+
+@pytest.mark.parametrize("dim", DIMS)
+def test_fused_equals_staged_f32_dims(va, dim):
+    """24-bit uniforms (the integer stratum path wherever the axes allow):
+    the fused sampler with its FP32 functor and the staged path with a
+    numpy integrand see the same points, so their first-iteration estimates
+    agree to the FP32 integrand's rounding."""
+    c = [0.5 + 0.02 * ((d % 3) - 1) for d in range(dim)]
+    f = va.GaussianMixture([1.0], [8.0], [c])
+    cc = np.array(c)
+
+    @va.lbatchintegrand
+    def fl(x):
+        return np.exp(-8.0 * ((x - cc) ** 2).sum(1))
+
+    ra = va.Integrator([[0, 1]] * dim, neval=20000, seed=29, precision='f32')(f, nitn=1)
+    rb = va.Integrator([[0, 1]] * dim, neval=20000, seed=29, precision='f32')(fl, nitn=1)
+    for a, b in zip(ra.itn_results, rb.itn_results):
+        assert abs(a.mean - b.mean) <= 2e-5 * abs(b.mean)
+        assert abs(a.sdev - b.sdev) <= 1e-3 * b.sdev
+
+
 @pytest.mark.parametrize("neval,scale", [(30000, 3.0), (2e5, 6.0)])
 def test_parity_variable_nh(va, orc, neval, scale):
     """Heavy-tailed sigf: n_h from 2 to thousands, hcubes spanning many wave
