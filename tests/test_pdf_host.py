@@ -1,0 +1,91 @@
+"""Host logic of PDFIntegrator and restratify (vegas_amd/pdf.py), on the
+CPU: the Gaussian change of variables, first-order error propagation of the
+ratio E = F / P, the batch conventions of user functions, and the strata
+re-division.  Each is checked against what the mathematics fixes (the
+covariance rebuilt from L, the determinant, a finite-difference Jacobian of
+the ratio, products and orderings of the new strata)."""
+import numpy as np
+import pytest
+
+pdf = pytest.importorskip("vegas_amd.pdf")
+
+
+def test_gauss_pdf_change_of_variables():
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((4, 4))
+    cov = A @ A.T + 0.1 * np.eye(4)
+    mean = rng.standard_normal(4)
+    g = pdf._GaussPDF(mean, cov, svdcut=1e-15)
+    np.testing.assert_allclose(g.L @ g.L.T, cov, rtol=1e-12, atol=1e-12)
+    assert abs(g.dp_dchiv - np.sqrt(np.linalg.det(cov))) <= 1e-12 * g.dp_dchiv
+    np.testing.assert_array_equal(g.pflat(np.zeros((1, 4)))[0], mean)
+    # a unit chiv step moves p along a column of L
+    e = np.zeros((1, 4))
+    e[0, 2] = 1.0
+    np.testing.assert_allclose(g.pflat(e)[0] - mean, g.L[:, 2], rtol=1e-14)
+
+
+def test_gauss_pdf_svdcut_floor():
+    cov = np.array([[1.0, 1.0], [1.0, 1.0]])  # singular
+    g = pdf._GaussPDF([0.0, 0.0], cov, svdcut=1e-6)
+    lam = np.linalg.eigvalsh(np.linalg.inv(np.diag([1.0, 1.0])) @ (g.L @ g.L.T))
+    assert lam.min() > 0 and abs(lam.min() / lam.max() - 1e-6) < 1e-9
+
+
+def test_ratio_propagation_matches_finite_differences():
+    rng = np.random.default_rng(2)
+    mean = np.array([2.0, 6.0, -3.0, 0.5])  # P at index 0
+    B = rng.standard_normal((4, 4)) * 0.1
+    cov = B @ B.T
+    E, cE = pdf._ratio(mean, cov, 0)
+    np.testing.assert_allclose(E, mean[1:] / mean[0])
+    J = np.zeros((3, 4))
+    h = 1e-6
+    for k in range(4):
+        dp = mean.copy()
+        dm = mean.copy()
+        dp[k] += h
+        dm[k] -= h
+        J[:, k] = (dp[1:] / dp[0] - dm[1:] / dm[0]) / (2 * h)
+    np.testing.assert_allclose(cE, J @ cov @ J.T, rtol=1e-7, atol=1e-14)
+    # P last (adapt_to_pdf=False layout)
+    E2, cE2 = pdf._ratio(np.roll(mean, -1), np.roll(np.roll(cov, -1, 0), -1, 1), 3)
+    np.testing.assert_allclose(E2, E)
+    np.testing.assert_allclose(cE2, cE, rtol=1e-12)
+
+
+def test_call_user_conventions():
+    from vegas_amd.batchintegrand import lbatchintegrand, rbatchintegrand
+    p = np.arange(12.0).reshape(4, 3)
+
+    @lbatchintegrand
+    def fl(q):
+        return q[:, 0] * 2
+
+    @rbatchintegrand
+    def fr(q):
+        return dict(a=q[0] * 2, b=q[1:])
+
+    def fs(q):
+        return q.sum()
+
+    np.testing.assert_array_equal(pdf._call_user(fl, p), p[:, 0] * 2)
+    r = pdf._call_user(fr, p)
+    np.testing.assert_array_equal(r['a'], p[:, 0] * 2)
+    np.testing.assert_array_equal(r['b'], p[:, 1:])
+    np.testing.assert_array_equal(pdf._call_user(fs, p), p.sum(1))
+    cols, (kind, lay) = pdf._flatten_f(r, 4)
+    assert kind == 'dict' and cols.shape == (4, 3)
+    assert [(k, s, a, b) for k, s, a, b in lay] == [('a', (), 0, 1), ('b', (2,), 1, 3)]
+
+
+def test_restratified_strata():
+    old = np.array([8, 8, 8, 8])
+    # one dominant axis takes the strata; the product never grows
+    new = pdf._restratified([1e6, 1.0, 1.0, 1.0], old, below_avg_nstrat=1)
+    assert list(new[1:]) == [1, 1, 1] and new[0] <= np.prod(old) and new[0] > 0.9 * np.prod(old)
+    # larger weight, more strata (gamma = 1); gamma = 0 keeps the old ones
+    new = pdf._restratified([16.0, 4.0, 2.0, 1.0], old)
+    assert new[0] >= new[1] >= new[2] >= new[3] >= 1 and np.prod(new) <= np.prod(old)
+    new0 = pdf._restratified([16.0, 4.0, 2.0, 1.0], old, gamma=0.0)
+    assert np.all(np.abs(new0 - old) <= 1)

@@ -345,6 +345,33 @@ class PDFIntegrator(Integrator):
         return wgts, samples
 
 
+def _restratified(weight, old_nstrat, gamma=1., below_avg_nstrat=None):
+    """New strata per axis from the axis weights (__init__.py:1350-1366):
+    nstrat_mu = old_mu * ((w_mu / gm(w)) * gm(old) / old_mu) ** gamma, then,
+    smallest first, each axis is cut to an integer (at least 1, or
+    below_avg_nstrat for below-average weights) and the axes still to come
+    absorb the change so the product is carried along."""
+    weight = np.asarray(weight, float)
+    old_nstrat = np.asarray(old_nstrat)
+    dim = len(old_nstrat)
+    w_avg = np.average(weight)
+    w_gm = np.prod(weight) ** (1 / dim)
+    nstrat_gm = np.prod(old_nstrat) ** (1 / dim)
+    nstrat = old_nstrat * ((weight / w_gm) * nstrat_gm / old_nstrat) ** gamma
+    nleft = dim
+    musort = np.array(np.argsort(nstrat))
+    new_nstrat = np.array(old_nstrat)
+    for mu in musort:
+        new_nstrat[mu] = nstrat[mu] if nstrat[mu] > 1 else 1
+        if below_avg_nstrat and weight[mu] < w_avg:
+            new_nstrat[mu] = below_avg_nstrat
+        nleft -= 1
+        if nleft > 0:
+            nstrat[musort[-nleft:]] *= (nstrat[mu] / new_nstrat[mu]) ** (1 / nleft)
+            nstrat[mu] = new_nstrat[mu]
+    return new_nstrat
+
+
 def restratify(integ, f=None, nitn=1, ndy=5, below_avg_nstrat=None, verbose=False, gamma=1., **vargs):
     """A copy of ``integ`` whose strata per axis follow how much the
     integral of f varies along each y axis (__init__.py:1314-1590): weight
@@ -395,21 +422,7 @@ def restratify(integ, f=None, nitn=1, ndy=5, below_avg_nstrat=None, verbose=Fals
     weight = np.array([np.sum((dIm[mu] - Im / ndy) ** 2) * ndy for mu in range(dim)])
     if verbose:
         print('WEIGHTS:', weight)
-    w_avg = np.average(weight)
-    w_gm = np.prod(weight) ** (1 / dim)
-    nstrat_gm = np.prod(old_nstrat) ** (1 / dim)
-    nstrat = old_nstrat * ((weight / w_gm) * nstrat_gm / old_nstrat) ** gamma
-    nleft = dim
-    musort = np.array(np.argsort(nstrat))
-    new_nstrat = np.array(new.nstrat)
-    for mu in musort:
-        new_nstrat[mu] = nstrat[mu] if nstrat[mu] > 1 else 1
-        if below_avg_nstrat and weight[mu] < w_avg:
-            new_nstrat[mu] = below_avg_nstrat
-        nleft -= 1
-        if nleft > 0:
-            nstrat[musort[-nleft:]] *= (nstrat[mu] / new_nstrat[mu]) ** (1 / nleft)
-            nstrat[mu] = new_nstrat[mu]
+    new_nstrat = _restratified(weight, old_nstrat, gamma, below_avg_nstrat)
     new.set(nstrat=new_nstrat, neval=new.neval)
     save = new.set(nitn=nitn, adapt=True)
     training = new(f)
