@@ -46,8 +46,10 @@ extern "C" int vgx_ctx_create(vgx_ctx** out) {
         return rc;
     }
     c->device = c->info.device;
+    // timing-only events: no system-scope fence (cache writeback and
+    // invalidate) when they complete, which would stall the next launch
     for (int i = 0; i < 2 * vgx_ctx::kMaxTimed; ++i) {
-        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+        if (hipEventCreateWithFlags(&c->ev[i], hipEventDisableSystemFence) != hipSuccess) {
             delete c;
             return VGX_EHIP;
         }
