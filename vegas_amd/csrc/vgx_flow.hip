@@ -785,21 +785,39 @@ __global__ __launch_bounds__(kWgThreads) void flow_wgrad_kernel(const uint16_t* 
     }
 }
 
-// Sum of the workgroup partials (fixed order: part 0, 1, ...) into grad
-// [kPLayer]; the loads of eight parts are issued before their adds.
-__global__ void flow_wgrad_reduce_kernel(const float* part, int nparts, float* grad) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kPLayer; i += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        int p = 0;
-        for (; p + 8 <= nparts; p += 8) {
-            float v[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = part[(size_t)(p + j) * kPLayer + i];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) s += v[j];
+// Sum of the workgroup partials into grad [kPLayer], in a fixed order: a
+// workgroup takes 32 consecutive entries; its 8 part groups (thread j sums
+// parts j, j + 8, ... in order) are added in group order.  Each wave reads
+// 2 x 128 contiguous bytes per load, and 709 workgroups keep the loads of
+// the whole device in flight.
+constexpr int kRedCols = 32, kRedGroups = 8;
+
+__global__ __launch_bounds__(kRedCols * kRedGroups) void flow_wgrad_reduce_kernel(const float* part, int nparts,
+                                                                                  float* grad) {
+    __shared__ double red[kRedGroups][kRedCols];
+    const int il = threadIdx.x % kRedCols, j = threadIdx.x / kRedCols;
+    const int i = blockIdx.x * kRedCols + il;
+    double s = 0.0;
+    if (i < kPLayer) {
+        int p = j;
+        for (; p + 3 * kRedGroups < nparts; p += 4 * kRedGroups) {
+            const float v0 = part[(size_t)p * kPLayer + i];
+            const float v1 = part[(size_t)(p + kRedGroups) * kPLayer + i];
+            const float v2 = part[(size_t)(p + 2 * kRedGroups) * kPLayer + i];
+            const float v3 = part[(size_t)(p + 3 * kRedGroups) * kPLayer + i];
+            s += v0;
+            s += v1;
+            s += v2;
+            s += v3;
         }
-        for (; p < nparts; ++p) s += part[(size_t)p * kPLayer + i];
-        grad[i] = (float)s;
+        for (; p < nparts; p += kRedGroups) s += part[(size_t)p * kPLayer + i];
+    }
+    red[j][il] = s;
+    __syncthreads();
+    if (j == 0 && i < kPLayer) {
+        double t = 0.0;
+        for (int q = 0; q < kRedGroups; ++q) t += red[q][il];
+        grad[i] = (float)t;
     }
 }
 
@@ -1134,11 +1152,14 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
         b.ncpart = nc;
         b.G = (float*)fl->G.ptr;
         b.ops = (uint16_t*)fl->ops.ptr;
-        const int bb = (int)std::min<uint64_t>((uint64_t)info.cus * 2, (ntiles + 7) / 8);
+        // one workgroup per CU: the layer's weights fill most of the LDS, so a
+        // second workgroup per CU would only run after the first
+        const int bb = (int)std::min<uint64_t>((uint64_t)info.cus, (ntiles + 7) / 8);
         hipLaunchKernelGGL(flow_backward_kernel, dim3(bb), dim3(kBwdThreads), blds, s, flow_args(fl), b);
         hipLaunchKernelGGL(flow_wgrad_kernel, dim3(nparts), dim3(kWgThreads), 0, s, (const uint16_t*)fl->ops.ptr,
                            ntiles, wg_chunk, (float*)fl->wpart.ptr);
-        hipLaunchKernelGGL(flow_wgrad_reduce_kernel, dim3((kPLayer + 255) / 256), dim3(256), 0, s,
+        hipLaunchKernelGGL(flow_wgrad_reduce_kernel, dim3((kPLayer + kRedCols - 1) / kRedCols),
+                           dim3(kRedCols * kRedGroups), 0, s,
                            (const float*)fl->wpart.ptr, nparts,
                            fl->grad + (size_t)k * kPLayer);
         if ((rc = check_launch())) return rc;
