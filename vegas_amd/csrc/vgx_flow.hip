@@ -494,13 +494,20 @@ __device__ inline float unord_f(uint32_t o) {
     return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
 }
 
-__global__ void flow_lossw_max_kernel(const float* w, const float* fv, const float* logq, uint64_t n,
-                                      float pf, float pq, uint32_t* mx) {
+// One atomic per workgroup (the maximum is exact in any order).
+__global__ __launch_bounds__(256) void flow_lossw_max_kernel(const float* w, const float* fv, const float* logq,
+                                                             uint64_t n, float pf, float pq, uint32_t* mx) {
+    __shared__ float red[4];
     float m = -INFINITY;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         m = fmaxf(m, lossw_log(w, fv, logq, i, pf, pq));
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, kWave));
-    if ((threadIdx.x & 63) == 0 && m > -INFINITY) atomicMax(mx, ord_f(m));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        if (m > -INFINITY) atomicMax(mx, ord_f(m));
+    }
 }
 
 __global__ void flow_lossw_sum_kernel(const float* w, const float* fv, const float* logq, uint64_t n,
@@ -1133,7 +1140,7 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
     uint32_t* mx = (uint32_t*)fl->misc.ptr;
     hipLaunchKernelGGL(flow_zero_u32_kernel, dim3(1), dim3(64), 0, s, mx, 1);
     const float pf = loss == VGX_FLOW_LOSS_KL ? 1.f : 2.f, pq = loss == VGX_FLOW_LOSS_KL ? 0.f : 1.f;
-    hipLaunchKernelGGL(flow_lossw_max_kernel, dim3(nc), dim3(256), 0, s, w, f, (const float*)fl->logq.ptr,
+    hipLaunchKernelGGL(flow_lossw_max_kernel, dim3(256), dim3(256), 0, s, w, f, (const float*)fl->logq.ptr,
                        (uint64_t)n, pf, pq, mx);
     hipLaunchKernelGGL(flow_lossw_sum_kernel, dim3(nc), dim3(256), 0, s, w, f, (const float*)fl->logq.ptr,
                        (uint64_t)n, pf, pq, (const uint32_t*)mx, (float*)fl->c.ptr, (double*)fl->cpart.ptr);
