@@ -606,6 +606,15 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
         f32x4_t o[2];
         MlpKeep kp;
         mlp<true>(L, ain, lane, o, &kp);
+        // the forward operands of the weight gradients go out now, so h1 and
+        // h2 are not live through the backward GEMMs (only their masks are)
+        uint16_t* to = a.ops + (size_t)tile * kRows * 16;
+        put_rows(to, kRa, ain, lane);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            put_rows(to, kRh1 + 16 * q, kp.h1[q], lane);
+            put_rows(to, kRh2 + 16 * q, kp.h2[q], lane);
+        }
         // coupling backward (inverse direction)
         f32x4_t dout0, dout1, Gin;
 #pragma unroll
@@ -626,6 +635,8 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
             }
         }
         const bf16x4_t do0 = pack_bf(dout0), do1 = pack_bf(dout1);
+        put_rows(to, kRdo, do0, lane);
+        put_rows(to, kRdo + 16, do1, lane);
         // dh2 = W3^T dout (relu mask h2); dh1 = W2^T dh2 (mask h1); da = W1^T dh1
         const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
         bf16x4_t dh2[8];
@@ -647,6 +658,8 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
             for (int mt = 0; mt < 8; ++mt) acc[mt] = mfma16(frag_a(Wt + kTW2, kT2, mt, kt, lane), dh2[kt], acc[mt]);
             __builtin_amdgcn_sched_barrier(0);
         }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) put_rows(to, kRdh2 + 16 * q, dh2[q], lane);
         bf16x4_t dh1[8];
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
@@ -664,18 +677,9 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
             if (d < f.D && ((am >> d) & 1u)) Gin[i] += da[i];
         }
         if (live) *reinterpret_cast<f32x4_t*>(a.G + sl * 16 + 4 * g) = Gin;
-        // GEMM operands of the weight gradients
-        uint16_t* to = a.ops + (size_t)tile * kRows * 16;
-        put_rows(to, kRa, ain, lane);
+        // the backward operands of the weight gradients
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            put_rows(to, kRh1 + 16 * q, kp.h1[q], lane);
-            put_rows(to, kRh2 + 16 * q, kp.h2[q], lane);
-            put_rows(to, kRdh2 + 16 * q, dh2[q], lane);
-            put_rows(to, kRdh1 + 16 * q, dh1[q], lane);
-        }
-        put_rows(to, kRdo, do0, lane);
-        put_rows(to, kRdo + 16, do1, lane);
+        for (int q = 0; q < 8; ++q) put_rows(to, kRdh1 + 16 * q, dh1[q], lane);
     }
 }
 
