@@ -114,6 +114,77 @@ def test_parity_variable_nh(va, orc, neval, scale):
     np.testing.assert_allclose(integ.sigf, o.sigf, rtol=1e-11)
 
 
+def test_adapt_to_errors_parity(va, orc):
+    """adapt_to_errors=True (pyx:1688-1692, 2186-2193): uniform allocation,
+    the grid trained on each hcube's variance; first iteration and the
+    adapted grid against the oracle in FP64 parity mode."""
+    f, fo = gauss4(va, orc)
+    integ = va.Integrator([[0, 1]] * 4, neval=30000, seed=31, precision='f64', exact=True,
+                          adapt_to_errors=True)
+    r = integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * 4, neval=30000, seed=31, adapt_to_errors=True)
+    m, v, nev = o.run(fo, 1)
+    assert integ.last_neval == nev[0]
+    g = r.itn_results[0]
+    assert abs(g.mean - m[0]) <= 1e-12 * abs(m[0])
+    assert abs(g.var - v[0]) <= 1e-10 * v[0]
+    np.testing.assert_allclose(integ.map.grid, o.map.grid, rtol=1e-11, atol=1e-13)
+
+
+def test_no_adaptive_stratification_parity(va, orc):
+    """beta = 0 (pyx:1663-1666): every hcube gets the same count and sigf is
+    not used; the map still adapts.  First iteration and the adapted grid."""
+    f, fo = gauss4(va, orc)
+    integ = va.Integrator([[0, 1]] * 4, neval=30000, seed=37, precision='f64', exact=True, beta=0.0)
+    r = integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * 4, neval=30000, seed=37, beta=0.0)
+    m, v, nev = o.run(fo, 1)
+    assert integ.last_neval == nev[0]
+    g = r.itn_results[0]
+    assert abs(g.mean - m[0]) <= 1e-12 * abs(m[0])
+    assert abs(g.var - v[0]) <= 1e-10 * v[0]
+    np.testing.assert_allclose(integ.map.grid, o.map.grid, rtol=1e-11, atol=1e-13)
+
+
+def test_max_neval_hcube_clamp_parity(va, orc):
+    """max_neval_hcube caps n_h (pyx:1696-1698): heavy-tailed sigf against
+    a cap of 40 points per hcube."""
+    f, fo = gauss4(va, orc)
+    integ = va.Integrator([[0, 1]] * 4, neval=30000, seed=41, precision='f64', exact=True, alpha=0.0,
+                          max_neval_hcube=40)
+    rng = np.random.default_rng(9)
+    sigf = np.exp(3.0 * rng.standard_normal(integ.nhcube))
+    integ.set(sigf=sigf)
+    r = integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * 4, neval=30000, seed=41, alpha=0.0, max_neval_hcube=40)
+    o.sigf = sigf.copy()
+    o.sum_sigf = float(np.sum(sigf))
+    out, sf, nf = o.iterate(fo)
+    assert integ.last_neval == out.last_neval
+    assert integ.neval_hcube_range[1] == out.nh_max == 40
+    g = r.itn_results[0]
+    assert abs(g.mean - out.mean) <= 1e-12 * abs(out.mean)
+    assert abs(g.var - out.var) <= 1e-10 * out.var
+
+
+def test_adapt_false_iterations_and_unweighted_average(va, orc):
+    """adapt=False: nothing is trained, every iteration samples the same
+    grid and allocation with a fresh stream, and RAvg is the unweighted
+    average (pyx:2392-2410); all three iterations match the oracle's."""
+    f, fo = gauss4(va, orc)
+    integ = va.Integrator([[0, 1]] * 4, neval=20000, seed=43, precision='f64', exact=True, adapt=False)
+    r = integ(f, nitn=3)
+    o = orc.OracleIntegrator([(0, 1)] * 4, neval=20000, seed=43, adapt=False)
+    m, v, nev = o.run(fo, 3)
+    for k in range(3):
+        assert abs(r.itn_results[k].mean - m[k]) <= 1e-12 * abs(m[k])
+        assert abs(r.itn_results[k].var - v[k]) <= 1e-10 * v[k]
+    mean, sdev, chi2 = orc.ravg(m, v, weighted=False)
+    assert abs(r.mean - mean) <= 1e-12 * abs(mean)
+    assert abs(r.sdev - sdev) <= 1e-10 * sdev
+    assert abs(r.chi2 - chi2) <= 1e-8 * max(chi2, 1e-300)
+
+
 def test_training_and_adapt_parity(va, orc):
     # one iteration with adaptation: histogram -> adapt on device vs oracle
     f, fo = gauss4(va, orc)
