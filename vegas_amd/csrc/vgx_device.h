@@ -36,6 +36,26 @@ __host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
     return c;
 }
 
+// The same function for device code: each round's two three-input XORs
+// (product half ^ counter word ^ key) are one v_bitop3_b32 each (truth
+// table 0x96 = a ^ b ^ c), half the XOR instructions of the plain form.
+__device__ __forceinline__ U4 philox4x32_10_dev(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        U4 n;
+        n.x = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c.y, k0, 0x96);
+        n.y = (uint32_t)p1;
+        n.z = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c.w, k1, 0x96);
+        n.w = (uint32_t)p0;
+        c = n;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
 // 24-bit uniform in [0,1) from one word (exact in float and double).
 __host__ __device__ inline double u24(uint32_t w) { return (double)(w >> 8) * (1.0 / 16777216.0); }
 // 53-bit uniform in [0,1) from two words.

@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kFlowThreads) void flow_sample_kernel(FlowArgs f, F
 #pragma unroll 1
         for (int q = 0; q < kFlowTiles; ++q) {
             const uint64_t sidx = a.base + pb + (uint64_t)(wave * kFlowTiles + q) * 16 + (lane & 15);
-            const U4 w = philox4x32_10(U4{(uint32_t)sidx, (uint32_t)(sidx >> 32), a.iteration, (uint32_t)g},
+            const U4 w = philox4x32_10_dev(U4{(uint32_t)sidx, (uint32_t)(sidx >> 32), a.iteration, (uint32_t)g},
                                        a.seed_lo, a.seed_hi);
             const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
             float l = 0.f;

@@ -156,7 +156,7 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
         int k, kv;
         double dl;
         if (R32) {
-            if ((d & 3) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
+            if ((d & 3) == 0) w = philox4x32_10_dev(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
             const int e = d & 3;
             const uint32_t wd = e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w;
             const uint32_t ui = wd >> 8;
@@ -176,7 +176,7 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
                 kv = (yv > 0 && yv < 1) ? (k < ni ? k : ni - 1) : -1;
             }
         } else {
-            if ((d & 1) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
+            if ((d & 1) == 0) w = philox4x32_10_dev(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
             const double u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);
             const double yv = g.exact_div ? ((double)c + u) / (double)a.nstrat : ((double)c + u) * a.inv_s;
             const double tt = fmin(yv * (double)ni, (double)ni);
@@ -217,7 +217,7 @@ __device__ __forceinline__ uint64_t per_zero_axes(const TileGeom& g, const Packe
     U4 w{0, 0, 0, 0};
     for (int d = 0; d < nd_max(DIM); ++d) {
         if (DIM <= 0 && d >= dim) break;
-        if ((d & 3) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
+        if ((d & 3) == 0) w = philox4x32_10_dev(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
         const int e = d & 3;
         const uint32_t wd = e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w;
         if ((pc.get(d) | (wd >> 8)) == 0) m |= 1ull << d;
@@ -238,11 +238,11 @@ __device__ inline void point_y(const TileGeom& g, const AxisC* __restrict__ ax, 
         t = q;
         double u;
         if (R32) {
-            if ((d & 3) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
+            if ((d & 3) == 0) w = philox4x32_10_dev(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
             const int e = d & 3;
             u = u24(e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w);
         } else {
-            if ((d & 1) == 0) w = philox4x32_10(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
+            if ((d & 1) == 0) w = philox4x32_10_dev(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
             u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);
         }
         y[d] = g.exact_div ? ((double)c + u) / (double)a.nstrat : ((double)c + u) * a.inv_s;
