@@ -120,10 +120,15 @@ def cpu_baseline(wl, iters):
     f = wl["f"]
     g = orc.GaussMix(f.A, f.B, f.centers)
 
+    errors = []
+
     def one(seed, out):
-        I = orc.OracleIntegrator([(0, 1)] * wl["dim"], neval=wl["neval"], seed=seed)
-        _, _, nev = I.run(g, iters)
-        out.append(float(nev.sum()))
+        try:
+            I = orc.OracleIntegrator([(0, 1)] * wl["dim"], neval=wl["neval"], seed=seed)
+            _, _, nev = I.run(g, iters)
+            out.append(float(nev.sum()))
+        except Exception as e:  # reported below: a partial count would misstate the rate
+            errors.append(e)
 
     solo = []
     t0 = time.perf_counter()
@@ -142,6 +147,8 @@ def cpu_baseline(wl, iters):
     for t in ths:
         t.join()
     dtn = time.perf_counter() - t0
+    if errors or len(outs) != ncores:
+        raise RuntimeError("CPU baseline run failed: %r" % (errors[:1],))
     return dict(value=sum(outs) / dtn, unit="evals/s", cores=ncores, kind="port",
                 value_1core=solo[0] / dt1,
                 sample="%d iteration(s) of the 8-D double peak, neval 1e7, through the FP64 C oracle: "
