@@ -63,7 +63,8 @@ extern "C" int vgx_ctx_destroy(vgx_ctx* c) {
     (void)hipDeviceSynchronize();
     vgx::Scratch* all[] = {&c->hist_cnt, &c->hist_sorted, &c->hist_start, &c->nh, &c->off,
                            &c->tile_h, &c->blocksum, &c->tile_slots, &c->blk_part,
-                           &c->strad_part, &c->misc, &c->adapt_tmp, &c->errv, &c->cov_part};
+                           &c->strad_part, &c->misc, &c->adapt_tmp, &c->errv, &c->cov_part,
+                           &c->hist_part};
     for (auto* s : all) s->release();
     for (int i = 0; i < 2 * vgx_ctx::kMaxTimed; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);

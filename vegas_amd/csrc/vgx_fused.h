@@ -595,15 +595,21 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
     __builtin_amdgcn_wave_barrier();
 }
 
-// Adds the LDS u16 pair-packed counts into n_f and clears them.
-__device__ inline void flush_counts(uint32_t* Lc, int tot, double* n_f, bool clear) {
+
+// The u16 pair-packed LDS counts into a workgroup's own row of u32 counts
+// (written, or added to when add), one lane per pair.
+__device__ inline void counts_to_row(uint32_t* Lc, int tot, uint32_t* row, bool add, bool clear) {
     for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) {
         const uint32_t wv = Lc[i];
-        if (wv) {
-            if (wv & 0xffffu) unsafeAtomicAdd(&n_f[2 * i], (double)(wv & 0xffffu));
-            if ((wv >> 16) && 2 * i + 1 < tot) unsafeAtomicAdd(&n_f[2 * i + 1], (double)(wv >> 16));
-            if (clear) Lc[i] = 0u;
+        const uint32_t lo = wv & 0xffffu, hi = wv >> 16;
+        if (add) {
+            row[2 * i] += lo;
+            if (2 * i + 1 < tot) row[2 * i + 1] += hi;
+        } else {
+            row[2 * i] = lo;
+            if (2 * i + 1 < tot) row[2 * i + 1] = hi;
         }
+        if (clear) Lc[i] = 0u;
     }
 }
 
@@ -681,6 +687,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     const uint32_t off_end = io.off[hend];
 
     TileAcc acc{{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, 0.0, 0};
+    bool cnt_flushed = false;  // this workgroup's count row holds earlier rounds
     for (uint32_t r = 0; r < nrounds; ++r) {
         const uint32_t t = tb + r * nwave + wave;
         if (t < te) {
@@ -693,7 +700,8 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
         }
         if (g.hist_lds && g.train_points && ((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
             __syncthreads();
-            flush_counts(Lc, tot, io.n_f, true);
+            counts_to_row(Lc, tot, io.hpart_c + (size_t)blockIdx.x * tot, cnt_flushed, true);
+            cnt_flushed = true;
             __syncthreads();
         }
     }
@@ -701,11 +709,12 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     // ---- block epilogue ----
     __syncthreads();
     if (g.hist_lds && g.train_points) {
-        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-            const double s = Ls[i];
-            if (s != 0.0) unsafeAtomicAdd(&io.sum_f[i], s);
-        }
-        flush_counts(Lc, tot, io.n_f, false);
+        // this workgroup's row of the histogram: plain stores (the rows are
+        // summed by hist_reduce_kernel; global atomics from every workgroup
+        // onto the same bins would serialise at the end of the kernel)
+        double* srow = io.hpart_s + (size_t)blockIdx.x * tot;
+        for (int i = threadIdx.x; i < tot; i += blockDim.x) srow[i] = Ls[i];
+        counts_to_row(Lc, tot, io.hpart_c + (size_t)blockIdx.x * tot, cnt_flushed, false);
     }
     block_close(acc, Lred, io.blk_part + 6 * (size_t)blockIdx.x, sc);
 }

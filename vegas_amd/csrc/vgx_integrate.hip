@@ -491,6 +491,58 @@ __global__ __launch_bounds__(1024) void close_kernel(IterGeom g, Scalars* sc, vg
     adapt_parallel_axis(a, blockIdx.x, av, P, ared);
 }
 
+// The training histogram of one fused launch: its workgroups' rows summed
+// in row order (row group j takes rows j, j + kHrGroups, ...; the groups are
+// added in order) and added to sum_f / n_f.  Each thread issues its loads in
+// batches of kHrBatch rows so the device keeps many in flight.
+constexpr int kHrCols = 32, kHrGroups = 16, kHrBatch = 8;
+
+__global__ __launch_bounds__(kHrCols * kHrGroups) void hist_reduce_kernel(const double* ps, const uint32_t* pc,
+                                                                          int nrows, int tot, double* sum_f,
+                                                                          double* n_f, const Scalars* sc) {
+    __shared__ double rs[kHrGroups][kHrCols];
+    __shared__ uint64_t rn[kHrGroups][kHrCols];
+    if (sc->status || sc->overflow) return;  // the sampler did not run (or its rows are void)
+    const int il = threadIdx.x % kHrCols, j = threadIdx.x / kHrCols;
+    const int i = blockIdx.x * kHrCols + il;
+    double s = 0.0;
+    uint64_t n = 0;
+    if (i < tot) {
+        int w = j;
+        for (; w + (kHrBatch - 1) * kHrGroups < nrows; w += kHrBatch * kHrGroups) {
+            double v[kHrBatch];
+            uint32_t c[kHrBatch];
+#pragma unroll
+            for (int b = 0; b < kHrBatch; ++b) {
+                v[b] = ps[(size_t)(w + b * kHrGroups) * tot + i];
+                c[b] = pc[(size_t)(w + b * kHrGroups) * tot + i];
+            }
+#pragma unroll
+            for (int b = 0; b < kHrBatch; ++b) {
+                s += v[b];
+                n += c[b];
+            }
+        }
+        for (; w < nrows; w += kHrGroups) {
+            s += ps[(size_t)w * tot + i];
+            n += pc[(size_t)w * tot + i];
+        }
+    }
+    rs[j][il] = s;
+    rn[j][il] = n;
+    __syncthreads();
+    if (j == 0 && i < tot) {
+        double t = 0.0;
+        uint64_t m = 0;
+        for (int q = 0; q < kHrGroups; ++q) {
+            t += rs[q][il];
+            m += rn[q][il];
+        }
+        sum_f[i] += t;
+        n_f[i] += (double)m;
+    }
+}
+
 __global__ void sigf_reset_kernel(double* sigf, int64_t n, Scalars* sc) {
     if (!sc->reset_sigf || sc->status) return;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -670,6 +722,7 @@ struct Run {
     AxisC* axes;
     double* errv;
     double* sigf_part;
+    int hist_tot;  // dim * nb when the fused sampler keeps the histogram in LDS and trains
 };
 
 static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
@@ -715,6 +768,16 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     io.blk_part = (double*)ctx->blk_part.ptr;
     io.sum_f = st->sum_f;
     io.n_f = st->n_f;
+    io.hpart_s = nullptr;
+    io.hpart_c = nullptr;
+    R->hist_tot = 0;
+    if (g.hist_lds && g.train_points) {
+        R->hist_tot = c->dim * g.nb;
+        const size_t cells = (size_t)R->fused_blocks * (size_t)R->hist_tot;
+        if ((rc = ctx->hist_part.ensure(cells * (sizeof(double) + sizeof(uint32_t))))) return rc;
+        io.hpart_s = (double*)ctx->hist_part.ptr;
+        io.hpart_c = (uint32_t*)((double*)ctx->hist_part.ptr + cells);
+    }
     io.errv = R->errv;
     io.sc = R->sc;
     StradArgs& sa = R->sa;
@@ -769,6 +832,11 @@ static int accumulate_kernels(vgx_ctx* ctx, Run* R, const Fcn& fcn, hipStream_t 
                  : launch_fused(g, R->io, fcn, R->fused_blocks, R->lds, s);
     if (rc) return rc;
     if (rec) ctx->ntimed++;
+    if (R->io.hpart_s)
+        hipLaunchKernelGGL(hist_reduce_kernel, dim3((R->hist_tot + kHrCols - 1) / kHrCols),
+                           dim3(kHrCols * kHrGroups), 0, s, (const double*)R->io.hpart_s,
+                           (const uint32_t*)R->io.hpart_c, R->fused_blocks, R->hist_tot, R->io.sum_f,
+                           R->io.n_f, (const Scalars*)R->sc);
     hipLaunchKernelGGL(strad_kernel, dim3(R->strad_blocks), dim3(kStradThreads), 0, s, g, R->sa);
     if (reduce)
         hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(kReduceThreads), 0, s,

@@ -63,6 +63,7 @@ struct vgx_ctx {
     vgx::Scratch nh, off, tile_h, blocksum, tile_slots, blk_part, strad_part, misc, errv;
     // multi-component integrands: per-workgroup partials
     vgx::Scratch cov_part;
+    vgx::Scratch hist_part;  // per-workgroup training-histogram rows of the fused sampler
     // adapt
     vgx::Scratch adapt_tmp;
     static constexpr int kMaxTimed = 256;

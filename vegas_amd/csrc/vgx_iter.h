@@ -91,6 +91,8 @@ struct FusedPtrs {
     double* blk_part;   // [gridDim.x][6]
     double* sum_f;
     double* n_f;
+    double* hpart_s;    // LDS histogram: [gridDim.x][dim * nb] sums and counts of each
+    uint32_t* hpart_c;  //   workgroup, summed in row order by hist_reduce_kernel
     double* errv;       // adapt_to_errors: |variance| per hcube
     Scalars* sc;
 };
