@@ -494,28 +494,40 @@ __global__ __launch_bounds__(1024) void close_kernel(IterGeom g, Scalars* sc, vg
 // The training histogram of one fused launch: its workgroups' rows summed
 // in row order (row group j takes rows j, j + kHrGroups, ...; the groups are
 // added in order) and added to sum_f / n_f.  Each thread issues its loads in
-// batches of kHrBatch rows so the device keeps many in flight.
-constexpr int kHrCols = 32, kHrGroups = 16, kHrBatch = 8;
+// batches of kHrBatch rows so the device keeps many in flight.  Runs as
+// extra workgroups of the straddler launch (the two are independent).
+constexpr int kHrCols = 32, kHrGroups = kStradThreads / kHrCols, kHrBatch = 8;
 
-__global__ __launch_bounds__(kHrCols * kHrGroups) void hist_reduce_kernel(const double* ps, const uint32_t* pc,
-                                                                          int nrows, int tot, double* sum_f,
-                                                                          double* n_f, const Scalars* sc) {
+struct HistArgs {
+    const double* ps;     // [nrows][tot] sums
+    const uint32_t* pc;   // [nrows][tot] counts
+    int nrows, tot;
+    double* sum_f;
+    double* n_f;
+};
+
+__device__ __forceinline__ void hist_reduce_block(const HistArgs& h, const Scalars* sc, uint32_t blk) {
     __shared__ double rs[kHrGroups][kHrCols];
     __shared__ uint64_t rn[kHrGroups][kHrCols];
-    if (sc->status || sc->overflow) return;  // the sampler did not run (or its rows are void)
+    __shared__ int skip;
+    // the sampler did not run, or its rows are void; read once per workgroup
+    // (straddler workgroups of the same launch may raise the status meanwhile)
+    if (threadIdx.x == 0) skip = (sc->status || sc->overflow) ? 1 : 0;
+    __syncthreads();
+    if (skip) return;
     const int il = threadIdx.x % kHrCols, j = threadIdx.x / kHrCols;
-    const int i = blockIdx.x * kHrCols + il;
+    const int i = (int)blk * kHrCols + il;
     double s = 0.0;
     uint64_t n = 0;
-    if (i < tot) {
+    if (i < h.tot) {
         int w = j;
-        for (; w + (kHrBatch - 1) * kHrGroups < nrows; w += kHrBatch * kHrGroups) {
+        for (; w + (kHrBatch - 1) * kHrGroups < h.nrows; w += kHrBatch * kHrGroups) {
             double v[kHrBatch];
             uint32_t c[kHrBatch];
 #pragma unroll
             for (int b = 0; b < kHrBatch; ++b) {
-                v[b] = ps[(size_t)(w + b * kHrGroups) * tot + i];
-                c[b] = pc[(size_t)(w + b * kHrGroups) * tot + i];
+                v[b] = h.ps[(size_t)(w + b * kHrGroups) * h.tot + i];
+                c[b] = h.pc[(size_t)(w + b * kHrGroups) * h.tot + i];
             }
 #pragma unroll
             for (int b = 0; b < kHrBatch; ++b) {
@@ -523,24 +535,33 @@ __global__ __launch_bounds__(kHrCols * kHrGroups) void hist_reduce_kernel(const 
                 n += c[b];
             }
         }
-        for (; w < nrows; w += kHrGroups) {
-            s += ps[(size_t)w * tot + i];
-            n += pc[(size_t)w * tot + i];
+        for (; w < h.nrows; w += kHrGroups) {
+            s += h.ps[(size_t)w * h.tot + i];
+            n += h.pc[(size_t)w * h.tot + i];
         }
     }
     rs[j][il] = s;
     rn[j][il] = n;
     __syncthreads();
-    if (j == 0 && i < tot) {
+    if (j == 0 && i < h.tot) {
         double t = 0.0;
         uint64_t m = 0;
         for (int q = 0; q < kHrGroups; ++q) {
             t += rs[q][il];
             m += rn[q][il];
         }
-        sum_f[i] += t;
-        n_f[i] += (double)m;
+        h.sum_f[i] += t;
+        h.n_f[i] += (double)m;
     }
+}
+
+// Straddler workgroups 0..nstrad-1, histogram-row workgroups after them.
+__global__ __launch_bounds__(kStradThreads) void strad_hist_kernel(IterGeom g, StradArgs a, HistArgs h,
+                                                                    int nstrad) {
+    if ((int)blockIdx.x < nstrad)
+        strad_block(g, a, blockIdx.x);
+    else
+        hist_reduce_block(h, a.sc, blockIdx.x - (uint32_t)nstrad);
 }
 
 __global__ void sigf_reset_kernel(double* sigf, int64_t n, Scalars* sc) {
@@ -832,12 +853,14 @@ static int accumulate_kernels(vgx_ctx* ctx, Run* R, const Fcn& fcn, hipStream_t 
                  : launch_fused(g, R->io, fcn, R->fused_blocks, R->lds, s);
     if (rc) return rc;
     if (rec) ctx->ntimed++;
-    if (R->io.hpart_s)
-        hipLaunchKernelGGL(hist_reduce_kernel, dim3((R->hist_tot + kHrCols - 1) / kHrCols),
-                           dim3(kHrCols * kHrGroups), 0, s, (const double*)R->io.hpart_s,
-                           (const uint32_t*)R->io.hpart_c, R->fused_blocks, R->hist_tot, R->io.sum_f,
-                           R->io.n_f, (const Scalars*)R->sc);
-    hipLaunchKernelGGL(strad_kernel, dim3(R->strad_blocks), dim3(kStradThreads), 0, s, g, R->sa);
+    if (R->io.hpart_s) {
+        const HistArgs h{R->io.hpart_s, R->io.hpart_c, R->fused_blocks, R->hist_tot, R->io.sum_f, R->io.n_f};
+        const int nh = (R->hist_tot + kHrCols - 1) / kHrCols;
+        hipLaunchKernelGGL(strad_hist_kernel, dim3(R->strad_blocks + nh), dim3(kStradThreads), 0, s, g, R->sa, h,
+                           R->strad_blocks);
+    } else {
+        hipLaunchKernelGGL(strad_kernel, dim3(R->strad_blocks), dim3(kStradThreads), 0, s, g, R->sa);
+    }
     if (reduce)
         hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(kReduceThreads), 0, s,
                            (const double*)R->io.blk_part, R->fused_blocks, (const double*)R->sa.strad_part,

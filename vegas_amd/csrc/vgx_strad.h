@@ -62,14 +62,16 @@ __device__ inline void strad_close(const IterGeom& g, const StradArgs& a, int32_
     if (g.train_errors) a.errv[h] = fabs(varc);
 }
 
-static __global__ __launch_bounds__(kStradThreads) void strad_kernel(IterGeom g, StradArgs a) {
+// Workgroup `blk` of the straddler pass (kStradThreads lanes); strad_kernel
+// runs it as blockIdx.x, a combined launch may place it at another index.
+__device__ __forceinline__ void strad_block(const IterGeom& g, const StradArgs& a, uint32_t blk) {
     __shared__ double acc_l[(kStradThreads / 64) * 6];
     const Scalars* sc = a.sc;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     DD m{0, 0}, v{0, 0}, ss{0, 0};
     if (!(sc->status || sc->overflow)) {
         const uint32_t ntiles = sc->b_ntiles, p0 = sc->b_p0;
-        const uint32_t s = blockIdx.x * kStradThreads + threadIdx.x;
+        const uint32_t s = blk * kStradThreads + threadIdx.x;
         double c[3] = {0.0, 0.0, 0.0};
         int32_t h = -1;
         uint32_t tf = 0, tl = 0, nh = 0;
@@ -138,9 +140,13 @@ static __global__ __launch_bounds__(kStradThreads) void strad_kernel(IterGeom g,
             V = dd_sum(V, DD{acc_l[6 * k + 2], acc_l[6 * k + 3]});
             T = dd_sum(T, DD{acc_l[6 * k + 4], acc_l[6 * k + 5]});
         }
-        double* o = a.strad_part + 6 * (size_t)blockIdx.x;
+        double* o = a.strad_part + 6 * (size_t)blk;
         o[0] = M.hi; o[1] = M.lo; o[2] = V.hi; o[3] = V.lo; o[4] = T.hi; o[5] = T.lo;
     }
+}
+
+static __global__ __launch_bounds__(kStradThreads) void strad_kernel(IterGeom g, StradArgs a) {
+    strad_block(g, a, blockIdx.x);
 }
 
 }  // namespace vgx
