@@ -127,6 +127,14 @@ __global__ __launch_bounds__(kAllocThreads) void offsets_kernel(IterGeom g, cons
     if (sc->status) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     constexpr int nw = kAllocThreads / 64;
+    const int64_t base = (int64_t)blockIdx.x * kAllocPerBlock;
+    // this workgroup's n_h into LDS first: those loads are in flight while
+    // the totals below are read and reduced (the barrier after the wave
+    // reductions orders them)
+    for (int i = threadIdx.x; i < kAllocPerBlock; i += kAllocThreads) {
+        const int64_t h = base + i;
+        ln[lds_pad(i)] = h < g.nhcube ? (uint32_t)nh[h] : 0u;
+    }
     // totals before this workgroup and over all workgroups
     uint64_t pre = 0, tot = 0;
     int32_t mn = g.nmin, mx = g.nmin;
@@ -175,12 +183,6 @@ __global__ __launch_bounds__(kAllocThreads) void offsets_kernel(IterGeom g, cons
         sc->b_ntiles = (sc->total + kT - 1) / kT;
     }
     if (!ok) return;
-    const int64_t base = (int64_t)blockIdx.x * kAllocPerBlock;
-    for (int i = threadIdx.x; i < kAllocPerBlock; i += kAllocThreads) {
-        const int64_t h = base + i;
-        ln[lds_pad(i)] = h < g.nhcube ? (uint32_t)nh[h] : 0u;
-    }
-    __syncthreads();
     uint32_t loc[kAllocItems];
     uint32_t run = 0;
     for (int i = 0; i < kAllocItems; ++i) {
