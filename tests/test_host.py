@@ -192,3 +192,42 @@ int main() {
         pytest.skip("hipcc host build unavailable")
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout
+
+
+def test_ravg_reference_docstring_example():
+    """src/vegas/__init__.py:1237-1262: the ten iteration results printed in
+    ravg's docstring, weighted (1.0018(22), chi2/dof 1.21, Q 0.28 after all
+    ten; the partial lines are too sensitive to the rounded inputs to pin
+    anything) and the unweighted average of the last
+    five (1.0018(26), chi2/dof 0.31, Q 0.87).  The inputs are the printed,
+    rounded values: each mean is off by up to half its last digit (up to
+    5e-4 for the first three), which moves the weighted mean by up to
+    ~5.5e-5 on top of the printed answer's own rounding of 5e-5, hence
+    1.1e-4 on means, and 0.02 on chi2/dof and Q."""
+    import vegas_amd as va
+    res = [(1.013, .019), (0.997, .014), (1.021, .012), (0.9785, .0097), (1.0067, .0085),
+           (0.9996, .0075), (1.0020, .0061), (1.0051, .0052), (1.0046, .0047), (0.9976, .0047)]
+    r = va.ravg([va.GVar(m, s) for m, s in res])
+    assert abs(r.mean - 1.0018) < 1.1e-4 and abs(r.sdev - 0.0022) < 6e-5
+    assert abs(r.chi2 / r.dof - 1.21) < 0.02 and abs(r.Q - 0.28) < 0.02
+    ur = va.ravg(r.itn_results[5:], weighted=False)
+    assert abs(ur.mean - 1.0018) < 1.1e-4 and abs(ur.sdev - 0.0026) < 6e-5
+    assert abs(ur.chi2 / ur.dof - 0.31) < 0.02 and abs(ur.Q - 0.87) < 0.02
+    again = va.ravg(r)  # an RAvg is averaged again with its own setting
+    assert again.mean == r.mean and again.sdev == r.sdev
+
+
+def test_ravg_arrays_and_dicts():
+    import numpy as np
+    import vegas_amd as va
+    g = [va.GVarArray([1.0 + 0.01 * k, 2.0 - 0.01 * k], np.diag([0.01, 0.04]), (2,)) for k in range(3)]
+    ra = va.ravg(g)
+    np.testing.assert_allclose(ra.mean, [1.01, 1.99], rtol=1e-12)
+    np.testing.assert_allclose(ra.cov, np.diag([0.01, 0.04]) / 3, rtol=1e-12)
+    d = [dict(a=va.GVar(1.0 + 0.01 * k, 0.1), b=va.GVarArray([2.0, 3.0], np.diag([0.04, 0.09]), (2,)))
+         for k in range(3)]
+    rd = va.ravg(d)
+    assert abs(rd['a'].mean - 1.01) < 1e-12 and abs(rd['a'].sdev - 0.1 / np.sqrt(3)) < 1e-12
+    np.testing.assert_allclose(rd['b'].mean, [2.0, 3.0], rtol=1e-12)
+    with pytest.raises(ValueError):
+        va.ravg([])

@@ -4,7 +4,7 @@ The call surface is gplepage/vegas' (``Integrator``, ``AdaptiveMap``,
 ``RAvg``, ``lbatchintegrand``/``rbatchintegrand``); the hot path runs as
 hand-written HIP kernels for gfx950 in ``libvgx.so`` (C-ABI in include/vgx.h).
 """
-from ._ravg import GVar, GVarArray, RAvg, RAvgArray, RAvgDict, gammaQ
+from ._ravg import GVar, GVarArray, RAvg, RAvgArray, RAvgDict, gammaQ, ravg
 from .batchintegrand import (BatchIntegrand, LBatchIntegrand, MPIintegrand, RBatchIntegrand,
                              VecIntegrand, batchintegrand, gpu_lbatchintegrand,
                              gpu_rbatchintegrand, lbatchintegrand, rbatchintegrand, vecintegrand)

@@ -544,3 +544,66 @@ def _rebuild_ravg_dict(layout, rarray):
     r.rarray = rarray
     r._refresh()
     return r
+
+
+def ravg(reslist, weighted=None, rescale=None):
+    """A running average of a list of vegas results (src/vegas/__init__.py:
+    1220-1312): GVars give an RAvg, GVarArrays an RAvgArray of their shape,
+    dictionaries of GVars / GVarArrays an RAvgDict (covariance block-diagonal
+    across keys, as independent entries).  ``reslist`` may also be an RAvg,
+    RAvgArray or RAvgDict, whose iteration results are averaged again.
+    weighted defaults to the input's own setting (True for a plain list);
+    rescale applies to array averages (True: by the last result)."""
+    if isinstance(reslist, RAvgDict):
+        if weighted is None:
+            weighted = reslist.rarray.weighted
+        out = RAvgDict(reslist.layout, weighted=weighted)
+        out.rarray = RAvgArray(reslist.rarray.shape, weighted=weighted, itn_results=reslist.rarray.itn_results,
+                               rescale=reslist.rarray.rescale if rescale is None else rescale)
+        out._refresh()
+        return out
+    if isinstance(reslist, (RAvg, RAvgArray)):
+        if weighted is None:
+            weighted = reslist.weighted
+        if rescale is None:
+            rescale = getattr(reslist, 'rescale', True)
+        reslist = reslist.itn_results
+    try:
+        if len(reslist) < 1:
+            raise ValueError('reslist empty')
+    except TypeError:
+        raise ValueError('improper type for reslist')
+    if weighted is None:
+        weighted = True
+    if rescale is None:
+        rescale = True
+    first = reslist[0]
+    if hasattr(first, 'keys'):
+        layout, a = [], 0
+        for k, v in first.items():
+            shape = () if isinstance(v, GVar) else tuple(np.shape(np.asarray(v.mean)))
+            m = int(np.prod(shape)) if shape else 1
+            layout.append((k, shape, a, a + m))
+            a += m
+        out = RAvgDict(layout, weighted=weighted)
+        out.rarray = RAvgArray((a,), weighted=weighted, rescale=rescale)
+        for r in reslist:
+            mean = np.zeros(a)
+            cov = np.zeros((a, a))
+            for k, shape, lo, hi in layout:
+                v = r[k]
+                if isinstance(v, GVar):
+                    mean[lo] = v.mean
+                    cov[lo, lo] = v.sdev ** 2
+                else:
+                    mean[lo:hi] = np.asarray(v.mean, float).reshape(-1)
+                    cov[lo:hi, lo:hi] = np.asarray(v.cov, float).reshape(hi - lo, hi - lo)
+            out.rarray.add(GVarArray(mean, cov, (a,)))
+        out._refresh()
+        return out
+    if isinstance(first, GVar) or np.shape(getattr(first, 'mean', first)) == ():
+        return RAvg(weighted=weighted, itn_results=[r if isinstance(r, GVar) else GVar(*r) for r in reslist])
+    if isinstance(first, GVarArray):
+        return RAvgArray(first.shape, weighted=weighted, itn_results=list(reslist), rescale=rescale)
+    raise ValueError('reslist[i] not GVar, GVarArray, or dictionary')
+
