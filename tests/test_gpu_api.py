@@ -1,0 +1,99 @@
+"""Integrator API behaviour on the GPU, following the reference's own tests:
+uses_jac (tests/test_vegas.py:1298-1340), set() and its returned old values
+(tests/test_vegas.py:703-735), and the rtol/atol stop and the reporter
+analyzer (pyx:2232-2269, 2389-2390)."""
+import contextlib
+import io
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def va():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import vegas_amd
+    return vegas_amd
+
+
+def test_uses_jac(va):
+    """f = 1 / prod(jac) integrates to 1 exactly: each point's weight times
+    f is dv_y / n_h (tests/test_vegas.py:1298-1340)."""
+    integ = va.Integrator(2 * [[0, 2.]], seed=3)
+
+    def check(f, mode):
+        r = integ(f, nitn=1, neval=10, uses_jac=True)
+        ans = r[0, 0] if mode == 'array' else (r['a'] if mode == 'dict' else r)
+        assert abs(ans.mean - 1.0) < 1e-7
+
+    def f(x, jac):
+        return 1. / np.prod(jac)
+    check(f, 'scalar')
+
+    def f(x, jac):
+        return [[1. / np.prod(jac)]]
+    check(f, 'array')
+
+    def f(x, jac):
+        return dict(a=1. / np.prod(jac))
+    check(f, 'dict')
+
+    @va.rbatchintegrand
+    def f(x, jac):
+        return 1. / np.prod(jac, axis=0)
+    check(f, 'scalar')
+
+    @va.rbatchintegrand
+    def f(x, jac):
+        return [[1. / np.prod(jac, axis=0)]]
+    check(f, 'array')
+
+    @va.lbatchintegrand
+    def f(x, jac):
+        return dict(a=1. / np.prod(jac, axis=1))
+    check(f, 'dict')
+
+
+def test_set_returns_old_values(va):
+    """tests/test_vegas.py:703-735: set() installs every keyword and returns
+    the previous values, which restore the old state."""
+    new = dict(map=va.AdaptiveMap([[1, 2], [0, 1]]), neval=100, maxinc_axis=100, min_neval_batch=10,
+               max_neval_hcube=1e1, nitn=100, alpha=0.35, beta=0.25, adapt_to_errors=True, rtol=0.1,
+               atol=0.2, analyzer=va.reporter(5))
+    I = va.Integrator([[1, 2]])
+    old = I.set(**new)
+    for k, v in new.items():
+        if k == 'map':
+            np.testing.assert_allclose([[I.map.grid[0, 0], I.map.grid[0, I.map.ninc[0]]],
+                                        [I.map.grid[1, 0], I.map.grid[1, I.map.ninc[1]]]], v.grid)
+        else:
+            assert getattr(I, k) == v, k
+    I.set(old)
+    assert I.neval == 1000 and I.nitn == 10 and I.alpha == 0.5 and I.map.dim == 1
+
+
+def test_rtol_and_atol_stop_early(va):
+    """Iterations stop once sdev < atol + rtol |mean| (pyx:2389-2390)."""
+    f = va.GaussianMixture([1.0], [20.0], [[0.5, 0.5, 0.5]])
+    integ = va.Integrator([[0, 1]] * 3, neval=20000, seed=5)
+    r = integ(f, nitn=200, rtol=2e-3)
+    assert r.nitn < 200 and r.sdev < 2e-3 * abs(r.mean)
+    r0 = integ(f, nitn=1)
+    # keywords given to a call persist (pyx:2041-2042, as set()): rtol back to 0
+    r = integ(f, nitn=200, rtol=0, atol=r0.sdev / 2)
+    assert r.nitn < 200 and r.sdev < r0.sdev / 2
+
+
+def test_reporter_prints_each_iteration(va):
+    f = va.GaussianMixture([1.0], [20.0], [[0.5, 0.5]])
+    integ = va.Integrator([[0, 1]] * 2, neval=5000, seed=6)
+    out = io.StringIO()
+    with contextlib.redirect_stdout(out):
+        r = integ(f, nitn=4, analyzer=va.reporter())
+    text = out.getvalue()
+    assert r.nitn == 4
+    assert text.count('itn') >= 4, text
