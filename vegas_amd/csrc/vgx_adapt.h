@@ -51,23 +51,24 @@ __device__ inline void adapt_axis_block(double* av, double* P, double* red, cons
         }
         __syncthreads();
     }
-    // inclusive prefix sums of av (chunked Hillis-Steele; fixed order)
+    // inclusive prefix sums of av, chunk by chunk of T: a DPP scan within
+    // each wave, the wave totals through LDS (red[0..T/64)), then each wave's
+    // offset = carry + the totals of the waves before it (fixed order; two
+    // barriers per chunk)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = T >> 6;
     double carry = 0.0;
     for (int base = 0; base < old; base += T) {
         const int i = base + threadIdx.x;
-        double v = i < old ? av[i] : 0.0;
-        for (int o = 1; o < T; o <<= 1) {
-            double add = 0.0;
-            if (i < old) P[i] = v;
-            __syncthreads();
-            if ((int)threadIdx.x >= o && i < old) add = P[i - o];
-            __syncthreads();
-            v += add;
-        }
-        if (i < old) P[i] = carry + v;
+        double v = wave_segscan(i < old ? av[i] : 0.0, lane == 0);
+        if (lane == 63) red[wv] = v;
         __syncthreads();
-        const int last = (base + T < old ? base + T : old) - 1;
-        carry = P[last];
+        double off = carry, tot = carry;
+        for (int k = 0; k < nw; ++k) {
+            if (k < wv) off += red[k];
+            tot += red[k];
+        }
+        if (i < old) P[i] = off + v;
+        carry = tot;
         __syncthreads();
     }
     const double total = P[old - 1];
