@@ -97,3 +97,36 @@ def test_reporter_prints_each_iteration(va):
     text = out.getvalue()
     assert r.nitn == 4
     assert text.count('itn') >= 4, text
+
+
+def test_settings_text(va):
+    """tests/test_vegas.py:597-622 (array limits) and the map's grid listing,
+    tests/test_vegas.py:121-129."""
+    I = va.Integrator([[0., 1.], [-1., 1.]], neval=254, nitn=123, neval_frac=0.75)
+    outstr = [
+        "Integrator Settings:",
+        "    {neval} (approx) integrand evaluations in each of 123 iterations",
+        "    number of: strata/axis = [{nstrat0} {nstrat1}]",
+        "               increments/axis = [{ninc0} {ninc1}]",
+        "               h-cubes = {nhcube}  processors = 1",
+        "               evaluations/batch >= {min_neval_batch:.2g}",
+        "               {min_neval_hcube} <= evaluations/h-cube <= {max_neval_hcube:.2g}",
+        "    minimize_mem = False  adapt_to_errors = False  adapt = True",
+        "    accuracy: relative = 0  absolute = 0",
+        "    damping: alpha = {alpha}  beta= {beta}",
+        "",
+        "    axis    integration limits",
+        "    --------------------------",
+        "       0            (0.0, 1.0)",
+        "       1           (-1.0, 1.0)\n",
+    ]
+    outstr = ('\n'.join(outstr)).format(
+        neval=I.neval, nstrat0=I.nstrat[0], nstrat1=I.nstrat[1], ninc0=I.map.ninc[0], ninc1=I.map.ninc[1],
+        nhcube=I.nhcube, min_neval_hcube=I.min_neval_hcube, alpha=I.alpha, beta=I.beta,
+        min_neval_batch=I.min_neval_batch, max_neval_hcube=float(I.max_neval_hcube))
+    assert I.settings() == outstr
+    m = va.AdaptiveMap(grid=[[0, 1, 3], [-2, 0, 6]], ninc=4)
+    out = "    grid[ 0] = [ 0.   0.5  1.   2.   3. ]\n" + "    grid[ 1] = [-2. -1.  0.  3.  6.]\n"
+    assert m.settings(5).replace(' ', '') == out.replace(' ', '')
+    out = "    grid[ 0] = [ 0.5  2. ]\n" + "    grid[ 1] = [-1.  3.]\n"
+    assert m.settings(2).replace(' ', '') == out.replace(' ', '')
