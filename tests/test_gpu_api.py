@@ -130,3 +130,110 @@ def test_settings_text(va):
     assert m.settings(5).replace(' ', '') == out.replace(' ', '')
     out = "    grid[ 0] = [ 0.5  2. ]\n" + "    grid[ 1] = [-1.  3.]\n"
     assert m.settings(2).replace(' ', '') == out.replace(' ', '')
+
+
+def _settings_head(I):
+    return '\n'.join([
+        "Integrator Settings:",
+        "    {neval} (approx) integrand evaluations in each of 123 iterations",
+        "    number of: strata/axis = [{nstrat0} {nstrat1}]",
+        "               increments/axis = [{ninc0} {ninc1}]",
+        "               h-cubes = {nhcube}  processors = 1",
+        "               evaluations/batch >= {min_neval_batch:.2g}",
+        "               {min_neval_hcube} <= evaluations/h-cube <= {max_neval_hcube:.2g}",
+        "    minimize_mem = False  adapt_to_errors = False  adapt = True",
+        "    accuracy: relative = 0  absolute = 0",
+        "    damping: alpha = {alpha}  beta= {beta}",
+        "",
+    ]).format(neval=I.neval, nstrat0=I.nstrat[0], nstrat1=I.nstrat[1], ninc0=I.map.ninc[0],
+              ninc1=I.map.ninc[1], nhcube=I.nhcube, min_neval_hcube=I.min_neval_hcube, alpha=I.alpha,
+              beta=I.beta, min_neval_batch=I.min_neval_batch, max_neval_hcube=float(I.max_neval_hcube))
+
+
+def test_settings_text_structured_regions(va):
+    """tests/test_vegas.py:623-673: a dictionary region and a region given as
+    an array of limits with more than one index get a key/index column."""
+    I = va.Integrator(dict(x=[[0., 1.]], y=[-1., 1.]), neval=254, nitn=123, neval_frac=0.75)
+    tail = ["    key/index    axis    integration limits",
+            "    ---------------------------------------",
+            "          x 0       0            (0.0, 1.0)",
+            "            y       1           (-1.0, 1.0)\n"]
+    assert I.settings() == _settings_head(I) + '\n' + '\n'.join(tail)
+    I = va.Integrator([[[0., 1.]], [[-1., 1.]]], neval=254, nitn=123, neval_frac=0.75)
+    tail = ["    key/index    axis    integration limits",
+            "    ---------------------------------------",
+            "          0,0       0            (0.0, 1.0)",
+            "          1,0       1           (-1.0, 1.0)\n"]
+    assert I.settings() == _settings_head(I) + '\n' + '\n'.join(tail)
+    # beyond 20 axes the table is printed in two columns side by side
+    text = va.Integrator([[0, 1]] * 25).settings()
+    rows = text.split('\n\n')[-1].rstrip('\n').split('\n')
+    assert len(rows) == 2 + 13 and rows[2].split()[0] == '0' and rows[2].split()[3] == '13'
+    assert rows[-1].split()[0] == '12' and len(rows[-1]) < len(rows[2])
+
+
+def test_sample_structured_regions(va):
+    """tests/test_vegas.py:1102-1150: integrands and sample() see x shaped
+    as the region, a dictionary or an array, in rbatch and lbatch form."""
+    neval, nitn = 1000, 5
+    re = 1 * (8 - 1) * (81 - 16)
+    region = dict(s=(0., 1.), v=[[(1., 2.)], [(2., 3.)]])
+
+    @va.rbatchintegrand
+    def f(x):
+        return 2 * x['s'] * 3 * x['v'][0, 0] ** 2 * 4 * x['v'][1, 0] ** 3
+    itg = va.Integrator(region, neval=neval, nitn=nitn, seed=1)
+    rv = itg(f)
+    assert abs(rv.mean - re) < 5 * rv.sdev
+    w, x = itg.sample(nbatch=nitn * itg.last_neval, mode='rbatch')
+    assert len(w) == nitn * itg.last_neval and x['v'].shape == (2, 1, len(w))
+    assert abs(np.sum(w * f(x)) - re) < 5 * rv.sdev
+
+    @va.lbatchintegrand
+    def f(x):
+        return 2 * x['s'] * 3 * x['v'][:, 0, 0] ** 2 * 4 * x['v'][:, 1, 0] ** 3
+    itg = va.Integrator(region, neval=neval, nitn=nitn, seed=2)
+    rv = itg(f)
+    w, x = itg.sample(nbatch=nitn * itg.last_neval, mode='lbatch')
+    assert len(w) == nitn * itg.last_neval and x['s'].shape == (len(w),)
+    assert abs(np.sum(w * f(x)) - re) < 5 * rv.sdev
+
+    @va.rbatchintegrand
+    def f(x):
+        return 2 * x[0, 0] * 3 * x[0, 1] ** 2 * 4 * x[0, 2] ** 3
+    itg = va.Integrator([[(0., 1.), (1., 2.), (2., 3.)]], neval=neval, nitn=nitn, seed=3)
+    rv = itg(f)
+    w, x = itg.sample(nbatch=nitn * itg.last_neval, mode='rbatch')
+    assert x.shape == (1, 3, len(w))
+    assert abs(np.sum(w * f(x)) - re) < 5 * rv.sdev
+
+    @va.lbatchintegrand
+    def f(x):
+        return 2 * x[:, 0, 0] * 3 * x[:, 0, 1] ** 2 * 4 * x[:, 0, 2] ** 3
+    itg = va.Integrator([[(0., 1.), (1., 2.), (2., 3.)]], neval=neval, nitn=nitn, seed=4)
+    rv = itg(f)
+    w, x = itg.sample(nbatch=nitn * itg.last_neval, mode='lbatch')
+    assert x.shape == (len(w), 1, 3)
+    assert abs(np.sum(w * f(x)) - re) < 5 * rv.sdev
+
+
+def test_structured_region_point_integrand_and_jac(va):
+    """A point-by-point integrand on a dictionary region gets x and jac as
+    dictionaries; f = 1 / prod(jac) integrates to 1 exactly (the uses_jac
+    identity of tests/test_vegas.py:1298-1340); the layout survives
+    pickling."""
+    import pickle
+    itg = va.Integrator(dict(a=[[0, 2.], [0, 1.]], b=(1., 3.)), seed=7)
+
+    def f(x, jac):
+        assert set(x) == {'a', 'b'} and np.shape(x['a']) == (2,) and np.shape(jac['b']) == ()
+        return 1. / (np.prod(jac['a']) * jac['b'])
+    r = itg(f, nitn=1, neval=50, uses_jac=True)
+    assert abs(r.mean - 1.0) < 1e-7
+
+    def g(x):
+        return x['a'][0] * x['a'][1] * x['b']
+    r = itg(g, nitn=6, neval=2000, uses_jac=False)
+    assert abs(r.mean - 2 * 0.5 * 4) < 5 * r.sdev
+    itg2 = pickle.loads(pickle.dumps(itg))
+    assert itg2.settings() == itg.settings() and 'key/index' in itg2.settings()

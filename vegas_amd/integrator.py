@@ -34,15 +34,42 @@ from .integrands import DeviceIntegrand
 HUGE = 10 ** (308 - 50)
 
 
+def _region_layout(region):
+    """Structure of the integration variables (the reference's xsample,
+    pyx:1209-1253): None for a plain list of limits or node lists; for a
+    dictionary, [(key, shape, start, stop)] over the flattened axes in key
+    order; for an array of limits of more than one dimension, [(None,
+    shape, 0, n)]."""
+    if isinstance(region, (AdaptiveMap, Integrator)):
+        return getattr(region, '_xlayout', None)
+    if hasattr(region, 'keys'):
+        layout, a = [], 0
+        for k in region:
+            shape = tuple(np.shape(np.asarray(region[k], float)))[:-1]
+            n = int(np.prod(shape)) if shape else 1
+            layout.append((k, shape, a, a + n))
+            a += n
+        return layout
+    try:
+        arr = np.asarray(region, float)
+    except ValueError:  # node lists of different lengths
+        return None
+    if arr.ndim > 2 and arr.shape[-1] == 2:
+        shape = arr.shape[:-1]
+        return [(None, shape, 0, int(np.prod(shape)))]
+    return None
+
+
 def _region_map(region):
-    """AdaptiveMap for an integration region given as limits or node lists
-    (pyx:1209-1253; dictionary regions are not supported)."""
+    """AdaptiveMap for an integration region given as limits or node lists,
+    a dictionary of them, or an array of limits (pyx:1209-1253)."""
     if isinstance(region, AdaptiveMap):
         return AdaptiveMap(region)
     if isinstance(region, Integrator):
         return AdaptiveMap(region.map)
     if hasattr(region, 'keys'):
-        raise NotImplementedError('dictionary integration regions are not supported')
+        lims = [np.asarray(region[k], float).reshape(-1, 2) for k in region]
+        return AdaptiveMap(np.concatenate(lims, axis=0).tolist())
     arr = np.array(region, dtype=object)
     if np.shape(arr.flat[0]) == ():
         grid = np.array(region, float).reshape(-1, 2).tolist()
@@ -95,6 +122,7 @@ class Integrator:
         self._rng_iter = 0
         self._sigf = None
         self.sum_sigf = HUGE
+        self._xlayout = _region_layout(map)
         if isinstance(map, Integrator):
             self.map = AdaptiveMap(map.map)
             args = {k: getattr(map, k) for k in Integrator.defaults if k != 'map'}
@@ -144,13 +172,16 @@ class Integrator:
         odict['_rng_iter'] = self._rng_iter
         odict['_sum_sigf'] = self.sum_sigf
         odict['_grid'] = self.map.extract_grid()
+        odict['_xlayout'] = self._xlayout
         return (Integrator, (self.map,), odict)
 
     def __setstate__(self, odict):
         rng_iter = odict.pop('_rng_iter', 0)
         sum_sigf = odict.pop('_sum_sigf', None)
         grid = odict.pop('_grid', None)
+        xlayout = odict.pop('_xlayout', None)
         self.set(odict)
+        self._xlayout = xlayout
         self._rng_iter = rng_iter
         if sum_sigf is not None and self._sigf is not None:
             self.sum_sigf = sum_sigf
@@ -170,6 +201,7 @@ class Integrator:
             if k == 'map':
                 old_val['map'] = self.map
                 self.map = _region_map(kargs['map'])
+                self._xlayout = _region_layout(kargs['map'])
             elif k == 'nstrat':
                 if kargs['nstrat'] is None:
                     continue
@@ -265,17 +297,53 @@ class Integrator:
             ans += "    damping: alpha = %g  beta= %g\n\n" % (self.alpha, 0.)
         else:
             ans += "    damping: alpha = %g  beta= %g\n\n" % (self.alpha, self.beta)
-        limits = ['({:.5}, {:.5})'.format(*lim) for lim in self.map.region()]
-        entries = [(str(a), lim) for a, lim in enumerate(limits)]
-        w1 = max(max(len(e[0]) for e in entries), len('axis'))
-        w2 = max(max(len(e[1]) for e in entries), len('integration limits'))
-        lines = ['{0:>{1}}    {2:>{3}}'.format('axis', w1, 'integration limits', w2)]
-        lines.append(len(lines[0]) * '-')
-        lines += ['{0:>{1}}    {2:>{3}}'.format(e[0], w1, e[1], w2) for e in entries]
-        ans += 4 * ' ' + ('\n' + 4 * ' ').join(lines) + '\n'
+        ans += self._limits_table()
         if ngrid > 0:
             ans += '\n' + self.map.settings(ngrid=ngrid)
         return ans
+
+    def _limits_table(self):
+        """The integration-limits table of settings() (pyx:1508-1586): a
+        key/index column for dictionary or multi-dimensional regions (the
+        key printed on a variable's first index only), two columns side by
+        side beyond 20 axes."""
+        limits = ['({:.5}, {:.5})'.format(*lim) for lim in self.map.region()]
+
+        def idx_str(idx):
+            return ','.join(str(i) for i in idx)
+
+        if self._xlayout is None:
+            keys = None
+        else:
+            keys = []
+            for k, shape, a, b in self._xlayout:
+                if k is None:
+                    keys += [idx_str(idx) for idx in np.ndindex(shape)]
+                elif shape == ():
+                    keys.append(str(k))
+                else:
+                    for j, idx in enumerate(np.ndindex(shape)):
+                        keys.append((str(k) + ' ' if j == 0 else '') + idx_str(idx))
+        entries = [(None if keys is None else keys[a], str(a), lim) for a, lim in enumerate(limits)]
+        ncol = 1 if self.map.dim <= 20 else 2
+        nl = -(-len(entries) // ncol)
+        cols = [entries[c * nl:(c + 1) * nl] for c in range(ncol)]
+        w0 = None if keys is None else max(max(len(e[0]) for e in entries), len('key/index'))
+        w1 = max(max(len(e[1]) for e in entries), len('axis'))
+        w2 = max(max(len(e[2]) for e in entries), len('integration limits'))
+
+        def fmt(e0, e1, e2):
+            s = '' if w0 is None else '{0:>{1}}    '.format(e0, w0)
+            return s + '{0:>{1}}    {2:>{3}}'.format(e1, w1, e2, w2)
+
+        tables = []
+        for col in cols:
+            head = fmt('key/index', 'axis', 'integration limits')
+            tables.append([head, len(head) * '-'] + [fmt(*e) for e in col])
+        lines = []
+        for i in range(len(tables[0])):
+            lines.append('  '.join(t[i] for t in tables if i < len(t)))
+        return 4 * ' ' + ('\n' + 4 * ' ').join(lines) + '\n'
 
     # --------------------------------------------------------- device plumbing
     def _precision(self, device_fcn):
@@ -435,6 +503,22 @@ class Integrator:
         on_gpu = bool(getattr(fcn, 'vgx_torch', False))
         uses_jac = self.uses_jac
 
+        xl = self._xlayout
+
+        def shaped(v, mode):
+            # flat axes -> the region's structure: a dict per key or one array
+            # (batch index first, last, or none for a single point)
+            if xl is None or v is None:
+                return v
+            if mode == 'point':
+                out = {k: (v[a] if shape == () else v[a:b].reshape(shape)) for k, shape, a, b in xl}
+            elif mode == 'lbatch':
+                out = {k: (v[:, a] if shape == () else v[:, a:b].reshape((v.shape[0],) + shape))
+                       for k, shape, a, b in xl}
+            else:
+                out = {k: (v[a] if shape == () else v[a:b].reshape(shape + (v.shape[-1],))) for k, shape, a, b in xl}
+            return out[None] if xl[0][0] is None else out
+
         def call(x, jac):
             if kind == 'rbatch':
                 xa, ja = x.T, (None if jac is None else jac.T)
@@ -442,15 +526,19 @@ class Integrator:
                 xa, ja = x, jac
             else:
                 xa, ja = x, jac
+            bmode = 'rbatch' if kind == 'rbatch' else 'lbatch'
             if on_gpu:
-                fx = fcn(xa) if ja is None else fcn(xa, jac=ja)
+                xs, js = shaped(xa, bmode), shaped(ja, bmode)
+                fx = fcn(xs) if js is None else fcn(xs, jac=js)
             else:
                 xn = xa.cpu().numpy()
                 jn = None if ja is None else ja.cpu().numpy()
                 if kind in ('lbatch', 'rbatch'):
-                    fx = fcn(xn) if jn is None else fcn(xn, jac=jn)
+                    xs, js = shaped(xn, bmode), shaped(jn, bmode)
+                    fx = fcn(xs) if js is None else fcn(xs, jac=js)
                 else:
-                    vals = [fcn(xi) if jn is None else fcn(xi, jac=jn[i]) for i, xi in enumerate(xn)]
+                    vals = [fcn(shaped(xi, 'point')) if jn is None else fcn(shaped(xi, 'point'), jac=shaped(jn[i], 'point'))
+                            for i, xi in enumerate(xn)]
                     if vals and isinstance(vals[0], dict):
                         fx = {k: np.array([v[k] for v in vals], dtype=float) for k in vals[0]}
                     else:
@@ -603,6 +691,20 @@ class Integrator:
                 wgts.append(w)
         samples = np.concatenate(samples, axis=0)
         wgts = np.concatenate(wgts) / nit
+        xl = self._xlayout
+        if xl is not None and xl[0][0] is not None:
+            # dictionary region: one array per key, batch index last (rbatch)
+            # or first (lbatch), pyx:1851-1855
+            n = samples.shape[0]
+            out = {}
+            for k, shape, a, b in xl:
+                v = samples[:, a:b].reshape((n,) + shape)
+                out[k] = np.moveaxis(v, 0, -1) if mode == 'rbatch' else v
+            return wgts, out
+        if xl is not None:
+            shape = xl[0][1]
+            samples = samples.reshape((-1,) + shape)
+            return wgts, (np.moveaxis(samples, 0, -1) if mode == 'rbatch' else samples)
         if mode == 'rbatch':
             samples = samples.T
         return wgts, samples
