@@ -237,3 +237,8 @@ def test_structured_region_point_integrand_and_jac(va):
     assert abs(r.mean - 2 * 0.5 * 4) < 5 * r.sdev
     itg2 = pickle.loads(pickle.dumps(itg))
     assert itg2.settings() == itg.settings() and 'key/index' in itg2.settings()
+    # set(map=...) and set(old) swap the structure along with the map
+    old = itg.set(map=[[0, 1]] * 3)
+    assert 'key/index' not in itg.settings()
+    itg.set(old)
+    assert itg.settings() == itg2.settings()

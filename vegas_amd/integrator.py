@@ -200,6 +200,9 @@ class Integrator:
         for k in kargs:
             if k == 'map':
                 old_val['map'] = self.map
+                # the old map carries the old region's structure, so that
+                # set(old) restores both
+                self.map._xlayout = self._xlayout
                 self.map = _region_map(kargs['map'])
                 self._xlayout = _region_layout(kargs['map'])
             elif k == 'nstrat':
