@@ -242,3 +242,50 @@ def test_structured_region_point_integrand_and_jac(va):
     assert 'key/index' not in itg.settings()
     itg.set(old)
     assert itg.settings() == itg2.settings()
+
+
+def test_volume_python_integrands(va):
+    """tests/test_vegas.py:774-789: constants integrate to value x volume
+    with (next to) zero error, scalar and array-valued."""
+    I = va.Integrator([[-1, 1], [0, 4]])
+    r = I(lambda x: 2.)
+    np.testing.assert_allclose(r.mean, 16, rtol=1e-6)
+    assert r.sdev < 1e-6
+    I = va.Integrator([[-1, 1], [0, 4]])
+    r = I(lambda x: [-1., 2.])
+    np.testing.assert_allclose(r[0].mean, -8, rtol=5e-2)
+    np.testing.assert_allclose(r[1].mean, 16, rtol=5e-2)
+    assert r[0].sdev < 1e-6 and r[1].sdev < 1e-6
+
+
+def test_integrand_exceptions_propagate(va):
+    """tests/test_vegas.py:1012-1029: an exception raised by the integrand
+    reaches the caller unchanged, point by point and batched."""
+    import math
+
+    def f(x):
+        return (math.sin(x[0]) ** 2 + math.cos(x[1]) ** 2) / math.pi ** 2 / 0.0
+    I = va.Integrator([[0, math.pi], [-math.pi / 2., math.pi / 2.]])
+    with pytest.raises(ZeroDivisionError):
+        I(f, neval=100)
+
+    @va.batchintegrand
+    class f_batch:
+        def __call__(self, x):
+            d = 1 / 0.
+            return (np.sin(x[:, 0]) ** 2 + np.cos(x[:, 1]) ** 2) / d
+    with pytest.raises(ZeroDivisionError):
+        I(f_batch(), neval=100)
+
+
+def test_correlate_integrals(va):
+    """tests/test_vegas.py:1347-1355: with correlate_integrals the two
+    results are positively correlated; without, their covariance is
+    exactly zero."""
+    def f(x):
+        return [np.prod(x), np.prod(x) ** 2]
+    a_b = va.Integrator(2 * [[0, 1]], correlate_integrals=True)(f)
+    cov = np.asarray(a_b.cov)
+    assert cov[0, 1] / np.sqrt(cov[0, 0] * cov[1, 1]) > 0.0
+    aa_bb = va.Integrator(2 * [[0, 1]], correlate_integrals=False)(f)
+    assert np.asarray(aa_bb.cov)[0, 1] == 0.0
