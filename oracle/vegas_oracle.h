@@ -11,7 +11,7 @@
  * compiled with -O2 -ffp-contract=off so that the order and rounding of each
  * floating-point operation is the reference's (no fused multiply-add).
  *
- * Pinning status of each function (details in DESIGN.md):
+ * Pinning status of each function (what each check fixes):
  *   vgo_philox4x32_10   Random123 known-answer vectors (SC'11 paper's KATs)
  *   vgo_uniforms_*      bit layout checked against independent numpy code
  *   vgo_make_uniform    numpy.linspace bit-for-bit

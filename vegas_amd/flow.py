@@ -8,7 +8,7 @@ minimising the variance of f/q, the per-sample variance of the importance
 sampling estimate: on samples x_i with sampling weights w_i (vegas' own
 weights, so that sum_i w_i f(x_i) estimates the integral) the loss
 sum_i w_i f_i^2 / q(x_i) estimates  int f^2/q dx = Var_q(f/q) + I^2.  The
-vegas grid stays the fallback and the judge: :meth:`FlowMap.accept` keeps
+vegas grid stays the fallback and the arbiter: :meth:`FlowMap.accept` keeps
 the flow only when its measured per-sample variance is lower.
 
 Layout of the FP32 parameter vector (per layer): W1[128][16] (column 15 is
