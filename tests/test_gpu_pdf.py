@@ -124,3 +124,32 @@ def test_restratify_moves_strata_to_the_varying_axis(va):
     assert new.nstrat[0] > old[0] and np.all(new.nstrat[1:] == 1)
     r = new(f, nitn=5)
     assert abs(r.mean - 1.0) < 5 * r.sdev
+
+
+def test_stats_moments(va):
+    """stats(moments=True) (__init__.py:999-1010): for p ~ N(MEAN, COV),
+    f = p has skew 0 and excess kurtosis 0; f = (p_0 - 0.5)^2 / 0.25 is
+    chi-square with one degree of freedom: mean 1, sdev sqrt(2), skew
+    sqrt(8), excess kurtosis 12."""
+    itg = va.PDFIntegrator((MEAN, COV), neval=40000, nitn=8, seed=21)
+    itg(nitn=5)
+    st = itg.stats(moments=True)
+    assert st.stats.shape == (2,)
+    for i in range(2):
+        s = st.stats[i]
+        assert abs(s.mean.mean - MEAN[i]) < 0.01
+        assert abs(s.sdev.mean - math.sqrt(COV[i, i])) < 0.01
+        assert abs(s.skew.mean) < 5 * s.skew.sdev + 0.02, s
+        assert abs(s.ex_kurt.mean) < 5 * s.ex_kurt.sdev + 0.05, s
+    np.testing.assert_allclose(st.mean, [s.mean.mean for s in st.stats], rtol=1e-12)
+
+    @va.lbatchintegrand
+    def chi2(p):
+        return (p[:, 0] - 0.5) ** 2 / 0.25
+    r = itg.stats(chi2, moments=True)
+    s = r.stats
+    assert abs(r.mean - 1.0) < 0.01 and abs(s.sdev.mean - math.sqrt(2.0)) < 0.03, s
+    assert abs(s.skew.mean - math.sqrt(8.0)) < 5 * s.skew.sdev + 0.1, s
+    assert abs(s.ex_kurt.mean - 12.0) < 5 * s.ex_kurt.sdev + 1.0, s
+    with pytest.raises(NotImplementedError):
+        itg.stats(histograms=True)

@@ -89,3 +89,31 @@ def test_restratified_strata():
     assert new[0] >= new[1] >= new[2] >= new[3] >= 1 and np.prod(new) <= np.prod(old)
     new0 = pdf._restratified([16.0, 4.0, 2.0, 1.0], old, gamma=0.0)
     assert np.all(np.abs(new0 - old) <= 1)
+
+
+def test_pdf_statistics_closed_forms_and_propagation():
+    """PDFStatistics from raw moments: the standard normal (1, 0, 3 for
+    E[z^2], E[z^3], E[z^4]) has skew 0 and excess kurtosis 0; z^2 for a
+    standard normal z is chi-square with one degree of freedom, raw moments
+    1, 3, 15, 105: mean 1, sdev sqrt(2), skew sqrt(8), excess kurtosis 12.
+    The propagated errors equal J C J^T with J by central differences."""
+    s = pdf.PDFStatistics([0.0, 1.0, 0.0, 3.0])
+    assert (s.mean.mean, s.sdev.mean, s.skew.mean, s.ex_kurt.mean) == (0.0, 1.0, 0.0, 0.0)
+    s = pdf.PDFStatistics([1.0, 3.0, 15.0, 105.0])
+    np.testing.assert_allclose([s.mean.mean, s.sdev.mean, s.skew.mean, s.ex_kurt.mean],
+                               [1.0, np.sqrt(2.0), np.sqrt(8.0), 12.0], rtol=1e-14)
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((4, 4)) * 1e-3
+    cov = A @ A.T
+    m = np.array([0.3, 1.2, 0.9, 4.1])
+    s = pdf.PDFStatistics(m, cov)
+
+    def vals(mm):
+        t = pdf.PDFStatistics(mm)
+        return np.array([t.mean.mean, t.sdev.mean, t.skew.mean, t.ex_kurt.mean])
+    h = 1e-6
+    J = np.array([(vals(m + h * e) - vals(m - h * e)) / (2 * h) for e in np.eye(4)]).T
+    want = np.sqrt(np.diag(J @ cov @ J.T))
+    got = [s.mean.sdev, s.sdev.sdev, s.skew.sdev, s.ex_kurt.sdev]
+    np.testing.assert_allclose(got, want, rtol=1e-6)
+    assert 'ex_kurt' in str(s)

@@ -21,7 +21,7 @@ def __getattr__(name):
     if name in ('Integrator', 'reporter'):
         from . import integrator
         return getattr(integrator, name)
-    if name in ('PDFIntegrator', 'PDFEV', 'PDFEVArray', 'PDFEVDict', 'restratify'):
+    if name in ('PDFIntegrator', 'PDFEV', 'PDFEVArray', 'PDFEVDict', 'PDFStatistics', 'restratify'):
         from . import pdf
         return getattr(pdf, name)
     raise AttributeError(name)

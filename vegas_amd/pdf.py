@@ -164,6 +164,41 @@ def _ratio(mean, cov, ipdf):
     return F / P, J @ cov @ J.T
 
 
+class PDFStatistics:
+    """Statistics of one component of f(p) from its first four raw moments
+    m1..m4 under the PDF (gvar.PDFStatistics, as used at
+    __init__.py:999-1010): ``mean``, ``sdev``, ``skew`` and ``ex_kurt``
+    (excess kurtosis), each a GVar whose error is propagated to first order
+    from the covariance ``cov`` of the moments' Monte Carlo estimates."""
+
+    def __init__(self, moments, cov=None):
+        m1, m2, m3, m4 = (float(v) for v in moments)
+        cov = np.zeros((4, 4)) if cov is None else np.asarray(cov, float).reshape(4, 4)
+        var = m2 - m1 * m1
+        c3 = m3 - 3 * m1 * m2 + 2 * m1 ** 3
+        c4 = m4 - 4 * m1 * m3 + 6 * m1 * m1 * m2 - 3 * m1 ** 4
+        dvar = np.array([-2 * m1, 1.0, 0.0, 0.0])
+        sd = math.sqrt(var) if var > 0 else 0.0
+        vals = [(m1, np.array([1.0, 0.0, 0.0, 0.0]))]
+        if var > 0:
+            dc3 = np.array([-3 * m2 + 6 * m1 * m1, -3 * m1, 1.0, 0.0])
+            dc4 = np.array([-4 * m3 + 12 * m1 * m2 - 12 * m1 ** 3, 6 * m1 * m1, -4 * m1, 1.0])
+            vals += [(sd, dvar / (2 * sd)),
+                     (c3 / var ** 1.5, dc3 / var ** 1.5 - 1.5 * c3 / var ** 2.5 * dvar),
+                     (c4 / var ** 2 - 3.0, dc4 / var ** 2 - 2 * c4 / var ** 3 * dvar)]
+        else:
+            vals += [(0.0, np.zeros(4)), (float('nan'), np.zeros(4)), (float('nan'), np.zeros(4))]
+        g = [GVar(v, math.sqrt(max(float(J @ cov @ J), 0.0))) for v, J in vals]
+        self.mean, self.sdev, self.skew, self.ex_kurt = g
+        self.moments = [m1, m2, m3, m4]
+
+    def __str__(self):
+        return 'mean = %s   sdev = %s   skew = %s   ex_kurt = %s' % (
+            self.mean, self.sdev, self.skew, self.ex_kurt)
+
+    __repr__ = __str__
+
+
 class PDFIntegrator(Integrator):
     """Expectation values with respect to a PDF (__init__.py:373-1188).
 
@@ -291,10 +326,14 @@ class PDFIntegrator(Integrator):
     def stats(self, f=None, moments=False, histograms=False, **kargs):
         """Mean and covariance of the distribution of f(p) under the PDF
         (__init__.py:729-946): E[f] and E[f f^T] - E[f] E[f]^T from one
-        integration.  f=None means f(p) = p.  ``moments`` and
-        ``histograms`` need gvar and are not provided."""
-        if moments or histograms:
-            raise NotImplementedError('stats(moments=..., histograms=...) needs gvar')
+        integration, plus the integration's own covariance of E[f]
+        (__init__.py:965-968).  f=None means f(p) = p.  With
+        ``moments=True`` the result also carries ``stats``: a
+        :class:`PDFStatistics` per component of f (__init__.py:999-1010),
+        from E[f], E[f^2], E[f^3] and E[f^4] of the same integration.
+        ``histograms`` needs gvar's histogram support and is not provided."""
+        if histograms:
+            raise NotImplementedError('stats(histograms=True) is not provided')
         shape_box = {}
 
         @lbatchintegrand
@@ -307,21 +346,34 @@ class PDFIntegrator(Integrator):
             shape_box['s'] = fp.shape[1:]
             v = fp.reshape(n, -1)
             il = np.tril_indices(v.shape[1])
-            return np.concatenate([v, (v[:, :, None] * v[:, None, :])[:, il[0], il[1]]], axis=1)
+            cols = [v, (v[:, :, None] * v[:, None, :])[:, il[0], il[1]]]
+            if moments:
+                cols += [v ** 3, v ** 4]
+            return np.concatenate(cols, axis=1)
 
         ev = self(fff, **kargs)
-        m = ev.mean.reshape(-1)
-        k = int(round((math.sqrt(8 * len(m) + 9) - 3) / 2))  # k + k(k+1)/2 = len(m)
+        m = np.asarray(ev.mean, float).reshape(-1)
+        C = np.asarray(ev.cov, float).reshape(len(m), len(m))
+        shape = shape_box['s']
+        k = int(np.prod(shape)) if shape else 1
         mu = m[:k]
         second = np.zeros((k, k))
         il = np.tril_indices(k)
-        second[il] = m[k:]
+        second[il] = m[k:k + len(il[0])]
         second = second + np.tril(second, -1).T
-        cov = second - np.outer(mu, mu)
-        shape = shape_box['s']
+        cov = second - np.outer(mu, mu) + C[:k, :k]
         if shape == ():
-            return GVar(mu[0], math.sqrt(max(cov[0, 0], 0.0)))
-        return GVarArray(mu, cov, shape)
+            ans = GVar(mu[0], math.sqrt(max(cov[0, 0], 0.0)))
+        else:
+            ans = GVarArray(mu, cov, shape)
+        if moments:
+            n2 = k + len(il[0])
+            st = []
+            for i in range(k):
+                idx = [i, k + i * (i + 1) // 2 + i, n2 + i, n2 + k + i]
+                st.append(PDFStatistics(m[idx], C[np.ix_(idx, idx)]))
+            ans.stats = st[0] if shape == () else np.array(st, dtype=object).reshape(shape)
+        return ans
 
     def sample(self, nbatch, mode='rbatch'):
         """Weighted samples of p from the PDF (__init__.py:1049-1188): at
