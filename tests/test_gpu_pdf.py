@@ -151,5 +151,34 @@ def test_stats_moments(va):
     assert abs(r.mean - 1.0) < 0.01 and abs(s.sdev.mean - math.sqrt(2.0)) < 0.03, s
     assert abs(s.skew.mean - math.sqrt(8.0)) < 5 * s.skew.sdev + 0.1, s
     assert abs(s.ex_kurt.mean - 12.0) < 5 * s.ex_kurt.sdev + 1.0, s
-    with pytest.raises(NotImplementedError):
-        itg.stats(histograms=True)
+
+
+def test_stats_histograms(va):
+    """stats(histograms=...) (__init__.py:889-946, 1036-1042): bin
+    probabilities of p ~ N(MEAN, COV) are differences of the normal CDF at
+    the edges mean + z sdev, with under- and overflow at the ends; the
+    median is the mean.  stats leaves adapt and nitn as they were."""
+    from scipy.stats import norm
+    itg = va.PDFIntegrator((MEAN, COV), neval=40000, nitn=6, seed=23)
+    itg(nitn=5)
+    sd = np.sqrt(np.diag(COV))
+    st = itg.stats(histograms=dict(nbin=4, binwidth=1.0, loc=(MEAN, sd)))
+    assert itg.adapt and itg.nitn == 5
+    z = np.array([-2., -1., 0., 1., 2.])
+    want = np.diff(np.concatenate([[0.], norm.cdf(z), [1.]]))
+    for i in range(2):
+        bins, prob, perr = st.stats[i].histogram
+        np.testing.assert_allclose(bins, MEAN[i] + z * sd[i], rtol=1e-12, atol=1e-15)
+        assert np.all(np.abs(prob - want) < 5 * perr + 2e-3), (prob, want, perr)
+        assert abs(st.stats[i].median - MEAN[i]) < 0.02 * sd[i]
+    # default histograms: 12 bins of half a standard deviation, centred by
+    # a one-iteration pre-run
+    st = itg.stats(histograms=True, moments=True)
+    assert itg.nitn == 5
+    z = np.linspace(-3, 3, 13)
+    want = np.diff(np.concatenate([[0.], norm.cdf(z), [1.]]))
+    for i in range(2):
+        bins, prob, perr = st.stats[i].histogram
+        assert len(bins) == 13 and len(prob) == 14
+        np.testing.assert_allclose(prob, want, atol=5e-3)
+        assert abs(st.stats[i].skew.mean) < 5 * st.stats[i].skew.sdev + 0.02

@@ -117,3 +117,27 @@ def test_pdf_statistics_closed_forms_and_propagation():
     got = [s.mean.sdev, s.sdev.sdev, s.skew.sdev, s.ex_kurt.sdev]
     np.testing.assert_allclose(got, want, rtol=1e-6)
     assert 'ex_kurt' in str(s)
+
+
+def test_hist_counts_and_median():
+    """_hist_counts is one-hot per column with underflow first and overflow
+    last (searchsorted side='right', __init__.py:1036-1042), so its column
+    sums are numpy's histogram with the two outer bins added; the median
+    read from bin probabilities of a uniform distribution on [0, 1] with
+    edges 0, 0.25, ..., 1 is 0.5, and 0.3 after moving mass below 0."""
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal((1000, 2))
+    bins = np.array([np.linspace(-2, 2, 9), np.linspace(-1, 3, 9)])
+    c = pdf._hist_counts(v, bins)
+    assert c.shape == (1000, 2, 10) and np.all(c.sum(axis=2) == 1)
+    for j in range(2):
+        inner = np.histogram(v[:, j], bins[j])[0].astype(float)
+        # numpy's last bin is closed; the right-side rule puts the edge value in overflow
+        tot = c[:, j].sum(axis=0)
+        assert tot[0] == np.sum(v[:, j] < bins[j][0]) and tot[-1] == np.sum(v[:, j] >= bins[j][-1])
+        np.testing.assert_array_equal(tot[1:-1], inner)
+    edges = np.linspace(0, 1, 5)
+    s = pdf.PDFStatistics([0.5, 1 / 3], histogram=(edges, [0, .25, .25, .25, .25, 0], np.zeros(6)))
+    assert abs(s.median - 0.5) < 1e-15 and s.skew is None and 'median' in str(s)
+    s = pdf.PDFStatistics([0.5, 1 / 3], histogram=(edges, [0.2, .25, .25, .15, .15, 0], np.zeros(6)))
+    assert abs(s.median - 0.3) < 1e-12

@@ -164,6 +164,19 @@ def _ratio(mean, cov, ipdf):
     return F / P, J @ cov @ J.T
 
 
+def _hist_counts(v, bins):
+    """One-hot bin membership (n, k, nbin + 2) of each column of v (n, k):
+    index 0 below bins[j][0], nbin + 1 at or above bins[j][-1], as
+    searchsorted(side='right') does (__init__.py:1036-1042)."""
+    n, k = v.shape
+    nb2 = bins.shape[1] + 1
+    out = np.zeros((n, k, nb2))
+    rows = np.arange(n)
+    for j in range(k):
+        out[rows, j, np.searchsorted(bins[j], v[:, j], side='right')] = 1.0
+    return out
+
+
 class PDFStatistics:
     """Statistics of one component of f(p) from its first four raw moments
     m1..m4 under the PDF (gvar.PDFStatistics, as used at
@@ -171,9 +184,17 @@ class PDFStatistics:
     (excess kurtosis), each a GVar whose error is propagated to first order
     from the covariance ``cov`` of the moments' Monte Carlo estimates."""
 
-    def __init__(self, moments, cov=None):
+    def __init__(self, moments, cov=None, histogram=None):
+        nm = len(moments)
+        if nm not in (2, 4):
+            raise ValueError('moments: [m1, m2] or [m1, m2, m3, m4]')
+        cov = np.zeros((nm, nm)) if cov is None else np.asarray(cov, float).reshape(nm, nm)
+        if nm == 2:  # mean and sdev only
+            moments = list(moments) + [0.0, 0.0]
+            c4 = np.zeros((4, 4))
+            c4[:2, :2] = cov
+            cov = c4
         m1, m2, m3, m4 = (float(v) for v in moments)
-        cov = np.zeros((4, 4)) if cov is None else np.asarray(cov, float).reshape(4, 4)
         var = m2 - m1 * m1
         c3 = m3 - 3 * m1 * m2 + 2 * m1 ** 3
         c4 = m4 - 4 * m1 * m3 + 6 * m1 * m1 * m2 - 3 * m1 ** 4
@@ -189,12 +210,32 @@ class PDFStatistics:
         else:
             vals += [(0.0, np.zeros(4)), (float('nan'), np.zeros(4)), (float('nan'), np.zeros(4))]
         g = [GVar(v, math.sqrt(max(float(J @ cov @ J), 0.0))) for v, J in vals]
-        self.mean, self.sdev, self.skew, self.ex_kurt = g
-        self.moments = [m1, m2, m3, m4]
+        self.mean, self.sdev = g[:2]
+        self.skew, self.ex_kurt = (g[2], g[3]) if nm == 4 else (None, None)
+        self.moments = [m1, m2, m3, m4][:nm]
+        # histogram: bin edges [nbin + 1] and the probability of each bin,
+        # below the first edge first and above the last edge last
+        self.histogram = None
+        self.median = None
+        if histogram is not None:
+            bins, prob, perr = (np.asarray(a, float) for a in histogram)
+            self.histogram = (bins, prob, perr)
+            # median: the CDF interpolated linearly inside the bin where it
+            # passes one half (parity unpinned: gvar's own rule is not here)
+            cdf = np.cumsum(prob[:-1])
+            j = int(np.searchsorted(cdf, 0.5 * np.sum(prob)))
+            if 1 <= j <= len(bins) - 1:
+                c0 = cdf[j - 1]
+                frac = (0.5 * np.sum(prob) - c0) / prob[j] if prob[j] > 0 else 0.5
+                self.median = float(bins[j - 1] + frac * (bins[j] - bins[j - 1]))
 
     def __str__(self):
-        return 'mean = %s   sdev = %s   skew = %s   ex_kurt = %s' % (
-            self.mean, self.sdev, self.skew, self.ex_kurt)
+        out = 'mean = %s   sdev = %s' % (self.mean, self.sdev)
+        if self.skew is not None:
+            out += '   skew = %s   ex_kurt = %s' % (self.skew, self.ex_kurt)
+        if self.median is not None:
+            out += '   median = %.5g' % self.median
+        return out
 
     __repr__ = __str__
 
@@ -327,31 +368,72 @@ class PDFIntegrator(Integrator):
         """Mean and covariance of the distribution of f(p) under the PDF
         (__init__.py:729-946): E[f] and E[f f^T] - E[f] E[f]^T from one
         integration, plus the integration's own covariance of E[f]
-        (__init__.py:965-968).  f=None means f(p) = p.  With
-        ``moments=True`` the result also carries ``stats``: a
-        :class:`PDFStatistics` per component of f (__init__.py:999-1010),
-        from E[f], E[f^2], E[f^3] and E[f^4] of the same integration.
-        ``histograms`` needs gvar's histogram support and is not provided."""
-        if histograms:
-            raise NotImplementedError('stats(histograms=True) is not provided')
+        (__init__.py:965-968).  f=None means f(p) = p.  The grid is not
+        adapted unless ``adapt`` is given (__init__.py:871-874).  With
+        ``moments=True`` or ``histograms`` the result also carries
+        ``stats``: a :class:`PDFStatistics` per component of f
+        (__init__.py:999-1010), from E[f], E[f^2], E[f^3], E[f^4] and the
+        probability of each histogram bin, all from the same integration.
+        ``histograms`` is True or a dict with ``nbin`` (12), ``binwidth``
+        (0.5) and ``loc`` = (mean, sdev) arrays of the histogram centres and
+        widths; without ``loc`` they come from a one-iteration integration
+        of f and f^2 (__init__.py:889-926)."""
+        old_adapt = None
+        if 'adapt' not in kargs:
+            old_adapt = self.adapt
+            kargs['adapt'] = False
         shape_box = {}
 
-        @lbatchintegrand
-        def fff(p):
+        def fvals(p):
             fp = p if f is None else _call_user(f, p)
             if isinstance(fp, dict):
                 raise NotImplementedError('stats of dictionary-valued f')
             fp = np.asarray(fp, float)
-            n = fp.shape[0]
             shape_box['s'] = fp.shape[1:]
-            v = fp.reshape(n, -1)
+            return fp.reshape(fp.shape[0], -1)
+
+        hist = None
+        if histograms is not False:
+            hist = {} if histograms is True else dict(histograms)
+            nbin = int(hist.get('nbin', 12))
+            binwidth = float(hist.get('binwidth', 0.5))
+            loc = hist.get('loc', None)
+            if loc is not None:
+                hmean = np.asarray(loc[0], float).reshape(-1)
+                hsdev = np.asarray(loc[1], float).reshape(-1)
+            else:
+                @lbatchintegrand
+                def ff2(p):
+                    v = fvals(p)
+                    return np.concatenate([v, v * v], axis=1)
+                old_nitn = self.nitn
+                r = self(ff2, nitn=1, **{k: v for k, v in kargs.items() if k != 'nitn'})
+                self.set(nitn=old_nitn)
+                m2 = np.asarray(r.mean, float).reshape(-1)
+                kk = len(m2) // 2
+                hmean = m2[:kk]
+                hsdev = np.fabs(m2[kk:] - hmean * hmean) ** 0.5
+            half = nbin / 2 * binwidth
+            hist['bins'] = np.array([hmean[i] + np.linspace(-half * hsdev[i], half * hsdev[i], nbin + 1)
+                                     for i in range(len(hmean))])
+            hist['nbin'] = nbin
+
+        @lbatchintegrand
+        def fff(p):
+            v = fvals(p)
             il = np.tril_indices(v.shape[1])
             cols = [v, (v[:, :, None] * v[:, None, :])[:, il[0], il[1]]]
             if moments:
                 cols += [v ** 3, v ** 4]
+            if hist is not None:
+                cols.append(_hist_counts(v, hist['bins']).reshape(v.shape[0], -1))
             return np.concatenate(cols, axis=1)
 
-        ev = self(fff, **kargs)
+        try:
+            ev = self(fff, **kargs)
+        finally:
+            if old_adapt is not None:
+                self.set(adapt=old_adapt)
         m = np.asarray(ev.mean, float).reshape(-1)
         C = np.asarray(ev.cov, float).reshape(len(m), len(m))
         shape = shape_box['s']
@@ -366,12 +448,18 @@ class PDFIntegrator(Integrator):
             ans = GVar(mu[0], math.sqrt(max(cov[0, 0], 0.0)))
         else:
             ans = GVarArray(mu, cov, shape)
-        if moments:
+        if moments or hist is not None:
             n2 = k + len(il[0])
+            n4 = n2 + (2 * k if moments else 0)
             st = []
             for i in range(k):
-                idx = [i, k + i * (i + 1) // 2 + i, n2 + i, n2 + k + i]
-                st.append(PDFStatistics(m[idx], C[np.ix_(idx, idx)]))
+                idx = [i, k + i * (i + 1) // 2 + i] + ([n2 + i, n2 + k + i] if moments else [])
+                h = None
+                if hist is not None:
+                    nb2 = hist['nbin'] + 2
+                    hi = np.arange(n4 + i * nb2, n4 + (i + 1) * nb2)
+                    h = (hist['bins'][i], m[hi], np.sqrt(np.maximum(np.diag(C)[hi], 0.0)))
+                st.append(PDFStatistics(m[idx], C[np.ix_(idx, idx)], histogram=h))
             ans.stats = st[0] if shape == () else np.array(st, dtype=object).reshape(shape)
         return ans
 
