@@ -182,3 +182,24 @@ def test_stats_histograms(va):
         assert len(bins) == 13 and len(prob) == 14
         np.testing.assert_allclose(prob, want, atol=5e-3)
         assert abs(st.stats[i].skew.mean) < 5 * st.stats[i].skew.sdev + 0.02
+
+
+def test_restratify_pdfintegrator(va):
+    """restratify of a PDFIntegrator (__init__.py:1436-1456, after
+    tests/test_vegas.py:1382-1397): the axes are weighed by the PDF itself
+    in the integrator's variables; a PDF narrow and bimodal along p_0 and
+    Gaussian along p_1 gets more strata on axis 0; neval is kept and the
+    PDF's normalisation agrees before and after."""
+    @va.lbatchintegrand
+    def pdf(p):
+        s2 = 0.01
+        a = np.exp(-(p[:, 0] - 1) ** 2 / (2 * s2)) + np.exp(-(p[:, 0] + 1) ** 2 / (2 * s2))
+        return a / (2 * math.sqrt(2 * math.pi * s2)) * np.exp(-p[:, 1] ** 2 / 2) / math.sqrt(2 * math.pi)
+    itg = va.PDFIntegrator(([0., 0.], [[1., 0.], [0., 1.]]), pdf=pdf, neval=2000, seed=31)
+    t = itg(nitn=2)
+    itg2 = va.restratify(itg)
+    assert isinstance(itg2, va.PDFIntegrator) and itg2.neval == itg.neval
+    assert itg2.nstrat[0] > itg2.nstrat[1], (itg.nstrat, itg2.nstrat, itg2.restratify_weight)
+    t2 = itg2(nitn=4)
+    d = t2.pdfnorm.mean - t.pdfnorm.mean
+    assert abs(d) < 5 * math.sqrt(t.pdfnorm.sdev ** 2 + t2.pdfnorm.sdev ** 2)

@@ -520,12 +520,20 @@ def restratify(integ, f=None, nitn=1, ndy=5, below_avg_nstrat=None, verbose=Fals
     nstrat / nstrat_mu) ** gamma, smallest first, the total kept by
     rescaling the remaining axes; axes of below-average weight get
     ``below_avg_nstrat`` strata when that is set.  The copy is then trained
-    on f for nitn iterations."""
-    if isinstance(integ, PDFIntegrator):
-        raise NotImplementedError('restratify of a PDFIntegrator')
-    if f is None:
-        raise ValueError('restratify needs the integrand f')
-    new = Integrator(integ)
+    on f for nitn iterations.  For a :class:`PDFIntegrator` the integrand
+    is the PDF itself, in the integrator's own variables, and the copy is a
+    PDFIntegrator trained on the PDF (__init__.py:1369-1374, 1436-1456)."""
+    is_pdf = isinstance(integ, PDFIntegrator)
+    if is_pdf:
+        # a copy through the pickle state, without serialising (the PDF
+        # may be a local function): settings, sigf, grid and stream position
+        cls, args, odict = integ.__reduce__()
+        new = cls(*args)
+        new.__setstate__(odict)
+    else:
+        if f is None:
+            raise ValueError('restratify needs the integrand f')
+        new = Integrator(integ)
     if vargs:
         new.set(vargs)
     dim = new.dim
@@ -538,7 +546,10 @@ def restratify(integ, f=None, nitn=1, ndy=5, below_avg_nstrat=None, verbose=Fals
         y = np.zeros(x.shape, float)
         jac = np.zeros(n, float)
         new.map.invmap(x, y, jac)
-        I = np.asarray(_call_user(f, x), float).reshape(n, -1)[:, 0]
+        if is_pdf:
+            I = new._p_and_weight(np.asarray(x, float))[1]
+        else:
+            I = np.asarray(_call_user(f, x), float).reshape(n, -1)[:, 0]
         dI = np.zeros((n, dim, ndy), float)
         for mu in range(dim):
             for i in range(ndy):
@@ -547,7 +558,8 @@ def restratify(integ, f=None, nitn=1, ndy=5, below_avg_nstrat=None, verbose=Fals
         return dict(I=I, dI=dI)
 
     save = new.set(correlate_integrals=False, nitn=nitn, alpha=0, adapt=True)
-    result = new(I_dI)
+    # in the integrator's own variables (for a PDFIntegrator: theta)
+    result = Integrator.__call__(new, I_dI)
     new.set(save)
     if verbose:
         print('\n==================== restratify')
@@ -565,7 +577,7 @@ def restratify(integ, f=None, nitn=1, ndy=5, below_avg_nstrat=None, verbose=Fals
     new_nstrat = _restratified(weight, old_nstrat, gamma, below_avg_nstrat)
     new.set(nstrat=new_nstrat, neval=new.neval)
     save = new.set(nitn=nitn, adapt=True)
-    training = new(f)
+    training = new() if is_pdf else new(f)
     new.set(save)
     new.restratify_weight = weight
     if verbose:
