@@ -203,3 +203,25 @@ def test_restratify_pdfintegrator(va):
     t2 = itg2(nitn=4)
     d = t2.pdfnorm.mean - t.pdfnorm.mean
     assert abs(d) < 5 * math.sqrt(t.pdfnorm.sdev ** 2 + t2.pdfnorm.sdev ** 2)
+
+
+def test_stats_dict_valued_f(va):
+    """stats of a dictionary-valued f (__init__.py:970-986): one value per
+    key with the covariance of all values in key order, and statistics per
+    key.  a = p_0 and b = p, so cov(a, b_0) = var(p_0) and
+    cov(a, b_1) = cov(p_0, p_1)."""
+    itg = va.PDFIntegrator((MEAN, COV), neval=20000, nitn=6, seed=25)
+    itg(nitn=4)
+
+    @va.lbatchintegrand
+    def f(p):
+        return dict(a=p[:, 0], b=p)
+    st = itg.stats(f, moments=True)
+    assert set(st) == {'a', 'b'}
+    assert abs(st['a'].mean - MEAN[0]) < 0.01
+    np.testing.assert_allclose(st['b'].mean, MEAN, atol=0.01)
+    cov = np.asarray(st.cov)
+    assert cov.shape == (3, 3)
+    np.testing.assert_allclose([cov[0, 0], cov[0, 1], cov[0, 2]], [COV[0, 0], COV[0, 0], COV[0, 1]], rtol=0.1)
+    assert st.stats['b'].shape == (2,)
+    assert abs(st.stats['a'].skew.mean) < 5 * st.stats['a'].skew.sdev + 0.02

@@ -385,12 +385,15 @@ class PDFIntegrator(Integrator):
         shape_box = {}
 
         def fvals(p):
+            # f's values as (n, k) columns; a dictionary's keys in order
             fp = p if f is None else _call_user(f, p)
+            n = len(p)
             if isinstance(fp, dict):
-                raise NotImplementedError('stats of dictionary-valued f')
+                v, shape_box['lay'] = _flatten_f(fp, n)
+                return v
             fp = np.asarray(fp, float)
-            shape_box['s'] = fp.shape[1:]
-            return fp.reshape(fp.shape[0], -1)
+            shape_box['lay'] = ('array', fp.shape[1:])
+            return fp.reshape(n, -1)
 
         hist = None
         if histograms is not False:
@@ -436,15 +439,22 @@ class PDFIntegrator(Integrator):
                 self.set(adapt=old_adapt)
         m = np.asarray(ev.mean, float).reshape(-1)
         C = np.asarray(ev.cov, float).reshape(len(m), len(m))
-        shape = shape_box['s']
-        k = int(np.prod(shape)) if shape else 1
+        kind, lay = shape_box['lay']
+        shape = lay if kind == 'array' else None
+        k = (int(np.prod(shape)) if shape else 1) if kind == 'array' else lay[-1][3]
         mu = m[:k]
         second = np.zeros((k, k))
         il = np.tril_indices(k)
         second[il] = m[k:k + len(il[0])]
         second = second + np.tril(second, -1).T
         cov = second - np.outer(mu, mu) + C[:k, :k]
-        if shape == ():
+        if kind == 'dict':
+            items = {}
+            for key, shp, a, b in lay:
+                items[key] = (GVar(mu[a], math.sqrt(max(cov[a, a], 0.0))) if shp == ()
+                              else GVarArray(mu[a:b], cov[a:b, a:b], shp))
+            ans = PDFEVDict(items, cov)
+        elif shape == ():
             ans = GVar(mu[0], math.sqrt(max(cov[0, 0], 0.0)))
         else:
             ans = GVarArray(mu, cov, shape)
@@ -460,7 +470,11 @@ class PDFIntegrator(Integrator):
                     hi = np.arange(n4 + i * nb2, n4 + (i + 1) * nb2)
                     h = (hist['bins'][i], m[hi], np.sqrt(np.maximum(np.diag(C)[hi], 0.0)))
                 st.append(PDFStatistics(m[idx], C[np.ix_(idx, idx)], histogram=h))
-            ans.stats = st[0] if shape == () else np.array(st, dtype=object).reshape(shape)
+            if kind == 'dict':
+                ans.stats = {key: (st[a] if shp == () else np.array(st[a:b], dtype=object).reshape(shp))
+                             for key, shp, a, b in lay}
+            else:
+                ans.stats = st[0] if shape == () else np.array(st, dtype=object).reshape(shape)
         return ans
 
     def sample(self, nbatch, mode='rbatch'):
