@@ -225,3 +225,26 @@ def test_stats_dict_valued_f(va):
     np.testing.assert_allclose([cov[0, 0], cov[0, 1], cov[0, 2]], [COV[0, 0], COV[0, 0], COV[0, 1]], rtol=0.1)
     assert st.stats['b'].shape == (2,)
     assert abs(st.stats['a'].skew.mean) < 5 * st.stats['a'].skew.sdev + 0.02
+
+
+def test_limit_scale(va):
+    """tests/test_vegas.py:1688-1702: with limit = 1 (2) standard deviations
+    the PDF's normalisation is the normal probability within 1 (2) sigma,
+    for scale 1 and 2."""
+    for scale in [1., 2.]:
+        norm = va.PDFIntegrator(([1.], [[0.01]]), limit=1., scale=scale, seed=41)(neval=1000, nitn=5).pdfnorm
+        assert abs(norm.mean - 0.682689492137) < 5 * norm.sdev, (scale, norm)
+        norm = va.PDFIntegrator(([1.], [[0.01]]), limit=2., scale=scale, seed=42)(neval=1000, nitn=5).pdfnorm
+        assert abs(norm.mean - 0.954499736104) < 5 * norm.sdev, (scale, norm)
+
+
+def test_adapt_to_pdf_false(va):
+    """tests/test_vegas.py:1677-1686: adapt_to_pdf=False adapts the grid to
+    f * pdf; E[p_0 + p_1] is still the sum of the means."""
+    itg = va.PDFIntegrator(([1., 2.], [[1, .1], [.1, 4]]), alpha=0, beta=0, adapt_to_pdf=False, seed=43)
+
+    @va.rbatchintegrand
+    def f(p):
+        return p[0] + p[1]
+    r = itg(f, nitn=2)
+    assert abs(r.mean - 3.0) < 5 * r.sdev, r
