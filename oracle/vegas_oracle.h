@@ -17,7 +17,13 @@
  *   vgo_make_uniform    numpy.linspace bit-for-bit
  *   vgo_map / invmap    tests/test_vegas.py:76-112 fixture, round trips
  *   vgo_add_training    brute-force numpy bincount restatement
- *   vgo_adapt           tests/test_vegas.py:39-63, 142-207 closed forms
+ *   vgo_adapt           tests/test_vegas.py:39-63, 142-207 closed forms;
+ *                       alpha > 0 smoothing and compression against the
+ *                       printed grids of doc/source/eg2a.out (an ensemble
+ *                       over seeded y; a dropped compression or an interior
+ *                       smoothing weight of 2 instead of 6 fails it, a
+ *                       weight of 4 does not: that weight is pinned only
+ *                       against gross changes)
  *   vgo_geometry        tests/test_vegas.py:561-595, 703-772, eg10.out, eg6a.out
  *   vgo_iterate         constant-integrand known answers (tests:774-789,
  *                       1272-1296), analytic integrals, closed-form invariants

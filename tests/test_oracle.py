@@ -161,6 +161,109 @@ def test_adapt_closed_forms(orc):
     assert_allclose(_gauss_train(orc, [[0, 2], [0, 4]], F, 20, -2.).grid, want)
 
 
+# doc/source/eg2.py / eg2a.out:1-24: AdaptiveMap ninc=5 on [0,1]^2 trained
+# with alpha=1.5 on f = x0 * x1^2 from 1000 uniform y, grid printed to three
+# decimals after each of 5 adapt steps.  The y of that run are unseeded numpy
+# draws, so the printed rows are one realisation of a random grid: the check
+# is that they are a plausible draw from the oracle's ensemble over seeded y.
+EG2_ROWS = {
+    0: [[0., 0.412, 0.62, 0.76, 0.883, 1.], [0., 0.506, 0.691, 0.821, 0.91, 1.]],   # eg2a.out:5-7
+    4: [[0., 0.436, 0.631, 0.772, 0.895, 1.], [0., 0.533, 0.715, 0.831, 0.924, 1.]],  # eg2a.out:21-23
+}
+EG2_NSEED = 400
+# chi^2 with 8 degrees of freedom (the interior nodes of both axes) at p = 1e-3
+EG2_CHI2_MAX = 26.12
+
+
+def _adapt_axis_np(grid, sum_f, n_f, alpha, compress=True, w_mid=6.0):
+    """One axis of the reference's adapt (pyx:530-587) restated in numpy, with
+    two knobs that plant mistakes: compress=False drops the (-(1-a)/ln a)^alpha
+    step (pyx:560-562), w_mid changes the interior smoothing weight
+    (pyx:547-552).  With the defaults it equals the C oracle to a few ulps
+    (numpy's log and power may round differently from libm's;
+    test_eg2_fixture_is_sensitive checks the agreement)."""
+    ninc = len(grid) - 1
+    a = np.where(n_f > 0, sum_f / np.where(n_f > 0, n_f, 1.0), 0.0)
+    t = np.empty(ninc)
+    t[0] = abs(7 * a[0] + a[1]) / 8
+    t[-1] = abs(7 * a[-1] + a[-2]) / 8
+    for i in range(1, ninc - 1):
+        t[i] = abs(w_mid * a[i] + a[i - 1] + a[i + 1]) / (w_mid + 2)
+    s = t[0] + t[-1] + sum(t[1:-1])
+    a = t / s + 1e-257 if s > 0 else np.full(ninc, 1e-257)
+    if compress:
+        m = (a > 0) & (a <= 0.99999999)
+        a[m] = (-(1 - a[m]) / np.log(a[m])) ** alpha
+    inc = np.diff(grid)
+    new = np.empty(ninc + 1)
+    new[0], new[-1] = grid[0], grid[-1]
+    fbar = a.sum() / ninc
+    acc, j = 0.0, -1
+    for i in range(1, ninc):
+        while acc < fbar:
+            j += 1
+            acc += a[j]
+        acc -= fbar
+        new[i] = grid[j + 1] - (acc / a[j]) * inc[j]
+    return new
+
+
+def _eg2_rows(orc, seed, variant=None):
+    """Grids after adapt steps 0..4 of eg2.py on the y drawn from seed: through
+    the C oracle (variant None), or through _adapt_axis_np with the variant's
+    knobs (the oracle's map and add_training_data feed it)."""
+    y = np.random.default_rng(seed).uniform(0., 1., (1000, 2))
+    m = orc.OracleMap([[0, 1], [0, 1]], ninc=5)
+    rows = []
+    for _ in range(5):
+        x, jac = m.map(y)
+        m.add_training_data(y, (jac * x[:, 0] * x[:, 1] ** 2) ** 2)
+        if variant is None:
+            m.adapt(alpha=1.5)
+            g = m.grid.copy()
+        else:
+            g = np.array([_adapt_axis_np(m.grid[d], m.sum_f[d], m.n_f[d], 1.5, **variant)
+                          for d in range(2)])
+            m = orc.OracleMap([list(g[0]), list(g[1])])
+        rows.append(g)
+    return np.array(rows)
+
+
+def _eg2_chi2(ens, step):
+    """Mahalanobis distance of the printed row of adapt step `step` from the
+    ensemble (its mean and covariance over seeds; the interior nodes of both
+    axes, with the print's rounding to 1e-3 added as uniform noise)."""
+    X = ens[:, step, :, 1:-1].reshape(len(ens), -1)
+    cov = np.cov(X.T) + np.eye(X.shape[1]) * (0.0005 ** 2 / 3)
+    d = np.array(EG2_ROWS[step])[:, 1:-1].reshape(-1) - X.mean(0)
+    return float(d @ np.linalg.solve(cov, d))
+
+
+def test_adapt_alpha_eg2_fixture(orc):
+    """The reference's printed eg2 grids lie inside the oracle's ensemble.
+
+    Pins the alpha > 0 steps of adapt (smoothing, normalisation, compression,
+    regrid; pyx:546-587) with data the reference holds.  Measured: chi^2 = 13.4
+    (step 0) and 11.1 (step 4) for the oracle; the same ensemble without the
+    compression gives 3600 at step 0, an interior smoothing weight of 2
+    instead of 6 gives 61 and 48 (test_eg2_fixture_is_sensitive)."""
+    ens = np.array([_eg2_rows(orc, s) for s in range(EG2_NSEED)])
+    for step in EG2_ROWS:
+        assert _eg2_chi2(ens, step) < EG2_CHI2_MAX, step
+
+
+def test_eg2_fixture_is_sensitive(orc):
+    # the numpy restatement with default knobs is the C oracle
+    for s in range(3):
+        assert_allclose(_eg2_rows(orc, s, {}), _eg2_rows(orc, s), rtol=1e-13, atol=1e-15)
+    # planted mistakes in the restated oracle miss the printed rows
+    n = EG2_NSEED // 2
+    nocomp = np.array([_eg2_rows(orc, s, {"compress": False}) for s in range(n)])
+    assert _eg2_chi2(nocomp, 0) > 10 * EG2_CHI2_MAX
+    w2 = np.array([_eg2_rows(orc, s, {"w_mid": 2.0}) for s in range(n)])
+    assert max(_eg2_chi2(w2, 0), _eg2_chi2(w2, 4)) > EG2_CHI2_MAX
+
+
 def test_adapt_regrid_preserves_jacobian(orc):
     # alpha = 0 regrid (pyx:133-137, tests:50-57): the new grid reproduces the
     # old piecewise-linear map at every new node, for any ninc.
