@@ -110,29 +110,92 @@ class GVar:
         return self._mean
 
 
+def _scalar_stats(means, variances, weighted):
+    """Average, its standard deviation, chi^2 and degrees of freedom of the
+    scalar estimates (means[k], variances[k]).
+
+    weighted: inverse-variance weights, each variance floored at TINY (the
+    reference's RAvg.add, pyx:2392-2410; chi^2 pyx:2344-2357);
+    otherwise the plain average, whose variance is sum(var)/n^2 and whose
+    chi^2 measures the spread against the mean variance sum(var)/n."""
+    m = np.asarray(means, np.float64)
+    v = np.asarray(variances, np.float64)
+    n = m.size
+    if n == 0:
+        return 0.0, 0.0, 0.0, 0
+    if weighted:
+        w = 1.0 / np.maximum(v, TINY)
+        wsum = w.sum()
+        avg = float(np.dot(w, m) / wsum)
+        sdev = float(math.sqrt(1.0 / wsum))
+        chi2 = float(np.dot(w, (m - avg) ** 2)) if n > 1 else 0.0
+    else:
+        avg = float(m.sum() / n)
+        vsum = float(v.sum())
+        sdev = math.sqrt(vsum) / n
+        with np.errstate(divide='ignore', invalid='ignore'):
+            chi2 = float(np.sum((m - avg) ** 2) / np.float64(vsum / n)) if n > 1 else 0.0
+    return avg, sdev, chi2, n - 1
+
+
+def _q_value(dof, chi2):
+    """p-value of chi2 with dof degrees of freedom (nan when undefined)."""
+    if dof <= 0 or not chi2 >= 0:
+        return float('nan')
+    return gammaQ(dof / 2.0, chi2 / 2.0)
+
+
+def _progress_table(rows, weighted):
+    """The iteration table printed by the reference's summary() (pyx:2412-
+    2451): a right-aligned iteration number, then the iteration's result, the
+    running average (both left-aligned), chi2/dof and Q (right-aligned); the
+    column widths are those of the widest entry of the rows below the header,
+    and a rule as long as the header line separates them."""
+    heads = ('itn', 'integral', 'wgt average' if weighted else 'average', 'chi2/dof', 'Q')
+    widths = [max((len(r[c]) for r in rows), default=0) for c in range(5)]
+
+    def line(cells):
+        left = cells[0].rjust(widths[0])
+        mid = [cells[1].ljust(widths[1]), cells[2].ljust(widths[2]),
+               cells[3].rjust(widths[3]), cells[4].rjust(widths[4])]
+        return left + '   ' + ' '.join(mid) + '\n'
+
+    top = line(heads)
+    return top + '-' * (len(top) - 1) + '\n' + ''.join(line(r) for r in rows)
+
+
+def _progress_row(k, result, average, chi2, dof):
+    """One row of the iteration table: k counts from 0; the first row shows
+    chi2/dof 0 and Q 1 as the reference's does."""
+    if k == 0:
+        cd, q = 0.0, 1.0
+    else:
+        with np.errstate(divide='ignore', invalid='ignore'):
+            cd = np.float64(chi2) / dof  # dof 0 (degenerate arrays): inf or nan
+        q = _q_value(dof, chi2)
+    return ('%3d' % (k + 1), '%-15s' % result, '%-15s' % average, '%8.2f' % cd, '%8.2f' % q)
+
+
 class RAvg(GVar):
-    """Running average of scalar Monte Carlo estimates (pyx:2276-2451):
-    inverse-variance weighted when ``weighted`` (vegas' ``adapt=True``),
-    plain average otherwise."""
+    """Running average of scalar Monte Carlo estimates (the reference's RAvg,
+    pyx:2276-2451): inverse-variance weighted when ``weighted`` (vegas'
+    ``adapt=True``), the plain average otherwise.
+
+    The only state is the list of iteration results; the average, chi^2 and
+    Q are derived from it by _scalar_stats whenever a result is added."""
 
     def __init__(self, weighted=True, itn_results=None, sum_neval=0):
         super().__init__(0.0, 0.0)
         self.weighted = weighted
-        self._wlist = []
-        self._mlist = []
-        self._msum = 0.0
-        self._varsum = 0.0
-        self._n = 0
         self.itn_results = []
+        self._chi2 = 0.0
         if itn_results is not None:
-            for r in itn_results:
-                self.add(r)
+            self.add_all(itn_results)
         self.sum_neval = sum_neval
 
     def extend(self, ravg):
         """Merge results from ``ravg`` after those in ``self`` (pyx:2314-2318)."""
-        for r in ravg.itn_results:
-            self.add(r)
+        self.add_all(ravg.itn_results)
         self.sum_neval += ravg.sum_neval
 
     def __reduce__(self):
@@ -140,47 +203,23 @@ class RAvg(GVar):
                                 self.sum_neval))
 
     def add(self, g):
-        """Add estimate ``g`` (pyx:2392-2410).  The variance re-read from g is
-        sdev**2, as the reference re-reads it from its gvar."""
+        """Add estimate ``g`` (a GVar or (mean, sdev)); its variance is sdev**2,
+        as the reference re-reads it from its gvar (pyx:2392-2410)."""
         self.add_all([g])
 
     def add_all(self, gs):
-        """Add estimates in order; the same result as add() on each in turn
-        (the weighted sums are formed once, over the whole list)."""
+        """Add estimates in order (the same result as add() on each)."""
         for g in gs:
-            if not isinstance(g, GVar):
-                g = GVar(*g)
-            self.itn_results.append(g)
-            self._mlist.append(g.mean)
-            if self.weighted:
-                self._wlist.append(1 / (g.var if g.var > TINY else TINY))
-            else:
-                self._msum += g.mean
-                self._varsum += g.var
-                self._n += 1
-        if not self.itn_results:
-            return
-        if self.weighted:
-            var = 1. / np.sum(self._wlist)
-            sdev = np.sqrt(var)
-            mean = np.sum([w * m for w, m in zip(self._wlist, self._mlist)]) * var
-        else:
-            mean = self._msum / self._n
-            sdev = np.sqrt(self._varsum / self._n ** 2)
-        self._mean = float(mean)
-        self._sdev = float(sdev)
+            self.itn_results.append(g if isinstance(g, GVar) else GVar(*g))
+        if self.itn_results:
+            means = [g.mean for g in self.itn_results]
+            variances = [g.var for g in self.itn_results]
+            self._mean, self._sdev, self._chi2, _ = _scalar_stats(means, variances, self.weighted)
 
     @property
     def chi2(self):
-        """pyx:2344-2357"""
-        if len(self.itn_results) <= 1:
-            return 0.0
-        if self.weighted:
-            ans = 0.0
-            for m, w in zip(self._mlist, self._wlist):
-                ans += (self._mean - m) ** 2 * w
-            return ans
-        return np.sum([(m - self._mean) ** 2 for m in self._mlist]) / (self._varsum / self._n)
+        """chi^2 of the average (pyx:2344-2357)."""
+        return self._chi2
 
     @property
     def dof(self):
@@ -192,9 +231,8 @@ class RAvg(GVar):
 
     @property
     def Q(self):
-        """pyx:2372-2377"""
-        return gammaQ(self.dof / 2., self.chi2 / 2.) if self.dof > 0 and self.chi2 >= 0 \
-            else float('nan')
+        """p-value of chi2 (pyx:2372-2377)."""
+        return _q_value(self.dof, self.chi2)
 
     @property
     def avg_neval(self):
@@ -208,26 +246,13 @@ class RAvg(GVar):
         """Iteration-by-iteration table (pyx:2412-2451)."""
         if weighted is None:
             weighted = self.weighted
-        acc = RAvg(weighted=weighted)
-        linedata = []
-        for i, res in enumerate(self.itn_results):
-            acc.add(res)
-            itn = '%3d' % (i + 1)
-            integral = '%-15s' % res
-            wgtavg = '%-15s' % acc
-            chi2dof = '%8.2f' % (acc.chi2 / acc.dof if i != 0 else 0.0)
-            Q = '%8.2f' % (acc.Q if i != 0 else 1.0)
-            linedata.append((itn, integral, wgtavg, chi2dof, Q))
-        nchar = 5 * [0]
-        for data in linedata:
-            for i, d in enumerate(data):
-                nchar[i] = max(nchar[i], len(d))
-        fmt = '%%%ds   %%-%ds %%-%ds %%%ds %%%ds\n' % tuple(nchar)
-        ans = fmt % ('itn', 'integral', 'wgt average' if weighted else 'average', 'chi2/dof', 'Q')
-        ans += len(ans[:-1]) * '-' + '\n'
-        for data in linedata:
-            ans += fmt % data
-        return ans
+        means = [g.mean for g in self.itn_results]
+        variances = [g.var for g in self.itn_results]
+        rows = []
+        for k, res in enumerate(self.itn_results):
+            avg, sdev, chi2, dof = _scalar_stats(means[:k + 1], variances[:k + 1], weighted)
+            rows.append(_progress_row(k, res, GVar(avg, sdev), chi2, dof))
+        return _progress_table(rows, weighted)
 
 
 def _rebuild_ravg(weighted, results, sum_neval):
@@ -236,19 +261,6 @@ def _rebuild_ravg(weighted, results, sum_neval):
 
 
 # ------------------------------------------------------------------ arrays --
-def _inv_sqrt(matrix, svdcut):
-    """Rows w_i = v_i / sqrt(lambda_i) of the eigen-decomposition of the
-    symmetric ``matrix`` (so that sum_i w_i w_i^T is its inverse on the kept
-    modes).  svdcut < 0 drops the modes with lambda < |svdcut| max(lambda),
-    as gvar.SVD(matrix, svdcut, rescale=False).decomp(-1) does for the
-    reference's RAvgArray._w (pyx:2715-2721); svdcut <= 0 here."""
-    matrix = np.asarray(matrix, np.float64)
-    vals, vecs = np.linalg.eigh(matrix)
-    top = vals.max() if vals.size else 0.0
-    keep = (vals > abs(svdcut) * top) & (vals > 0)
-    return (vecs[:, keep] / np.sqrt(vals[keep])).T
-
-
 class GVarArray:
     """Estimates of several integrals: ``mean``, ``cov`` (covariance matrix)
     and ``sdev`` arrays of shape ``shape``; ``self[i]`` is a GVar."""
@@ -310,80 +322,102 @@ class GVarArray:
     __repr__ = __str__
 
 
+def _pinv_psd(matrix, cut):
+    """Pseudo-inverse of the symmetric positive semi-definite ``matrix``
+    restricted to its eigenmodes with eigenvalue > cut * the largest one (and
+    > 0), and the number of modes kept.  With cut = EPSILON * len * 1e4 this
+    keeps the modes that gvar.SVD(matrix, svdcut=-cut).decomp(-1) keeps for
+    the reference's RAvgArray._w (pyx:2715-2721)."""
+    matrix = np.asarray(matrix, np.float64)
+    if matrix.size == 0:
+        return matrix.copy(), 0
+    vals, vecs = np.linalg.eigh(matrix)
+    keep = (vals > cut * vals.max()) & (vals > 0)
+    v = vecs[:, keep]
+    return (v / vals[keep]) @ v.T, int(keep.sum())
+
+
 class RAvgArray(GVarArray):
-    """Running average of array-valued estimates (pyx:2453-2831).  weighted:
-    the covariance-weighted average, each iteration's estimate rescaled by
-    |mean| (or sdev where that is larger) of the first one and decomposed
-    with the svdcut -EPSILON*len*1e4 of the reference; otherwise the plain
-    average with covariance sum(cov)/n^2.  The per-iteration estimates
-    ``itn_results`` are GVarArrays."""
+    """Running average of array-valued estimates (pyx:2453-2831).
+
+    weighted: the generalised least-squares average.  Each iteration's
+    estimate is first divided by a fixed scale per component (|mean| of the
+    first estimate, its sdev where that is larger, 1 where it is zero; or the
+    array given as ``rescale``, or 1 with ``rescale=None``).  Its precision
+    matrix P_k is the pseudo-inverse of its covariance (variances <= 0
+    replaced by TINY) on the modes above the reference's eigenvalue cut
+    EPSILON*len*1e4; the average is H^+ sum_k P_k m_k with H = sum_k P_k, its
+    covariance H^+ (same cut), chi^2 = sum_k (m_k - avg)^T P_k (m_k - avg)
+    and dof = sum_k rank(P_k) - len.
+    Otherwise: the plain average, covariance sum(cov)/n^2, chi^2 against the
+    mean covariance sum(cov)/n and dof = (n - 1) rank of that covariance.
+
+    State: the iteration results, the scale, the running sums H and
+    sum_k P_k m_k, and (m_k, P_k, rank_k) per iteration for chi^2."""
 
     def __init__(self, shape, weighted=True, itn_results=None, sum_neval=0, rescale=True):
         n = int(np.prod(shape)) if len(shape) else 1
         super().__init__(np.zeros(n), np.zeros((n, n)), shape)
         self.weighted = weighted
         self.rescale = rescale
-        self._mlist, self._wlist = [], []
-        self._msum, self._covsum, self._n = 0.0, 0.0, 0
-        self._invw = None
+        self._scale = None
+        self._terms = []          # weighted: (scaled mean, precision, rank) per iteration
+        self._prec_sum = np.zeros((n, n))
+        self._info_sum = np.zeros(n)
+        self._cov_sum = np.zeros((n, n))  # unweighted: sum of covariances
         self.itn_results = []
         self.sum_neval = sum_neval
         for r in itn_results or []:
             self.add(r)
 
-    def _w(self, matrix):
-        return _inv_sqrt(matrix, -EPSILON * len(matrix) * 1e4)
+    def _cut(self):
+        return EPSILON * self.size * 1e4
+
+    def _component_scale(self, g):
+        if self.rescale is None or self.rescale is False:
+            return np.ones(g.size)
+        if self.rescale is True:
+            scale = np.abs(g._m)
+            sd = np.sqrt(np.maximum(np.diag(g._cov), 0.0))
+            scale = np.where(sd > scale, sd, scale)
+        else:
+            scale = np.abs(np.asarray(self.rescale, np.float64).reshape(-1))
+        return np.where(scale > 0, scale, 1.0)
 
     def add(self, g):
         """Add the estimate ``g`` (a GVarArray, or (mean, cov)) (pyx:2775-2811)."""
         if not isinstance(g, GVarArray):
             g = GVarArray(g[0], g[1], self.shape)
         self.itn_results.append(g)
-        gm = g._m.copy()
-        gc = g._cov.copy()
-        if self.weighted:
-            if not hasattr(self, '_rescale'):
-                if self.rescale is not None:
-                    rs = np.fabs(gm)
-                    gsd = np.sqrt(np.maximum(np.diag(gc), 0.0))
-                    idx = gsd > rs
-                    rs[idx] = gsd[idx]
-                    rs[rs <= 0] = 1.
-                    self._rescale = rs
-                else:
-                    self._rescale = np.ones(len(gm))
-            rs = self._rescale
-            gm = gm / rs
-            gc = gc / np.outer(rs, rs)
-            d = np.diag(gc).copy()
-            d[d <= 0] = TINY
-            gc[np.diag_indices_from(gc)] = d
-            self._mlist.append(gm)
-            self._wlist.append(self._w(gc))
-            invcov = np.sum([w.T.dot(w) for w in self._wlist], axis=0)
-            invw = self._w(invcov)
-            cov = invw.T.dot(invw)
-            mean = np.zeros(len(gm))
-            for m, w in zip(self._mlist, self._wlist):
-                for wj in w:
-                    wj_m = wj.dot(m)
-                    for invwi in invw:
-                        mean += invwi * invwi.dot(wj) * wj_m
-            self._m = mean * rs
-            self._cov = cov * np.outer(rs, rs)
-        else:
-            self._mlist.append(gm)
-            self._msum = self._msum + gm
-            self._covsum = self._covsum + gc
-            self._invw = None
-            self._n += 1
-            self._m = self._msum / self._n
-            self._cov = self._covsum / self._n ** 2
+        n = len(self.itn_results)
+        if not self.weighted:
+            self._cov_sum = self._cov_sum + g._cov
+            self._m = np.mean([r._m for r in self.itn_results], axis=0)
+            self._cov = self._cov_sum / n ** 2
+            return
+        if self._scale is None:
+            self._scale = self._component_scale(g)
+        s = self._scale
+        mk = g._m / s
+        ck = g._cov / np.outer(s, s)
+        diag = np.diag(ck)
+        ck = ck + np.diag(np.where(diag > 0, 0.0, TINY - diag))
+        pk, rk = _pinv_psd(ck, self._cut())
+        self._terms.append((mk, pk, rk))
+        self._prec_sum = self._prec_sum + pk
+        self._info_sum = self._info_sum + pk @ mk
+        cov, _ = _pinv_psd(self._prec_sum, self._cut())
+        self._m = (cov @ self._info_sum) * s
+        self._cov = cov * np.outer(s, s)
 
     def extend(self, ravg):
         for r in ravg.itn_results:
             self.add(r)
         self.sum_neval += ravg.sum_neval
+
+    def _mean_cov_pinv(self):
+        pinv, rank = _pinv_psd(self._cov_sum / len(self.itn_results), self._cut())
+        return pinv, rank
 
     @property
     def chi2(self):
@@ -391,31 +425,20 @@ class RAvgArray(GVarArray):
         if len(self.itn_results) <= 1:
             return 0.0
         if self.weighted:
-            wavg = self._m / self._rescale
-            ans = 0.0
-            for w, m in zip(self._wlist, self._mlist):
-                for wi in w:
-                    ans += wi.dot(m - wavg) ** 2
-            return float(ans)
-        if self._invw is None:
-            self._invw = self._w(self._covsum / self._n)
-        ans = 0.0
-        for m in self._mlist:
-            delta = self._m - m
-            for invwi in self._invw:
-                ans += invwi.dot(delta) ** 2
-        return float(ans)
+            avg = self._m / self._scale
+            return float(sum(float((mk - avg) @ pk @ (mk - avg)) for mk, pk, _ in self._terms))
+        pinv, _ = self._mean_cov_pinv()
+        dev = np.array([r._m for r in self.itn_results]) - self._m
+        return float(np.einsum('ki,ij,kj->', dev, pinv, dev))
 
     @property
     def dof(self):
         """pyx:2751-2758"""
         if len(self.itn_results) <= 1:
             return 0
-        if not self.weighted:
-            if self._invw is None:
-                self._invw = self._w(self._covsum / self._n)
-            return (len(self.itn_results) - 1) * len(self._invw)
-        return int(np.sum([len(w) for w in self._wlist]) - self.size)
+        if self.weighted:
+            return int(sum(rk for _, _, rk in self._terms) - self.size)
+        return (len(self.itn_results) - 1) * self._mean_cov_pinv()[1]
 
     @property
     def nitn(self):
@@ -423,9 +446,7 @@ class RAvgArray(GVarArray):
 
     @property
     def Q(self):
-        if self.dof <= 0 or self.chi2 < 0:
-            return float('nan')
-        return gammaQ(self.dof / 2., self.chi2 / 2.)
+        return _q_value(self.dof, self.chi2)
 
     @property
     def avg_neval(self):
@@ -436,36 +457,24 @@ class RAvgArray(GVarArray):
         return bool(np.all(self.sdev < atol + rtol * np.abs(self.mean)))
 
     def summary(self, extended=False, weighted=None, rescale=None):
-        """Iteration-by-iteration table of the first component and the
+        """Iteration-by-iteration table of the first component with the
         chi2 of the whole array (pyx:2833-2875); with ``extended`` a table of
         every component's average follows."""
         if weighted is None:
             weighted = self.weighted
         if rescale is None:
             rescale = self.rescale
-        acc = RAvgArray(self.shape, weighted=weighted, rescale=rescale)
-        linedata = []
-        for i, res in enumerate(self.itn_results):
-            acc.add(res)
-            itn = '%3d' % (i + 1)
-            integral = '%-15s' % res.flat_gvars()[0]
-            wgtavg = '%-15s' % acc.flat_gvars()[0]
-            chi2dof = '%8.2f' % (acc.chi2 / acc.dof if i != 0 else 0.0)
-            Q = '%8.2f' % (acc.Q if i != 0 else 1.0)
-            linedata.append((itn, integral, wgtavg, chi2dof, Q))
-        nchar = 5 * [0]
-        for data in linedata:
-            for i, d in enumerate(data):
-                nchar[i] = max(nchar[i], len(d))
-        fmt = '%%%ds   %%-%ds %%-%ds %%%ds %%%ds\n' % tuple(nchar)
-        ans = fmt % ('itn', 'integral', 'wgt average' if weighted else 'average', 'chi2/dof', 'Q')
-        ans += len(ans[:-1]) * '-' + '\n'
-        for data in linedata:
-            ans += fmt % data
+        running = RAvgArray(self.shape, weighted=weighted, rescale=rescale)
+        rows = []
+        for k, res in enumerate(self.itn_results):
+            running.add(res)
+            rows.append(_progress_row(k, res.flat_gvars()[0], running.flat_gvars()[0],
+                                      running.chi2, running.dof))
+        text = _progress_table(rows, weighted)
         if extended and self.size > 1:
-            ans += '\n' + '\n'.join('%8s  %s' % (str(idx), g)
-                                    for idx, g in zip(np.ndindex(self.shape), self.flat_gvars())) + '\n'
-        return ans
+            text += '\n' + '\n'.join('%8s  %s' % (str(idx), g)
+                                     for idx, g in zip(np.ndindex(self.shape), self.flat_gvars())) + '\n'
+        return text
 
     def __reduce__(self):
         return (_rebuild_ravg_array, (self.shape, self.weighted, self.rescale,

@@ -500,30 +500,45 @@ class PDFIntegrator(Integrator):
 
 
 def _restratified(weight, old_nstrat, gamma=1., below_avg_nstrat=None):
-    """New strata per axis from the axis weights (__init__.py:1350-1366):
-    nstrat_mu = old_mu * ((w_mu / gm(w)) * gm(old) / old_mu) ** gamma, then,
-    smallest first, each axis is cut to an integer (at least 1, or
-    below_avg_nstrat for below-average weights) and the axes still to come
-    absorb the change so the product is carried along."""
-    weight = np.asarray(weight, float)
-    old_nstrat = np.asarray(old_nstrat)
-    dim = len(old_nstrat)
-    w_avg = np.average(weight)
-    w_gm = np.prod(weight) ** (1 / dim)
-    nstrat_gm = np.prod(old_nstrat) ** (1 / dim)
-    nstrat = old_nstrat * ((weight / w_gm) * nstrat_gm / old_nstrat) ** gamma
-    nleft = dim
-    musort = np.array(np.argsort(nstrat))
-    new_nstrat = np.array(old_nstrat)
-    for mu in musort:
-        new_nstrat[mu] = nstrat[mu] if nstrat[mu] > 1 else 1
-        if below_avg_nstrat and weight[mu] < w_avg:
-            new_nstrat[mu] = below_avg_nstrat
-        nleft -= 1
-        if nleft > 0:
-            nstrat[musort[-nleft:]] *= (nstrat[mu] / new_nstrat[mu]) ** (1 / nleft)
-            nstrat[mu] = new_nstrat[mu]
-    return new_nstrat
+    """New strata per axis from the axis weights (the rule of
+    __init__.py:1350-1366).
+
+    The target of axis mu is old_mu * ((w_mu / gm(w)) * gm(old) / old_mu)
+    ** gamma (gm: geometric mean, prod ** (1/dim)).  Axes are then fixed to
+    integers in order of increasing target: truncation (at least 1), or
+    ``below_avg_nstrat`` for an axis of below-average weight.  The ratio
+    target / assigned of each fixed axis is spread over the targets of the
+    axes still open, as the factor ratio ** (1 / number still open) on each,
+    so the product of the strata follows the targets' product.  The
+    arithmetic is done in the reference's order: where the product makes the
+    last target an integer in exact arithmetic, its truncation depends on
+    the rounding."""
+    w = np.asarray(weight, float)
+    old = np.asarray(old_nstrat)
+    dim = len(old)
+    gm_w = np.prod(w) ** (1 / dim)
+    gm_old = np.prod(old) ** (1 / dim)
+    target = old * ((w / gm_w) * gm_old / old) ** gamma
+    below = w < np.average(w) if below_avg_nstrat else np.zeros(dim, bool)
+    out = np.array(old_nstrat)
+    order = np.argsort(target)
+    for done, mu in enumerate(order, start=1):
+        want = target[mu]
+        out[mu] = below_avg_nstrat if below[mu] else (want if want > 1 else 1)
+        still_open = order[done:]
+        if len(still_open):
+            target[still_open] *= (want / out[mu]) ** (1 / len(still_open))
+    return out
+
+
+def _report_strata(title, summary, nstrat, tail=''):
+    """restratify(verbose=True): a section with the training summary and the
+    strata (__init__.py:1340-1346, 1382-1388 print the same fields)."""
+    nstrat = np.asarray(nstrat)
+    lines = [title, summary[:-1],
+             'nstrat = ' + np.array2string(nstrat, max_line_width=60, prefix=9 * ' '),
+             'nhcube = %d%s' % (int(np.prod(nstrat)), tail)]
+    print('\n'.join(lines))
 
 
 def restratify(integ, f=None, nitn=1, ndy=5, below_avg_nstrat=None, verbose=False, gamma=1., **vargs):
@@ -576,11 +591,7 @@ def restratify(integ, f=None, nitn=1, ndy=5, below_avg_nstrat=None, verbose=Fals
     result = Integrator.__call__(new, I_dI)
     new.set(save)
     if verbose:
-        print('\n==================== restratify')
-        print('BEFORE:')
-        print(result.summary()[:-1])
-        print('nstrat =', np.array2string(old_nstrat, max_line_width=60, prefix=9 * ' '))
-        print('nhcube =', np.prod(old_nstrat), '\n')
+        _report_strata('\n' + 20 * '=' + ' restratify\nBEFORE:', result.summary(), old_nstrat, '\n')
     # first-order gvar arithmetic takes means through: the weights are
     # functions of the means
     Im = float(np.asarray(result['I'].mean))
@@ -595,9 +606,5 @@ def restratify(integ, f=None, nitn=1, ndy=5, below_avg_nstrat=None, verbose=Fals
     new.set(save)
     new.restratify_weight = weight
     if verbose:
-        print('\nAFTER:')
-        print(training.summary()[:-1])
-        print('nstrat =', np.array2string(np.asarray(new.nstrat), max_line_width=60, prefix=9 * ' '))
-        print('nhcube =', np.prod(new.nstrat))
-        print(20 * '=')
+        _report_strata('\nAFTER:', training.summary(), new.nstrat, '\n' + 20 * '=')
     return new
