@@ -24,7 +24,7 @@ import math
 
 import numpy as np
 
-from ._ravg import GVar, GVarArray
+from ._ravg import GVar, GVarArray, RAvgDict
 from .integrator import Integrator
 from .adaptive_map import AdaptiveMap
 from .batchintegrand import lbatchintegrand
@@ -151,6 +151,14 @@ class PDFEVDict(dict):
         return getattr(self.results, k)
 
 
+def _view(g, a, b, shape):
+    """Entries a..b of the flat GVarArray g as a GVar (shape ()) or a
+    GVarArray of the given shape with their covariance block."""
+    if shape == ():
+        return GVar(g._m[a], math.sqrt(max(g._cov[a, a], 0.0)))
+    return GVarArray(g._m[a:b], g._cov[a:b, a:b], shape)
+
+
 def _ratio(mean, cov, ipdf):
     """E = F / P and its covariance to first order, from the means and
     covariance of (P, F_1..F_m) with P at index ipdf."""
@@ -265,7 +273,7 @@ class PDFIntegrator(Integrator):
         self.svdcut = svdcut
         self.pdf = pdf
         self.adapt_to_pdf = adapt_to_pdf
-        grid = self._make_map(self.limit / self.scale)
+        grid = self._make_map(self.limit / self.scale, self.scale)
         kargs = {k: v for k, v in kargs.items() if k != 'uses_jac'}
         super().__init__(AdaptiveMap(self.param_pdf.size * [list(grid)]), **kargs)
 
@@ -289,10 +297,11 @@ class PDFIntegrator(Integrator):
         return old
 
     @staticmethod
-    def _make_map(limit):
-        """Grid nodes in theta = arctan(chiv) adapted to the unit Gaussian
-        (__init__.py:576-592: 2000 points, 100 increments, 10 passes of
-        alpha 1.5)."""
+    def _make_map(limit, scale=1.0):
+        """Grid nodes in theta = arctan(chiv / scale) adapted to the unit
+        Gaussian in chiv = scale * tan(theta) (__init__.py:576-592: 2000
+        points, 100 increments, 10 passes of alpha 1.5; the training weight
+        is dchiv/dtheta / scale times exp(-chiv^2 / 2))."""
         ny = 2000
         y = np.random.default_rng(20240601).random((ny, 1))
         lim = math.atan(limit)
@@ -302,7 +311,8 @@ class PDFIntegrator(Integrator):
             theta = theta.cpu().numpy()[:, 0]
             jac = jac.cpu().numpy()
             t = np.tan(theta)
-            fx = (t ** 2 + 1) * np.exp(-(t ** 2) / 2.)
+            chiv = scale * t
+            fx = (t ** 2 + 1) * np.exp(-(chiv ** 2) / 2.)
             m.add_training_data(y, (jac * fx) ** 2)
             m.adapt(alpha=1.5)
         return np.array(m.extract_grid()[0])
@@ -324,8 +334,8 @@ class PDFIntegrator(Integrator):
 
     def __call__(self, f=None, save=None, saveall=None, **kargs):
         """E[f(p)] over the PDF (__init__.py:632-718): a PDFEV, PDFEVArray or
-        PDFEVDict; with f=None the integral of the PDF alone (an RAvgArray
-        with ``pdfnorm``)."""
+        PDFEVDict; with f=None the integral of the PDF alone (a running
+        average keyed 'pdf', as the reference's RAvgDict, with ``pdfnorm``)."""
         kargs.pop('uses_jac', None)
         if kargs:
             self.set(kargs)
@@ -348,8 +358,12 @@ class PDFIntegrator(Integrator):
         if pdfnorm.mean == 0:
             raise RuntimeError('Integral of PDF vanishes; increase neval?')
         if f is None:
-            results.pdfnorm = pdfnorm
-            return results
+            # the reference returns its RAvgDict of the one integral 'pdf'
+            ans = RAvgDict([('pdf', (), 0, 1)], weighted=results.weighted)
+            ans.rarray = results
+            ans._refresh()
+            ans.pdfnorm = pdfnorm
+            return ans
         mean, cov = _ratio(np.asarray(results.mean, float).reshape(-1), results.cov, ipdf)
         kind, lay = layout['l']
         if kind == 'array':
@@ -369,11 +383,15 @@ class PDFIntegrator(Integrator):
         (__init__.py:729-946): E[f] and E[f f^T] - E[f] E[f]^T from one
         integration, plus the integration's own covariance of E[f]
         (__init__.py:965-968).  f=None means f(p) = p.  The grid is not
-        adapted unless ``adapt`` is given (__init__.py:871-874).  With
-        ``moments=True`` or ``histograms`` the result also carries
-        ``stats``: a :class:`PDFStatistics` per component of f
-        (__init__.py:999-1010), from E[f], E[f^2], E[f^3], E[f^4] and the
-        probability of each histogram bin, all from the same integration.
+        adapted unless ``adapt`` is given (__init__.py:871-874).  The result
+        (PDFEV, PDFEVArray or PDFEVDict, with ``results`` and ``pdfnorm``)
+        carries, as the reference's does (__init__.py:948-1017):
+        ``stats``, a :class:`PDFStatistics` per component of f from E[f] and
+        E[f^2] (and E[f^3], E[f^4] with ``moments=True``, and the probability
+        of each histogram bin with ``histograms``), all from the same
+        integration; ``vegas_mean`` (E[f] with its integration error),
+        ``vegas_cov`` (the covariance of f, with the error propagated from
+        the integrals of f and f f^T) and ``vegas_sdev``.
         ``histograms`` is True or a dict with ``nbin`` (12), ``binwidth``
         (0.5) and ``loc`` = (mean, sdev) arrays of the histogram centres and
         widths; without ``loc`` they come from a one-iteration integration
@@ -448,33 +466,72 @@ class PDFIntegrator(Integrator):
         second[il] = m[k:k + len(il[0])]
         second = second + np.tril(second, -1).T
         cov = second - np.outer(mu, mu) + C[:k, :k]
+        # vegas_cov: the covariance of f under the PDF as estimated, with its
+        # first-order error from the integration covariance of (E[f],
+        # E[f f^T]); vegas_sdev its diagonal's square root
+        nt = len(il[0])
+        tri = np.zeros((k, k), int)
+        tri[il] = np.arange(nt)
+        tri = np.maximum(tri, tri.T)
+        jac = np.zeros((k * k, k + nt))
+        for i in range(k):
+            for j in range(k):
+                r = i * k + j
+                jac[r, k + tri[i, j]] += 1.0
+                jac[r, i] -= mu[j]
+                jac[r, j] -= mu[i]
+        cov_err = jac @ C[:k + nt, :k + nt] @ jac.T
+        dg = np.arange(k) * (k + 1)
+        sdev_val = np.sqrt(np.fabs(np.diag(cov)))
+        with np.errstate(divide='ignore', invalid='ignore'):
+            jsd = np.where(sdev_val > 0, np.sign(np.diag(cov)) / (2 * sdev_val), 0.0)
+        sdev_err = jsd[:, None] * cov_err[np.ix_(dg, dg)] * jsd[None, :]
+        vmean = GVarArray(mu, C[:k, :k], (k,))
+        vcov = GVarArray(cov.reshape(-1), cov_err, (k, k))
+        vsdev = GVarArray(sdev_val, sdev_err, (k,))
+        # PDFStatistics per component: always from the first two moments,
+        # plus the third and fourth with moments=True, and the histogram
+        n2 = k + nt
+        n4 = n2 + (2 * k if moments else 0)
+        st = []
+        for i in range(k):
+            idx = [i, k + i * (i + 1) // 2 + i] + ([n2 + i, n2 + k + i] if moments else [])
+            h = None
+            if hist is not None:
+                nb2 = hist['nbin'] + 2
+                hi = np.arange(n4 + i * nb2, n4 + (i + 1) * nb2)
+                h = (hist['bins'][i], m[hi], np.sqrt(np.maximum(np.diag(C)[hi], 0.0)))
+            st.append(PDFStatistics(m[idx], C[np.ix_(idx, idx)], histogram=h))
+        results, pdfnorm = ev.results, ev.pdfnorm
         if kind == 'dict':
-            items = {}
+            items, vm, vs, vc = {}, {}, {}, {}
             for key, shp, a, b in lay:
                 items[key] = (GVar(mu[a], math.sqrt(max(cov[a, a], 0.0))) if shp == ()
                               else GVarArray(mu[a:b], cov[a:b, a:b], shp))
-            ans = PDFEVDict(items, cov)
+                vm[key] = _view(vmean, a, b, shp)
+                vs[key] = _view(vsdev, a, b, shp)
+                for key2, shp2, a2, b2 in lay:
+                    rows = (np.arange(a, b)[:, None] * k + np.arange(a2, b2)[None, :]).reshape(-1)
+                    blk = (b - a, b2 - a2) if (shp, shp2) != ((), ()) else ()
+                    vc[key, key2] = (GVar(cov[a, a2], math.sqrt(max(cov_err[rows[0], rows[0]], 0.0)))
+                                     if blk == () else
+                                     GVarArray(cov[a:b, a2:b2].reshape(-1), cov_err[np.ix_(rows, rows)], blk))
+            ans = PDFEVDict(items, cov, results, pdfnorm)
+            ans.stats = {key: (st[a] if shp == () else np.array(st[a:b], dtype=object).reshape(shp))
+                         for key, shp, a, b in lay}
+            ans.vegas_mean, ans.vegas_cov, ans.vegas_sdev = vm, vc, vs
         elif shape == ():
-            ans = GVar(mu[0], math.sqrt(max(cov[0, 0], 0.0)))
+            ans = PDFEV(mu[0], math.sqrt(max(cov[0, 0], 0.0)), results, pdfnorm)
+            ans.stats = st[0]
+            ans.vegas_mean = GVar(mu[0], math.sqrt(max(C[0, 0], 0.0)))
+            ans.vegas_cov = GVar(cov[0, 0], math.sqrt(max(cov_err[0, 0], 0.0)))
+            ans.vegas_sdev = GVar(sdev_val[0], math.sqrt(max(sdev_err[0, 0], 0.0)))
         else:
-            ans = GVarArray(mu, cov, shape)
-        if moments or hist is not None:
-            n2 = k + len(il[0])
-            n4 = n2 + (2 * k if moments else 0)
-            st = []
-            for i in range(k):
-                idx = [i, k + i * (i + 1) // 2 + i] + ([n2 + i, n2 + k + i] if moments else [])
-                h = None
-                if hist is not None:
-                    nb2 = hist['nbin'] + 2
-                    hi = np.arange(n4 + i * nb2, n4 + (i + 1) * nb2)
-                    h = (hist['bins'][i], m[hi], np.sqrt(np.maximum(np.diag(C)[hi], 0.0)))
-                st.append(PDFStatistics(m[idx], C[np.ix_(idx, idx)], histogram=h))
-            if kind == 'dict':
-                ans.stats = {key: (st[a] if shp == () else np.array(st[a:b], dtype=object).reshape(shp))
-                             for key, shp, a, b in lay}
-            else:
-                ans.stats = st[0] if shape == () else np.array(st, dtype=object).reshape(shape)
+            ans = PDFEVArray(mu, cov, shape, results, pdfnorm)
+            ans.stats = np.array(st, dtype=object).reshape(shape)
+            ans.vegas_mean = GVarArray(mu, C[:k, :k], shape)
+            ans.vegas_cov = GVarArray(cov.reshape(-1), cov_err, shape + shape)
+            ans.vegas_sdev = GVarArray(sdev_val, sdev_err, shape)
         return ans
 
     def sample(self, nbatch, mode='rbatch'):
