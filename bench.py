@@ -27,6 +27,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "integrand evals/s; GPU wall-time to 1e-4 rel err (vegas grid vs learned map)"
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0   # dense MFMA
+PEAK_HBM_GBPS = 8000.0
+TRAIN_REC_BYTES_8D = 24     # training record per evaluation at 8-D: 8 u16 bins + (w f n)^2
+# algorithmic flops per sample and coupling layer of the flow (MLP D -> 128 -> 128 -> 2D)
+def flow_layer_flops(d):
+    return 2 * (d * 128 + 128 * 128 + 128 * 2 * d)
 
 
 def parse():
@@ -39,7 +45,8 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 512 MiB device to device 4 times (HBM counter calibration)")
     p.add_argument("--workload", default="double_peak_8d",
-                   choices=["double_peak_8d", "ridge_8d", "batched_4096", "flow_ridge_9d", "moments_8d"],
+                   choices=["double_peak_8d", "ridge_8d", "batched_4096", "flow_ridge_9d", "flow_ridge_8d",
+                            "moments_8d"],
                    help="double_peak_8d (default): the headline; ridge_8d: 1000-Gaussian diagonal "
                         "ridge, neval 2e7 (diagonal-ridge and generic mixture functors); batched_4096: "
                         "4096 independent Gaussian integrals in one graph, nitn 8 per call (--steps "
@@ -78,6 +85,18 @@ def batched(a):
     r = bi(A, B, cs, nitn=8)
     evals = float(r.sum_neval)  # per call (the timed calls draw the same counts to within the allocation)
     pull = (r.mean - exact) / np.where(r.sdev > 0, r.sdev, np.inf)
+    # the sampler's launch time: one more call of the same kernels launched
+    # directly (not through the graph) with HIP events around each sampler
+    # launch; algorithmic work per launch = sum_i n_i (12 d_i + 13) flops
+    from vegas_amd import _lib, workloads as wls
+    L, cx = _lib.lib(), _lib.ctx()
+    L.vgx_set_timing(cx, 1)
+    rt = bi(A, B, cs, nitn=8)
+    bms = kernel_ms("b_fused_kernel")
+    L.vgx_set_timing(cx, 0)
+    fpe_i = np.array([wls.flops_per_eval(d, 1) for d in dims], dtype=np.float64)
+    flops_it = (np.asarray(rt.itn_neval, np.float64) * fpe_i[None, :]).sum(axis=1)
+    achieved = (float(flops_it.sum()) / (sum(bms) * 1e-3) / 1e12) if len(bms) == len(flops_it) else None
     # plain vegas+ misses some narrow high-dimensional peaks at 1e4 samples
     # (as the reference does): those return ~0 with a tiny error
     resolved = r.mean > 0.5 * exact
@@ -92,9 +111,43 @@ def batched(a):
         "result": {"integrals": len(exact), "resolved": int(resolved.sum()),
                    "resolved_within_5sigma": int((np.abs(pull[resolved]) < 5).sum()),
                    "median_rel_sdev_resolved": float(np.median(r.sdev[resolved] / exact[resolved]))},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None, "traffic": None,
+                     "kernel": "b_fused_kernel<true>", "kernel_ms_avg": (sum(bms) / len(bms)) if bms else None,
+                     "flops_per_eval": "12 d + 13 (d = the integral's dimension)",
+                     "note": "sampler launches of one extra call with the kernels launched directly and "
+                             "event-timed (8 launches, one per iteration); peak is the FP32 vector peak"},
+        "cpu_baseline": None if a.no_cpu else batched_cpu(probs, A, B),
     }
     print(json.dumps(out))
+
+
+def batched_cpu(probs, A, B, stride=64, nitn=8):
+    """The FP64 C oracle on a sample of the batched workload: every
+    stride-th integral (64 of 4096) for nitn iterations at its own neval
+    (maxinc_axis 500), the sample's integrals spread over the host cores."""
+    from oracle import oracle as orc
+    pick = list(range(0, len(probs), stride))
+    ncores = _ncores()
+    groups = [pick[k::ncores] for k in range(ncores)]
+
+    def job(ids):
+        def run():
+            tot = 0.0
+            for i in ids:
+                p = probs[i]
+                g = orc.GaussMix([A[i]], [B[i]], [p["center"]])
+                I = orc.OracleIntegrator([(0, 1)] * p["dim"], neval=p["neval"], seed=i + 1, maxinc_axis=500)
+                _, _, nev = I.run(g, nitn)
+                tot += float(nev.sum())
+            return tot
+        return run
+
+    tot, dt = _cpu_runs([job(gr) for gr in groups if gr])
+    return dict(value=tot / dt, unit="evals/s", cores=ncores, kind="port",
+                sample="%d of the 4096 integrals (every %dth), %d iterations each at their own neval, "
+                       "through the FP64 C oracle on %d threads: %.3g evals in %.1f s"
+                       % (len(pick), stride, nitn, ncores, tot, dt))
 
 
 def copy_calibration():
@@ -110,50 +163,71 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": n, "dispatches": 4}))
 
 
-def cpu_baseline(wl, iters):
-    """Reference-structured FP64 oracle on `iters` iterations of the workload
-    (integrand inlined in C): one thread alone, then one independent run per
-    host core (up to 16, the box's CPU share) side by side; the C call runs
-    without the GIL.  value is the all-core rate."""
+def _ncores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))  # the box's CPU share
+
+
+def _cpu_runs(jobs):
+    """Run the oracle jobs (callables returning an evaluation count) one per
+    host core side by side (the C calls run without the GIL); returns the
+    total count and the wall time.  Any failure fails the baseline loudly:
+    a partial count would misstate the rate."""
     import threading
-    from oracle import oracle as orc
-    f = wl["f"]
-    g = orc.GaussMix(f.A, f.B, f.centers)
+    errors, outs = [], []
 
-    errors = []
-
-    def one(seed, out):
+    def run(job):
         try:
-            I = orc.OracleIntegrator([(0, 1)] * wl["dim"], neval=wl["neval"], seed=seed)
-            _, _, nev = I.run(g, iters)
-            out.append(float(nev.sum()))
-        except Exception as e:  # reported below: a partial count would misstate the rate
+            outs.append(float(job()))
+        except Exception as e:
             errors.append(e)
 
-    solo = []
-    t0 = time.perf_counter()
-    one(1, solo)
-    dt1 = time.perf_counter() - t0
-    try:
-        ncores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        ncores = os.cpu_count() or 1
-    ncores = max(1, min(ncores, 16))
-    outs = []
-    ths = [threading.Thread(target=one, args=(s + 1, outs)) for s in range(ncores)]
+    ths = [threading.Thread(target=run, args=(j,)) for j in jobs]
     t0 = time.perf_counter()
     for t in ths:
         t.start()
     for t in ths:
         t.join()
-    dtn = time.perf_counter() - t0
-    if errors or len(outs) != ncores:
+    dt = time.perf_counter() - t0
+    if errors or len(outs) != len(jobs):
         raise RuntimeError("CPU baseline run failed: %r" % (errors[:1],))
-    return dict(value=sum(outs) / dtn, unit="evals/s", cores=ncores, kind="port",
-                value_1core=solo[0] / dt1,
-                sample="%d iteration(s) of the 8-D double peak, neval 1e7, through the FP64 C oracle: "
-                       "%d independent runs on %d threads in %.1f s (%.3g evals); one thread alone "
-                       "%.3g evals/s" % (iters, ncores, ncores, dtn, sum(outs), solo[0] / dt1))
+    return sum(outs), dt
+
+
+def cpu_oracle_rate(f, dim, neval, iters, what):
+    """The reference-structured FP64 oracle (integrand inlined in C) on
+    `iters` iterations of f at neval: one thread alone, then one independent
+    run per host core (seeds 1..n) side by side.  value is the all-core rate."""
+    from oracle import oracle as orc
+    g = orc.GaussMix(f.A, f.B, f.centers)
+
+    def job(seed):
+        def run():
+            I = orc.OracleIntegrator([(0, 1)] * dim, neval=neval, seed=seed)
+            _, _, nev = I.run(g, iters)
+            return nev.sum()
+        return run
+
+    n1, dt1 = _cpu_runs([job(1)])
+    ncores = _ncores()
+    tot, dtn = _cpu_runs([job(s + 1) for s in range(ncores)])
+    return dict(value=tot / dtn, unit="evals/s", cores=ncores, kind="port", value_1core=n1 / dt1,
+                sample="%d iteration(s) of %s, neval %.3g, through the FP64 C oracle: %d independent runs on "
+                       "%d threads in %.1f s (%.3g evals); one thread alone %.3g evals/s"
+                       % (iters, what, neval, ncores, ncores, dtn, tot, n1 / dt1))
+
+
+def cpu_baseline(wl, iters):
+    return cpu_oracle_rate(wl["f"], wl["dim"], wl["neval"], iters, "the 8-D double peak")
+
+
+def kernel_ms(name):
+    """Durations (ms) of the recorded launches of one kernel."""
+    from vegas_amd import _lib
+    return [ms for k, ms in _lib.timed_launches() if k == name]
 
 
 def ridge(a):
@@ -198,13 +272,15 @@ def ridge(a):
                                "neval 2e7/iteration", "dim": 8, "neval": 20000000,
                    "nstrat": d["nstrat"], "nhcube": d["nhcube"], "parallelism": "single GPU"},
         "result": out_r,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None, "traffic": None,
                      "kernel": "fused_kernel<8, true, DiagRidge32>", "kernel_ms_avg": d["kernel_ms_avg"],
                      "flops_per_eval": fpe,
                      "note": "algorithmic flops of the 1000-term mixture (the diagonal functor evaluates "
                              "it in about a third of them); peak is the FP32 vector peak"},
-        "cpu_baseline": None,
+        "cpu_baseline": None if a.no_cpu else cpu_oracle_rate(
+            workloads.ridge_8d(diagonal=False)["f"], 8, 1e5, 1,
+            "the 8-D ridge (1000 Gaussians; the per-evaluation rate on a smaller neval than the GPU's 2e7)"),
     }
     print(json.dumps(out))
 
@@ -234,10 +310,20 @@ def moments(a):
     if a.warmup > 0:
         integ(g, nitn=a.warmup)
     torch.cuda.synchronize()
+    from vegas_amd import _lib
+    L, cx = _lib.lib(), _lib.ctx()
+    L.vgx_set_timing(cx, 1)
     t0 = time.perf_counter()
     r = integ(g, nitn=a.steps)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    cov_ms = kernel_ms("cov")
+    L.vgx_set_timing(cx, 0)
+    # the covariance pass reads each point's 4 values and weight once
+    # (algorithmic 40 B per point; the second pass over an hcube is served
+    # from cache)
+    cov_bytes = float(r.sum_neval) * 40.0
+    cov_gbps = cov_bytes / (sum(cov_ms) * 1e-3) / 1e9 if cov_ms else None
     out = {
         "metric": METRIC, "value": float(r.sum_neval) / dt, "unit": "evals/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": dt * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
@@ -247,39 +333,51 @@ def moments(a):
                    "parallelism": "single GPU"},
         "result": {"mean": [float(v) for v in r.mean], "sdev": [float(v) for v in r.sdev],
                    "chi2_dof": r.chi2 / max(r.dof, 1), "Q": r.Q},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "hbm", "achieved": cov_gbps, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": (cov_gbps / PEAK_HBM_GBPS) if cov_gbps else None, "traffic": None,
+                     "kernel": "cov_hcube_all_kernel<4> + cov_reduce_kernel",
+                     "kernel_ms_avg": (sum(cov_ms) / len(cov_ms)) if cov_ms else None,
+                     "launches": len(cov_ms),
+                     "note": "algorithmic bytes 40 per point (4 FP64 values + weight); the torch "
+                             "integrand's own kernels take most of the iteration"},
+        "cpu_baseline": None if a.no_cpu else cpu_oracle_rate(
+            wl["f"], 8, 1e6, 1, "the 8-D double peak, component 0 only (a lower bound on the reference's "
+                                "4-component cost)"),
     }
     print(json.dumps(out))
 
 
 def flow_ridge(a):
-    """GPU wall-time to a relative error rtol on the 9-D ridge (500 diagonal
-    Gaussians, sigma 0.01): the learned map (vegas warm-up, 2e7 vegas
-    samples, flow training, flow sampling) against the plain vegas+ grid
-    (neval 2e7 iterations), which gets 3x the flow's time.  With --warmup > 0
-    one untimed run of the whole pipeline (other seeds) comes first."""
+    """GPU wall-time to a relative error rtol on a non-separable ridge: the
+    9-D ridge (500 diagonal Gaussians, sigma 0.01; flow_ridge_9d) or the 8-D
+    ridge (1000 diagonal Gaussians, sigma 0.005; flow_ridge_8d).  The learned
+    map (vegas warm-up, 2e7 vegas samples, flow training, flow sampling)
+    against the plain vegas+ grid (neval 2e7 iterations), which gets 3x the
+    flow's time.  With --warmup > 0 one untimed run of the whole pipeline
+    (other seeds) comes first."""
     import torch
     import vegas_amd as va
     from vegas_amd import workloads
     from vegas_amd.flow import FlowMap
-    wl = workloads.ridge_9d(diagonal=True)
+    wl = workloads.ridge_8d(diagonal=True) if a.workload == "flow_ridge_8d" else workloads.ridge_9d(diagonal=True)
     f, exact = wl["f"], wl["exact"]
+    D = wl["dim"]
     # warm-up: one whole untimed run (vegas warm-up, training, sampling) with
     # other seeds, so the timed run excludes first-use costs (module loads,
     # allocator growth) as the iteration benches do
     nwarm = min(max(a.warmup, 0), 1)
     for _ in range(nwarm):
-        wi = va.Integrator([[0, 1]] * 9, neval=wl["neval"], seed=11)
+        wi = va.Integrator([[0, 1]] * D, neval=wl["neval"], seed=11)
         wi(f, nitn=10)
-        wm = FlowMap(9, seed=7)
+        wm = FlowMap(D, seed=7)
         wm.fit(f, wi, ntrain=2e7)
         wm.integrate(f, rtol=a.rtol, batch=20000000, max_samples=2e10)
         del wi, wm
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    integ = va.Integrator([[0, 1]] * 9, neval=wl["neval"], seed=5)
+    integ = va.Integrator([[0, 1]] * D, neval=wl["neval"], seed=5)
     integ(f, nitn=10)
-    fm = FlowMap(9, seed=1)
+    fm = FlowMap(D, seed=1)
     fm.fit(f, integ, ntrain=2e7)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
@@ -290,7 +388,7 @@ def flow_ridge(a):
     # plain vegas+: iterations of 2e7 until sdev < rtol |mean| or 3 t_flow
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    vi = va.Integrator([[0, 1]] * 9, neval=wl["neval"], seed=5)
+    vi = va.Integrator([[0, 1]] * D, neval=wl["neval"], seed=5)
     r = None
     nit = 0
     while time.perf_counter() - t1 < 3 * t_flow:
@@ -301,13 +399,34 @@ def flow_ridge(a):
     torch.cuda.synchronize()
     t_vegas = time.perf_counter() - t1
     vegas_ok = r.sdev < a.rtol * abs(r.mean) and abs(r.mean - exact) < 5 * r.sdev
+    # kernel times after the timed region: two more training steps on the
+    # flow's own samples (batches of 2.5e5) with the flow kernels event-timed
+    from vegas_amd import _lib
+    L, cx = _lib.lib(), _lib.ctx()
+    nb = 250000
+    L.vgx_set_timing(cx, 1)
+    fm.train(f, None, None, None, epochs=0, self_batches=2, self_batch=nb, lr=1e-5, loss='variance')
+    torch.cuda.synchronize()
+    tl = _lib.timed_launches()
+    L.vgx_set_timing(cx, 0)
+    kt = {}
+    for k, ms in tl:
+        kt.setdefault(k, []).append(ms)
+    lf = flow_layer_flops(D)
+    work = {"flow_backward_kernel": nb * 2 * lf,   # recomputed MLP + backpropagation per layer
+            "flow_wgrad_kernel": nb * lf,          # weight gradients per layer
+            "flow_sample_kernel": nb * 6 * lf}     # 6 layers forward (the integrand's flops aside)
+    kern = {k: {"ms_avg": sum(v) / len(v), "launches": len(v),
+                "TFLOPs": work[k] / (sum(v) / len(v) * 1e-3) / 1e12} for k, v in kt.items() if k in work}
+    bwd = kern.get("flow_backward_kernel")
     out = {
         "metric": METRIC, "value": t_flow, "unit": "s (GPU wall-time to rel err %g, learned map)" % a.rtol,
         "n_gpus": 1, "steps": 1, "warmup": nwarm, "ms_per_step": t_flow * 1e3, "higher_is_better": False,
         "scaling": "strong", "vs_baseline": None, "dtype": "bf16 (flow MLP), f32 (integrand), f64 (sums)",
         "data": "synthetic (device Philox samples)",
-        "config": {"workload": "9-D ridge: 500 diagonal Gaussians sigma 0.01, [0,1]^9; 6-layer affine coupling "
-                               "flow, MLP 9-128-128 per layer, trained on 2e7 vegas samples in 2.5e5 batches",
+        "config": {"workload": "%s: %d diagonal Gaussians sigma %g, [0,1]^%d; 6-layer affine coupling "
+                               "flow, MLP %d-128-128 per layer, trained on 2e7 vegas samples in 2.5e5 batches"
+                               % (wl["name"], wl["ncomp"], wl["sigma"], D, D),
                    "parallelism": "single GPU"},
         "result": {"flow": {"mean": est.mean, "sdev": est.sdev, "samples": int(nflow), "train_s": t_train,
                             "reached_target": bool(flow_ok)},
@@ -317,7 +436,18 @@ def flow_ridge(a):
                    "speedup_vs_vegas": (t_vegas / t_flow) if vegas_ok else ">= %.1f (vegas did not reach the "
                                                                           "target in 3x the flow's time)" % (
                                                                               t_vegas / t_flow)},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "mfma", "achieved": bwd["TFLOPs"] if bwd else None, "peak": PEAK_BF16_TFLOPS,
+                     "unit": "TFLOP/s", "frac": (bwd["TFLOPs"] / PEAK_BF16_TFLOPS) if bwd else None,
+                     "traffic": None, "kernel": "flow_backward_kernel", "kernel_ms_avg": bwd["ms_avg"] if bwd else None,
+                     "kernels": kern,
+                     "note": "BF16 MFMA kernels of a training step on 2.5e5 samples per layer; algorithmic "
+                             "flops 2(D*128 + 128*128 + 128*2D) per sample and layer for the forward MLP "
+                             "(backward kernel: twice that, weight gradients: once); peak is the BF16 dense "
+                             "MFMA peak"},
+        "cpu_baseline": None if a.no_cpu else cpu_oracle_rate(
+            (workloads.ridge_8d if D == 8 else workloads.ridge_9d)(diagonal=False)["f"], D, 1e5, 1,
+            "the %s (per-evaluation rate: plain vegas+ does not reach 1e-4 on this integrand in the "
+            "GPU comparison either, see result.vegas)" % wl["name"]),
     }
     print(json.dumps(out))
 
@@ -334,7 +464,7 @@ def main():
     if a.workload == "batched_4096":
         batched(a)
         return
-    if a.workload == "flow_ridge_9d":
+    if a.workload in ("flow_ridge_9d", "flow_ridge_8d"):
         flow_ridge(a)
         return
     if a.workload == "ridge_8d":
@@ -365,6 +495,7 @@ def main():
     nt = max(a.steps, 1)
     ms = (C.c_float * nt)()
     got = L.vgx_fused_times(cx, ms, nt)
+    rows_ms = kernel_ms("train_rows_kernel")
     L.vgx_set_timing(cx, 0)
     evals = [int(g) for g in getattr(integ, "_last_itn_neval", [])] or [integ.last_neval] * a.steps
     total = float(r.sum_neval)
@@ -391,14 +522,19 @@ def main():
                    "parallelism": "single GPU"},
         "result": {"mean": r.mean, "sdev": r.sdev, "chi2_dof": r.chi2 / max(r.dof, 1), "Q": r.Q,
                    "exact": wl["exact"]},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
                      "traffic": traffic, "kernel": "fused_kernel<8, true, GaussMix32>",
                      "kernel_ms_avg": avg_ms, "flops_per_eval": fpe,
                      "hbm_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if (traffic and kern_ms) else None,
                      "hbm_frac": (traffic / (avg_ms * 1e-3) / 8.0e12) if (traffic and kern_ms) else None,
-                     "note": "VALU-bound (no MFMA in this kernel); peak is the FP32 vector peak; "
-                             "hbm_GBps = measured HBM bytes per launch / launch time, against 8 TB/s"},
+                     "train_rows_ms_avg": (sum(rows_ms) / len(rows_ms)) if rows_ms else None,
+                     "train_rows_GBps": (avg_evals * TRAIN_REC_BYTES_8D / (sum(rows_ms) / len(rows_ms) * 1e-3) / 1e9)
+                                        if rows_ms else None,
+                     "note": "VALU-bound FP32 kernel (no MFMA in it); peak is the FP32 vector peak; "
+                             "hbm_GBps = measured HBM bytes per launch / launch time, against 8 TB/s; "
+                             "train_rows: the training histogram built from the sampler's records "
+                             "(24 B per evaluation read)"},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_iters)

@@ -36,7 +36,8 @@ enum {
     VGX_ENAN = -3,     /* integrand evaluated to NaN (pyx:2133-2134) */
     VGX_ENOMEM = -4,   /* device allocation failed (MemoryError) */
     VGX_ERANGE = -5,   /* size beyond what the kernels support */
-    VGX_ENODEV = -6    /* no gfx950 device */
+    VGX_ENODEV = -6,   /* no gfx950 device */
+    VGX_ETIMEOUT = -7  /* a kernel's bounded wait gave up (never in a correct launch) */
 };
 
 const char* vgx_strerror(int status);
@@ -196,11 +197,18 @@ int vgx_iter_accumulate_cov(vgx_ctx* ctx, const vgx_config* cfg, int64_t h0, int
                             int64_t p0, int64_t p1, const double* f, const double* wgt,
                             int m, int correlate, double* acc, int32_t* status, void* stream);
 
-/* Timing of the fused sampler kernel: vgx_set_timing(ctx, 1) clears the
- * record and brackets every following fused launch (up to 256) with HIP
- * events on its stream; vgx_fused_times writes the durations in ms and
- * returns how many were recorded (synchronises on the last event).      */
+/* Kernel timing (measurement; no counterpart in the reference).
+ * vgx_set_timing(ctx, 1) clears the record and brackets the following
+ * launches (up to 256 in all) of the timed kernels with HIP events on their
+ * stream: the fused sampler (tag 1), the batched sampler (2; while timing
+ * is on, vgx_batch_integrate launches its kernels directly instead of
+ * through its graph), the flow's backward (3), weight-gradient (4) and
+ * sampling (6) kernels, and the training-histogram rows (5).
+ * vgx_timed_launches writes the durations in ms and the tags in launch
+ * order and returns how many were recorded; vgx_fused_times writes those
+ * of the fused sampler only.  Both synchronise on the events they read. */
 int vgx_set_timing(vgx_ctx* ctx, int on);
+int vgx_timed_launches(vgx_ctx* ctx, float* ms, int32_t* tags, int cap);
 int vgx_fused_times(vgx_ctx* ctx, float* ms, int cap);
 
 /* ---------------------------------------------------------------------- */

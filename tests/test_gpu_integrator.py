@@ -2,8 +2,9 @@
 
 FP64 parity mode (precision='f64', exact=True) feeds the device and the
 oracle the same Philox uniforms, so the first iteration's y, x, J and weights
-are identical; the estimate, variance, sigf and training histogram agree to
-1e-12 relative (the integrand's exp differs in the last bit).  Later
+are identical; the estimate, variance and sigf agree to 1e-12 relative
+(the integrand's exp differs in the last bit), and the training histogram is
+bit for bit the oracle's when the integrand is evaluated identically.  Later
 iterations are compared statistically: a one-ulp sigf difference may flip an
 int() in the allocation and re-deal the random numbers.
 """
@@ -89,6 +90,32 @@ def test_fused_equals_staged_f32_dims(va, dim):
     for a, b in zip(ra.itn_results, rb.itn_results):
         assert abs(a.mean - b.mean) <= 2e-5 * abs(b.mean)
         assert abs(a.sdev - b.sdev) <= 1e-3 * b.sdev
+
+
+@pytest.mark.parametrize("wl", ["gaussian_4d", "double_peak_8d"])
+def test_fp32_path_error_against_oracle(va, orc, wl):
+    """The FP32 path (24-bit uniforms, FP32 integrand functor; map, weights
+    and sums in FP64) against the FP64 oracle fed the same 24-bit uniforms
+    (rng_f32): the same points, so the difference is the FP32 integrand's
+    rounding (and the map's integer stratum path rounding y once less).
+    Bars: estimate 2e-5 relative, sdev 1e-3 relative, identical allocation;
+    the measured errors are printed."""
+    from vegas_amd import workloads
+    w = getattr(workloads, wl)()
+    f, dim = w["f"], w["dim"]
+    fo = orc.GaussMix(f.A, f.B, f.centers)
+    neval = 2e5
+    integ = va.Integrator([[0, 1]] * dim, neval=neval, seed=19, precision='f32', alpha=0.0)
+    r = integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * dim, neval=neval, seed=19, alpha=0.0, rng_f32=True)
+    out, _, _ = o.iterate(fo)
+    assert integ.last_neval == out.last_neval
+    m = r.itn_results[0]
+    em = abs(m.mean - out.mean) / abs(out.mean)
+    es = abs(m.sdev - math.sqrt(out.var)) / math.sqrt(out.var)
+    print("FP32 path vs FP64 oracle (%s, neval %d): estimate %.2e relative, sdev %.2e relative"
+          % (wl, integ.last_neval, em, es))
+    assert em <= 2e-5 and es <= 1e-3, (em, es)
 
 
 @pytest.mark.parametrize("neval,scale", [(30000, 3.0), (2e5, 6.0)])
@@ -193,6 +220,59 @@ def test_training_and_adapt_parity(va, orc):
     o = orc.OracleIntegrator([(0, 1)] * 4, neval=50000, seed=5)
     o.run(fo, 1)
     np.testing.assert_allclose(integ.map.grid, o.map.grid, rtol=1e-11, atol=1e-13)
+
+
+@pytest.mark.parametrize("neval,nstrat", [(20000, None), (3300, [5, 5, 4, 4]), (2e5, None)])
+def test_training_histogram_exact_bitexact(va, orc, neval, nstrat):
+    """FP64 parity mode: the training histogram of an iteration (sum_f of
+    (w f n)^2 and n_f per increment, pyx:421-464 fed by pyx:2173-2196) is
+    the oracle's bit for bit: the sampler's records are summed per bin in
+    point order, as the reference's sequential loop is.  The integrands are
+    evaluated identically on both sides (the device exp may differ from
+    glibc's in the last bit): a numpy polynomial through the staged path,
+    and a constant device integrand on a non-uniform grid, where the
+    Jacobian makes every (w f n)^2 different."""
+    @va.lbatchintegrand
+    def poly(x):
+        return x[:, 0] * x[:, 1] * x[:, 1] + 0.5 * x[:, 2] * x[:, 3] + 0.25
+
+    def poly_o(x):
+        return x[:, 0] * x[:, 1] * x[:, 1] + 0.5 * x[:, 2] * x[:, 3] + 0.25
+
+    ninc = orc.geometry(4, neval=neval, nstrat=nstrat)["ninc"]
+    nodes = [list(np.linspace(0, 1, int(n) + 1) ** (1.0 + 0.3 * d)) for d, n in enumerate(ninc)]
+    uniform = [list(np.linspace(0, 1, int(n) + 1)) for n in ninc]
+    cases = [(poly, poly_o, uniform),
+             (va.GaussianMixture([1.0], [0.0], [[0.5] * 4]), orc.GaussMix([1.0], [0.0], [[0.5] * 4]), nodes)]
+    for f, fo, grid in cases:
+        integ = va.Integrator(va.AdaptiveMap(grid), neval=neval, nstrat=nstrat, seed=11, precision='f64',
+                              exact=True, alpha=0.5)
+        integ(f, nitn=1)
+        o = orc.OracleIntegrator([(0, 1)] * 4, neval=neval, nstrat=nstrat, seed=11, alpha=0.5)
+        o.map = orc.OracleMap(grid)
+        assert list(o.map.ninc) == list(ninc)
+        out, sf, nf = o.iterate(fo)
+        assert integ.last_neval == out.last_neval
+        gs, gn = integ._last_training()
+        assert gs.shape == sf.shape and np.count_nonzero(sf) > 0.9 * sf.size
+        assert np.array_equal(gn, nf)
+        assert np.array_equal(gs, sf), np.max(np.abs(gs - sf) / np.maximum(sf, 1e-300))
+
+
+def test_training_histogram_fast_mode(va, orc):
+    """Default mode (53-bit uniforms for an FP64 integrand): the histogram
+    built from the records in a fixed order has the oracle's counts exactly
+    and its sums to 1e-12, on the same points (y differs from the oracle's
+    division by at most an ulp, which moves no point across a bin edge
+    here: the counts are equal)."""
+    f, fo = gauss4(va, orc)
+    integ = va.Integrator([[0, 1]] * 4, neval=2e5, seed=13, precision='f64', alpha=0.5)
+    integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * 4, neval=2e5, seed=13, alpha=0.5)
+    _, sf, nf = o.iterate(fo)
+    gs, gn = integ._last_training()
+    assert np.array_equal(gn, nf)
+    np.testing.assert_allclose(gs, sf, rtol=1e-12, atol=0)
 
 
 def test_y_zero_left_out_of_training(va, orc):
@@ -349,12 +429,41 @@ def test_random_batch_integral(va):
     assert abs(result.mean - integral) < 5 * result.sdev
 
 
-def test_seed_determinism(va):
-    f = double_peak(va, dim=4, sigma=0.05)
-    a = va.Integrator([[0, 1]] * 4, neval=2e5, seed=42)(f, nitn=4)
-    b = va.Integrator([[0, 1]] * 4, neval=2e5, seed=42)(f, nitn=4)
-    for x, y in zip(a.itn_results[:1], b.itn_results[:1]):
+@pytest.mark.parametrize("dim,precision", [(4, None), (8, 'f32'), (5, 'f64')])
+def test_seed_determinism(va, dim, precision):
+    """Same seed, same bits: every iteration's estimate, the adapted grid,
+    sigf and the last training histogram of two runs are identical (the
+    training histogram is summed in a fixed order, independent of timing)."""
+    f = double_peak(va, dim=dim, sigma=0.05)
+    runs = []
+    for _ in range(2):
+        integ = va.Integrator([[0, 1]] * dim, neval=2e5, seed=42, precision=precision)
+        r = integ(f, nitn=4)
+        runs.append((r, integ))
+    (a, ia), (b, ib) = runs
+    assert len(a.itn_results) == 4
+    for x, y in zip(a.itn_results, b.itn_results):
         assert x.mean == y.mean and x.sdev == y.sdev
+    assert np.array_equal(ia.map.grid, ib.map.grid)
+    assert np.array_equal(ia.sigf, ib.sigf)
+    for u, v in zip(ia._last_training(), ib._last_training()):
+        assert np.array_equal(u, v)
+
+
+def test_copy_samples_independently(va):
+    """Integrator(integ) is trained like integ but draws its own points (a
+    fresh seed), as copies sharing the reference's global RNG do; seed= fixes
+    the copy's stream."""
+    f = double_peak(va, dim=4, sigma=0.05)
+    a = va.Integrator([[0, 1]] * 4, neval=2e4, seed=42)
+    a(f, nitn=2)
+    b = va.Integrator(a)
+    assert b.seed != a.seed
+    np.testing.assert_array_equal(b.map.extract_grid()[0], a.map.extract_grid()[0])
+    ra, rb = a(f, nitn=1), b(f, nitn=1)
+    assert ra.itn_results[0].mean != rb.itn_results[0].mean
+    c, d = va.Integrator(a, seed=7), va.Integrator(a, seed=7)
+    assert c(f, nitn=1).mean == d(f, nitn=1).mean
 
 
 def test_diagonal_ridge_first_iteration_parity(va, orc):

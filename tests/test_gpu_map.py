@@ -210,3 +210,46 @@ def test_adapt_ninc_and_quirk(va, orc):
     np.testing.assert_allclose(m.grid, o.grid, rtol=1e-13)
     m.adapt(ninc=[1, 1, 1])
     np.testing.assert_allclose(m.grid, [[0, 1], [0, 1], [0, 3]])
+
+
+def _oracle_adapt_to_samples(o, x, fx, nitn, alpha, maxinc):
+    """The reference's adapt_to_samples (pyx:772-804) spelled out on the
+    oracle: regrid to tmp_ninc, nitn rounds of invmap -> add_training_data
+    ((jac f)^2) -> adapt, then regrid to max(ninc, maxinc_axis)."""
+    old_ninc = max(max(int(v) for v in o.ninc), maxinc)
+    tmp_ninc = int(min(old_ninc, x.shape[0] / 10.))
+    o.adapt(ninc=tmp_ninc)
+    for _ in range(nitn):
+        y, jac = o.invmap(x)
+        o.add_training_data(y, (jac * fx) ** 2)
+        o.adapt(alpha=alpha, ninc=tmp_ninc)
+    if tmp_ninc != old_ninc:
+        o.adapt(ninc=old_ninc)
+
+
+@pytest.mark.parametrize("nx,alpha", [(5000, 1.0), (30000, 0.5)])
+def test_adapt_to_samples_vs_oracle(va, orc, nx, alpha):
+    """AdaptiveMap.adapt_to_samples (pyx:728-804) on the device equals the
+    oracle's invmap / add_training_data / adapt chain: 1e-12 relative."""
+    rng = np.random.default_rng(nx)
+    grid = [[0, 1], [-1, 2], [0, 5]]
+    x = np.stack([rng.beta(2, 5, nx), -1 + 3 * rng.beta(5, 2, nx), 5 * rng.uniform(0, 1, nx)], axis=1)
+    f = lambda x: np.exp(-10 * (x[:, 0] - 0.3) ** 2) * (1 + x[:, 1] ** 2)  # noqa: E731
+    m, o = _maps(va, orc, grid, ninc=[50, 60, 7])
+    m.adapt_to_samples(x, f, nitn=4, alpha=alpha)
+    _oracle_adapt_to_samples(o, x, f(x), 4, alpha, va.Integrator.defaults['maxinc_axis'])
+    assert list(m.ninc) == [int(v) for v in o.ninc] == [1000] * 3
+    np.testing.assert_allclose(m.grid, o.grid, rtol=1e-12, atol=1e-14)
+    # f given as values, and the maxinc_axis default read from Integrator
+    saved = va.Integrator.defaults['maxinc_axis']
+    try:
+        va.Integrator.defaults['maxinc_axis'] = 300
+        m2, o2 = _maps(va, orc, grid, ninc=[50, 60, 7])
+        m2.adapt_to_samples(x, f(x), nitn=2, alpha=alpha)
+        _oracle_adapt_to_samples(o2, x, f(x), 2, alpha, 300)
+        assert list(m2.ninc) == [300] * 3
+        np.testing.assert_allclose(m2.grid, o2.grid, rtol=1e-12, atol=1e-14)
+    finally:
+        va.Integrator.defaults['maxinc_axis'] = saved
+    with pytest.raises(ValueError):
+        m.adapt_to_samples(x[:15], f)

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libvgx.so")
 MAXDIM = 64
 
 VGX_OK, VGX_EINVAL, VGX_EHIP, VGX_ENAN, VGX_ENOMEM, VGX_ERANGE, VGX_ENODEV = 0, -1, -2, -3, -4, -5, -6
+VGX_ETIMEOUT = -7
 PREC_F64, PREC_F32 = 0, 1
 FCN_GAUSSMIX, FCN_DIAGRIDGE, FCN_BUFFER = 1, 2, 3
 FLAG_EXACT_ADAPT, FLAG_EXACT_DIV = 1, 2
@@ -84,6 +85,7 @@ def lib():
                                           C.c_int64, vp, vp, C.c_int, C.c_int, vp, vp, vp]
     L.vgx_set_timing.argtypes = [vp, C.c_int]
     L.vgx_fused_times.argtypes = [vp, C.POINTER(C.c_float), C.c_int]
+    L.vgx_timed_launches.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int]
     _lib = L
     return L
 
@@ -119,6 +121,21 @@ def ctx():
         check(lib().vgx_ctx_create(C.byref(p)), "vgx_ctx_create")
         c = _ctx[dev] = p
     return c
+
+
+TIMED_TAGS = {1: 'fused_kernel', 2: 'b_fused_kernel', 3: 'flow_backward_kernel', 4: 'flow_wgrad_kernel',
+              5: 'train_rows_kernel', 6: 'flow_sample_kernel', 7: 'cov'}
+
+
+def timed_launches(cap=256):
+    """The launches recorded since vgx_set_timing(ctx, 1): a list of
+    (kernel name, ms) in launch order."""
+    ms = (C.c_float * cap)()
+    tags = (C.c_int32 * cap)()
+    n = lib().vgx_timed_launches(ctx(), ms, tags, cap)
+    if n < 0:
+        check(n, 'vgx_timed_launches')
+    return [(TIMED_TAGS.get(int(tags[i]), str(int(tags[i]))), float(ms[i])) for i in range(n)]
 
 
 def stream_ptr():

@@ -295,7 +295,8 @@ class AdaptiveMap:
         fx = np.ascontiguousarray(fx) if not isinstance(fx, torch.Tensor) else fx
         if fx.shape[0] != x.shape[0]:
             raise ValueError('shape of x and f(x) mismatch: {} vs {}'.format(x.shape, fx.shape))
-        old_ninc = max(max(self.ninc), 1000)
+        from .integrator import Integrator  # the class default, as pyx:784 reads it
+        old_ninc = int(max(max(self.ninc), Integrator.defaults['maxinc_axis']))
         tmp_ninc = int(min(old_ninc, x.shape[0] / 10.))
         if tmp_ninc < 2:
             raise ValueError('not enough samples: {}'.format(x.shape[0]))

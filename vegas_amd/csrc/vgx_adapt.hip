@@ -109,8 +109,9 @@ __global__ __launch_bounds__(1024) void adapt_parallel_kernel(AdaptArgs a) {
     adapt_parallel_axis(a, blockIdx.x, av, P, red);
 }
 
-// The integrator's adapt: in place on `grid`, skipped when *status != 0,
-// clearing the training data after reading it.
+// The integrator's adapt: in place on `grid`, skipped when *status != 0.
+// The training data are left as they are (the next iteration's allocation
+// kernel clears them).
 int adapt_in_place_setup(vgx_ctx* ctx, int dim, const int32_t* ninc, double* grid, int ld, double* sum_f,
                          double* n_f, int has_training, double alpha, int exact, const int32_t* status,
                          AdaptArgs* out, int* parallel) {
@@ -127,8 +128,8 @@ int adapt_in_place_setup(vgx_ctx* ctx, int dim, const int32_t* ninc, double* gri
     a.n_f = n_f;
     a.new_grid = grid;
     a.status = status;
-    a.clear_sum_f = sum_f;
-    a.clear_n_f = n_f;
+    a.clear_sum_f = nullptr;
+    a.clear_n_f = nullptr;
     int max_old = 0;
     for (int d = 0; d < dim; ++d) {
         a.ninc[d] = ninc[d];

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it,
         for (int k = threadIdx.x; k < (tot + 1) / 2; k += blockDim.x) Lc[k] = 0u;
         __syncthreads();
         const GaussMix32 fcn{1, a.fA + i, a.fB + i, a.fc + (size_t)i * kBD};
-        const TileIO tio{a.off, a.sigf, a.slots, nullptr, nullptr, nullptr};
+        const TileIO tio{a.off, a.sigf, a.slots, nullptr, nullptr, nullptr, 0u, 0u};  // LDS histogram
         TileAcc acc{{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, 0.0, 0};
         const uint32_t hend = q.hbase + (uint32_t)q.nhcube, off_end = pl.P + pl.T;
         const uint32_t nr = (itm.t1 - itm.t0 + 15) / 16;
@@ -483,8 +483,15 @@ __global__ __launch_bounds__(256) void b_adapt_kernel(BArgs a, int train) {
     adapt_axis_block(av, P, red, g, row, old, old, a.alpha, kBLd);
 }
 
+// First kernel of every batched call: the integrand parameters in FP32, and
+// the status flags of an earlier call (NaN, overflow) cleared, so that a
+// replayed graph starts afresh (a kernel, not a memset node, in the graph).
 __global__ void b_to_float_kernel(const double* A, const double* B, const double* c, float* fA, float* fB,
-                                  float* fc, int n) {
+                                  float* fc, int n, Scalars* sc) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->status = 0;
+        sc->overflow = 0u;
+    }
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n * kBD; k += gridDim.x * blockDim.x) {
         fc[k] = (float)c[k];
         if (k < n) {
@@ -694,16 +701,18 @@ extern "C" int vgx_batch_set_iteration(vgx_batch* b, uint32_t iteration) {
     return VGX_OK;
 }
 
-// The kernels of nitn iterations on stream s (recorded into the graph).
+// The kernels of nitn iterations on stream s (recorded into the graph, or
+// launched directly when eager; eager launches of the sampler are timed when
+// vgx_set_timing is on).
 static int b_enqueue(vgx_batch* b, const double* A, const double* B, const double* c, const vgx_batch_state& st,
-                     int nitn, hipStream_t s) {
+                     int nitn, hipStream_t s, bool eager) {
     BArgs a = b->a;
     a.grid = st.grid;
     a.sigf = st.sigf;
     a.sum_sigf = st.sum_sigf;
     const int train = b->alpha > 0 ? 1 : 0;
     hipLaunchKernelGGL(b_to_float_kernel, dim3(64), dim3(256), 0, s, A, B, c, (float*)a.fA, (float*)a.fB,
-                       (float*)a.fc, b->n);
+                       (float*)a.fc, b->n, a.sc);
     StradArgs sa{};
     sa.errv = nullptr;
     sa.off = a.off;
@@ -716,10 +725,22 @@ static int b_enqueue(vgx_batch* b, const double* A, const double* B, const doubl
         hipLaunchKernelGGL(b_alloc_kernel, dim3(b->nblk), dim3(kAllocThreads), 0, s, a);
         hipLaunchKernelGGL(b_plan_kernel, dim3(1), dim3(1024), 0, s, a);
         hipLaunchKernelGGL(b_offsets_kernel, dim3(b->nblk), dim3(kAllocThreads), 0, s, a);
-        if (b->per_all)
-            hipLaunchKernelGGL(b_fused_kernel<true>, dim3(b->blocks), dim3(kFusedThreads), kBLds, s, a, it, train);
-        else
-            hipLaunchKernelGGL(b_fused_kernel<false>, dim3(b->blocks), dim3(kFusedThreads), kBLds, s, a, it, train);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool timed = eager && timed_slot(g_timing_ctx, kTimedBatchFused, &e0, &e1);
+        if (b->per_all) {
+            if (timed)
+                hipExtLaunchKernelGGL(b_fused_kernel<true>, dim3(b->blocks), dim3(kFusedThreads), (uint32_t)kBLds, s,
+                                      e0, e1, 0, a, it, train);
+            else
+                hipLaunchKernelGGL(b_fused_kernel<true>, dim3(b->blocks), dim3(kFusedThreads), kBLds, s, a, it, train);
+        } else {
+            if (timed)
+                hipExtLaunchKernelGGL(b_fused_kernel<false>, dim3(b->blocks), dim3(kFusedThreads), (uint32_t)kBLds, s,
+                                      e0, e1, 0, a, it, train);
+            else
+                hipLaunchKernelGGL(b_fused_kernel<false>, dim3(b->blocks), dim3(kFusedThreads), kBLds, s, a, it,
+                                   train);
+        }
         hipLaunchKernelGGL(strad_kernel, dim3(b->strad_blocks), dim3(kStradThreads), 0, s, b->sg, sa);
         hipLaunchKernelGGL(b_close_kernel, dim3((b->n + 3) / 4), dim3(256), 0, s, a, st, it, nitn);
         hipLaunchKernelGGL(b_adapt_kernel, dim3(b->nad), dim3(256), 0, s, a, train);
@@ -733,6 +754,11 @@ extern "C" int vgx_batch_integrate(vgx_batch* b, const double* A, const double* 
         !st->itn_var || !st->itn_neval || !st->ravg || nitn < 1)
         return VGX_EINVAL;
     hipStream_t s = (hipStream_t)stream;
+    if (g_timing_ctx && g_timing_ctx->timing_on) {
+        // measurement mode: the same kernels launched directly, the sampler
+        // timed (the captured graph, if any, is left as it is)
+        return b_enqueue(b, A, B, c, *st, nitn, s, true);
+    }
     const bool same = b->exec && b->g_nitn == nitn && b->g_A == A && b->g_B == B && b->g_c == c &&
                       memcmp(&b->g_st, st, sizeof(vgx_batch_state)) == 0;
     if (!same) {
@@ -740,11 +766,10 @@ extern "C" int vgx_batch_integrate(vgx_batch* b, const double* A, const double* 
         if (b->graph) (void)hipGraphDestroy(b->graph);
         b->exec = nullptr;
         b->graph = nullptr;
-        // status flags of the previous run (NaN, overflow) do not carry over
-        if (hipMemsetAsync(b->a.sc, 0, sizeof(Scalars), s) != hipSuccess) return VGX_EHIP;
+        // (the graph's first kernel clears the status flags of earlier calls)
         if (hipStreamSynchronize(s) != hipSuccess) return VGX_EHIP;
         VGX_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-        const int rc = b_enqueue(b, A, B, c, *st, nitn, b->cap);
+        const int rc = b_enqueue(b, A, B, c, *st, nitn, b->cap, false);
         hipGraph_t gr = nullptr;
         const hipError_t e = hipStreamEndCapture(b->cap, &gr);
         if (rc) {

@@ -391,6 +391,10 @@ extern "C" int vgx_iter_accumulate_cov(vgx_ctx* ctx, const vgx_config* c, int64_
     if ((rc = ctx->cov_part.ensure((size_t)npairs * nbx * 4 * sizeof(double)))) return rc;
     const uint32_t* off = (const uint32_t*)ctx->off.ptr;
     double* part = (double*)ctx->cov_part.ptr;
+    // timed as one span: per-hcube sums and their reduction (events around both)
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool timed = timed_slot(ctx, kTimedCov, &e0, &e1);
+    if (timed) VGX_HIP(hipEventRecord(e0, s));
     switch (m <= kCovAllMax ? m : 0) {
         case 1: hipLaunchKernelGGL(cov_hcube_all_kernel<1>, dim3(nbx), dim3(kCovThreads), 0, s, off, (uint32_t)h0,
                                    (uint32_t)h1, (uint32_t)p0, (uint32_t)p1, wgt, f, correlate, part, status); break;
@@ -406,5 +410,6 @@ extern "C" int vgx_iter_accumulate_cov(vgx_ctx* ctx, const vgx_config* c, int64_
     }
     hipLaunchKernelGGL(cov_reduce_kernel, dim3(npairs), dim3(kCovThreads), 0, s,
                        (const double*)ctx->cov_part.ptr, nbx, npairs, m, correlate, acc);
+    if (timed) VGX_HIP(hipEventRecord(e1, s));
     return check_launch();
 }

@@ -31,6 +31,7 @@ extern "C" const char* vgx_strerror(int status) {
         case VGX_ENOMEM: return "device allocation failed";
         case VGX_ERANGE: return "size beyond kernel limits";
         case VGX_ENODEV: return "no gfx950 device";
+        case VGX_ETIMEOUT: return "a device wait timed out";
         default: return "unknown vgx status";
     }
 }
@@ -64,7 +65,8 @@ extern "C" int vgx_ctx_destroy(vgx_ctx* c) {
     vgx::Scratch* all[] = {&c->hist_cnt, &c->hist_sorted, &c->hist_start, &c->nh, &c->off,
                            &c->tile_h, &c->blocksum, &c->tile_slots, &c->blk_part,
                            &c->strad_part, &c->misc, &c->adapt_tmp, &c->errv, &c->cov_part,
-                           &c->hist_part};
+                           &c->hist_part, &c->trec};
+    if (vgx::g_timing_ctx == c) vgx::g_timing_ctx = nullptr;
     for (auto* s : all) s->release();
     for (int i = 0; i < 2 * vgx_ctx::kMaxTimed; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -72,17 +74,39 @@ extern "C" int vgx_ctx_destroy(vgx_ctx* c) {
     return VGX_OK;
 }
 
+namespace vgx {
+vgx_ctx* g_timing_ctx = nullptr;
+}
+
 extern "C" int vgx_set_timing(vgx_ctx* c, int on) {
     if (!c) return VGX_EINVAL;
     c->timing_on = on ? 1 : 0;
     c->ntimed = 0;
+    if (on)
+        vgx::g_timing_ctx = c;
+    else if (vgx::g_timing_ctx == c)
+        vgx::g_timing_ctx = nullptr;
     return VGX_OK;
+}
+
+extern "C" int vgx_timed_launches(vgx_ctx* c, float* ms, int32_t* tags, int cap) {
+    if (!c || ((!ms || !tags) && cap > 0)) return VGX_EINVAL;
+    const int n = c->ntimed < cap ? c->ntimed : cap;
+    for (int i = 0; i < n; ++i) {
+        VGX_HIP(hipEventSynchronize(c->ev[2 * i + 1]));
+        VGX_HIP(hipEventElapsedTime(&ms[i], c->ev[2 * i], c->ev[2 * i + 1]));
+        tags[i] = c->tags[i];
+    }
+    return n;
 }
 
 extern "C" int vgx_fused_times(vgx_ctx* c, float* ms, int cap) {
     if (!c || (!ms && cap > 0)) return VGX_EINVAL;
-    const int n = c->ntimed < cap ? c->ntimed : cap;
-    if (c->ntimed > 0) VGX_HIP(hipEventSynchronize(c->ev[2 * (c->ntimed - 1) + 1]));
-    for (int i = 0; i < n; ++i) VGX_HIP(hipEventElapsedTime(&ms[i], c->ev[2 * i], c->ev[2 * i + 1]));
+    int n = 0;
+    for (int i = 0; i < c->ntimed && n < cap; ++i) {
+        if (c->tags[i] != vgx::kTimedFused) continue;
+        VGX_HIP(hipEventSynchronize(c->ev[2 * i + 1]));
+        VGX_HIP(hipEventElapsedTime(&ms[n++], c->ev[2 * i], c->ev[2 * i + 1]));
+    }
     return n;
 }

@@ -12,6 +12,10 @@ constexpr double kTiny = 1e-257;
 constexpr double kEpsilon = 2.220446049250313080847263336181640625e-16 * 1e4;
 constexpr int kWave = 64;
 
+// Training records of the fused sampler: one u16 bin per point and axis, or
+// kNoBin where the point is not trained on the axis.
+constexpr uint32_t kNoBin = 0xFFFFu;
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Counter layout shared
 // with the sampler: ctr = (p lo, p hi, iteration, block), key = (seed lo, hi).

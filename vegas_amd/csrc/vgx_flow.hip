@@ -1077,7 +1077,12 @@ extern "C" int vgx_flow_sample(vgx_flow* fl, const vgx_integrand* fcn, int64_t n
     if (hipFuncSetAttribute((const void*)flow_sample_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds) != hipSuccess)
         return VGX_EHIP;
-    hipLaunchKernelGGL(flow_sample_kernel, dim3(blocks), dim3(kFlowThreads), lds, s, flow_args(fl), F, a);
+    hipEvent_t e0, e1;
+    if (timed_slot(g_timing_ctx, kTimedFlowSample, &e0, &e1))
+        hipExtLaunchKernelGGL(flow_sample_kernel, dim3(blocks), dim3(kFlowThreads), (uint32_t)lds, s, e0, e1, 0,
+                              flow_args(fl), F, a);
+    else
+        hipLaunchKernelGGL(flow_sample_kernel, dim3(blocks), dim3(kFlowThreads), lds, s, flow_args(fl), F, a);
     if ((rc = check_launch())) return rc;
     if (!sums) return VGX_OK;
     std::vector<double> h((size_t)blocks * 4);
@@ -1166,9 +1171,18 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
         // one workgroup per CU: the layer's weights fill most of the LDS, so a
         // second workgroup per CU would only run after the first
         const int bb = (int)std::min<uint64_t>((uint64_t)info.cus, (ntiles + 7) / 8);
-        hipLaunchKernelGGL(flow_backward_kernel, dim3(bb), dim3(kBwdThreads), blds, s, flow_args(fl), b);
-        hipLaunchKernelGGL(flow_wgrad_kernel, dim3(nparts), dim3(kWgThreads), 0, s, (const uint16_t*)fl->ops.ptr,
-                           ntiles, wg_chunk, (float*)fl->wpart.ptr);
+        hipEvent_t e0, e1;
+        if (timed_slot(g_timing_ctx, kTimedFlowBwd, &e0, &e1))
+            hipExtLaunchKernelGGL(flow_backward_kernel, dim3(bb), dim3(kBwdThreads), (uint32_t)blds, s, e0, e1, 0,
+                                  flow_args(fl), b);
+        else
+            hipLaunchKernelGGL(flow_backward_kernel, dim3(bb), dim3(kBwdThreads), blds, s, flow_args(fl), b);
+        if (timed_slot(g_timing_ctx, kTimedFlowWgrad, &e0, &e1))
+            hipExtLaunchKernelGGL(flow_wgrad_kernel, dim3(nparts), dim3(kWgThreads), 0, s, e0, e1, 0,
+                                  (const uint16_t*)fl->ops.ptr, ntiles, wg_chunk, (float*)fl->wpart.ptr);
+        else
+            hipLaunchKernelGGL(flow_wgrad_kernel, dim3(nparts), dim3(kWgThreads), 0, s, (const uint16_t*)fl->ops.ptr,
+                               ntiles, wg_chunk, (float*)fl->wpart.ptr);
         hipLaunchKernelGGL(flow_wgrad_reduce_kernel, dim3((kPLayer + kRedCols - 1) / kRedCols),
                            dim3(kRedCols * kRedGroups), 0, s,
                            (const float*)fl->wpart.ptr, nparts,

@@ -7,8 +7,10 @@
 // J*(dv_y/n_h) (pyx:1749) -> integrand -> w f.  Per hcube: the reference's
 // two-pass sums with the EPSILON guard (pyx:2146-2183) as wave-level
 // segmented scans over the tile; hcubes cut by a tile edge are finished by
-// the straddler kernel from per-tile part sums.  Training data (w f n)^2 (pyx:2173) goes into an
-// LDS-privatised histogram (FP64 sums, packed u16 counts).
+// the straddler kernel from per-tile part sums.  Training data (w f n)^2 (pyx:2173) is written
+// as one record per point (its bin on every axis and the value), from which a separate pass
+// builds the histogram in a fixed order; the batched sampler, which calls the same tile
+// body, keeps an LDS-privatised histogram (FP64 sums, packed u16 counts) instead.
 #pragma once
 #include "vgx_iter.h"
 
@@ -300,9 +302,35 @@ struct TileIO {
     double* sigf;
     Slot* slots;
     double* errv;
-    double* sum_f;           // global histogram (when it is not in LDS)
-    double* n_f;
+    uint32_t* rec_bins;      // training records (single integral), or null: LDS histogram
+    double* rec_c2;
+    uint32_t rec_p0;         // point index of record 0
+    uint32_t rec_stride;     // records per axis array
 };
+
+// The training record of point ri: its bins on axes 2k and 2k+1 as the low
+// and high halves of rec_bins[k * rec_stride + ri] (kNoBin where kb[d] < 0,
+// where bit d of zm is set, and past the last axis), so that a pass over one
+// axis reads one contiguous array; c2 = (w f n)^2 at rec_c2[ri].
+template <int DIM>
+__device__ __forceinline__ void store_record(const TileIO& io, int dim, uint32_t ri, const int* kb,
+                                             uint64_t zm, double c2) {
+    constexpr int ND = nd_max(DIM);
+    uint32_t* dst = io.rec_bins + ri;
+#pragma unroll
+    for (int k = 0; k < (ND + 1) / 2; ++k) {
+        if (DIM <= 0 && 2 * k >= dim) break;
+        uint32_t h[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int d = 2 * k + e;
+            h[e] = (d < ND && d < dim && kb[d < ND ? d : 0] >= 0 && !((zm >> d) & 1))
+                       ? (uint32_t)kb[d < ND ? d : 0] : kNoBin;
+        }
+        dst[(size_t)k * io.rec_stride] = h[0] | (h[1] << 16);
+    }
+    io.rec_c2[ri] = c2;
+}
 
 // Per-wave accumulators across tiles.
 struct TileAcc {
@@ -415,31 +443,32 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
             vj = w * f;
             const double tn = vj * (double)ncur;
             acc.wmax = fmax(acc.wmax, fabs(tn));
-            if (g.train_points && PER && g.hist_lds && zmin != 0) {
-                // every axis inside (0,1): no per-axis test
+            if (g.train_points) {
                 const double c2 = tn * tn;
+                if (io.rec_c2) {
+                    // a record per point; the histogram is built from them in
+                    // point order or in a fixed order (vgx_integrate.hip)
+                    const uint64_t zm = (PER && zmin == 0) ? per_zero_axes<DIM>(g, pc, pj - tp.pshift) : 0ull;
+                    store_record<DIM>(io, dim, pj - io.rec_p0, kb, zm, c2);
+                } else if (PER && zmin != 0) {
+                    // LDS histogram, every axis inside (0,1): no per-axis test
 #pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    if (DIM <= 0 && d >= dim) break;
-                    const int idx = d * g.nb + kb[d];
-                    unsafeAtomicAdd(&Ls[idx], c2);
-                    atomicAdd(&Lc[idx >> 1], 1u + (uint32_t)(idx & 1) * 65535u);
-                }
-            } else if (g.train_points) {
-                const double c2 = tn * tn;
-                const uint64_t zm = (PER && zmin == 0) ? per_zero_axes<DIM>(g, pc, pj - tp.pshift) : 0ull;
+                    for (int d = 0; d < ND; ++d) {
+                        if (DIM <= 0 && d >= dim) break;
+                        const int idx = d * g.nb + kb[d];
+                        unsafeAtomicAdd(&Ls[idx], c2);
+                        atomicAdd(&Lc[idx >> 1], 1u + (uint32_t)(idx & 1) * 65535u);
+                    }
+                } else {
+                    const uint64_t zm = (PER && zmin == 0) ? per_zero_axes<DIM>(g, pc, pj - tp.pshift) : 0ull;
 #pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    if (DIM <= 0 && d >= dim) break;
-                    const int k = kb[d];
-                    if (k < 0 || ((zm >> d) & 1)) continue;
-                    const int idx = d * g.nb + k;
-                    if (g.hist_lds) {
+                    for (int d = 0; d < ND; ++d) {
+                        if (DIM <= 0 && d >= dim) break;
+                        const int k = kb[d];
+                        if (k < 0 || ((zm >> d) & 1)) continue;
+                        const int idx = d * g.nb + k;
                         unsafeAtomicAdd(&Ls[idx], c2);
                         atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
-                    } else {
-                        unsafeAtomicAdd(&io.sum_f[idx], c2);
-                        unsafeAtomicAdd(&io.n_f[idx], 1.0);
                     }
                 }
             }
@@ -613,6 +642,74 @@ __device__ inline void counts_to_row(uint32_t* Lc, int tot, uint32_t* row, bool 
     }
 }
 
+// ---- training histogram in a fixed order ----------------------------------
+// Each axis d has a ticket counter in LDS.  In round r wave w may add to the
+// bins of axis d only when the counter reads r * nwave + w, and moves it on
+// afterwards: every bin receives the additions of round 0 wave 0, round 0
+// wave 1, ..., round 1 wave 0, ..., each wave's in point order per lane
+// slot (the LDS resolves the lanes of one instruction that hit one bin in a
+// fixed order), so the histogram does not depend on timing.  The wait is
+// bounded: past kTicketSpins polls the workgroup is marked broken, every
+// later wait returns at once and the launch reports VGX_ETIMEOUT (a wave
+// always passes its tickets, with or without a tile, so this does not
+// happen in a correct launch).
+constexpr uint32_t kTicketSpins = 1u << 20;
+
+__device__ __forceinline__ bool ticket_wait(uint32_t* tick, uint32_t want, uint32_t* broken) {
+    for (uint32_t spins = 0;; ++spins) {
+        if (__hip_atomic_load(broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
+        if (__hip_atomic_load(tick, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == want) return true;
+        if (spins >= kTicketSpins) {
+            __hip_atomic_store(broken, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Round r of wave `wave`: the records of its tile (if it had one) added to
+// the LDS histogram, axis by axis under the axis tickets.  The records were
+// written by these very lanes (each lane reads back its own 4 points).
+template <int DIM>
+__device__ __forceinline__ void tile_train_ordered(const TileGeom& g, const TileIO& io, int dim, bool have,
+                                                   uint32_t ri0, uint32_t nvalid, uint32_t want, uint32_t* tick,
+                                                   uint32_t* broken, double* Ls, uint32_t* Lc, int lane) {
+    constexpr int ND = nd_max(DIM);
+    constexpr int NW = (ND + 1) / 2;  // bin words per point (two axes each)
+    // the record stores of this wave are complete before it reads them back
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    const uint32_t r0 = ri0 + (uint32_t)lane * kK;  // the lane's first record
+    // every load of the round before the first ticket: one memory round trip
+    double cv[kK];
+    uint32_t wd[NW][kK];
+#pragma unroll
+    for (int j = 0; j < kK; ++j) {
+        const bool ok = have && (uint32_t)lane * kK + j < nvalid;
+        cv[j] = ok ? io.rec_c2[r0 + j] : 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            if (DIM <= 0 && 2 * k >= dim) break;
+            wd[k][j] = ok ? io.rec_bins[(size_t)k * io.rec_stride + r0 + j] : (kNoBin | (kNoBin << 16));
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        if (DIM <= 0 && d >= dim) break;
+        if (ticket_wait(&tick[d], want, broken) && have) {
+#pragma unroll
+            for (int j = 0; j < kK; ++j) {
+                const uint32_t kb = (wd[d >> 1][j] >> (16 * (d & 1))) & 0xFFFFu;
+                if (kb != kNoBin) {
+                    const int idx = d * g.nb + (int)kb;
+                    unsafeAtomicAdd(&Ls[idx], cv[j]);
+                    atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
+                }
+            }
+        }
+        __hip_atomic_store(&tick[d], want + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
 // Workgroup reduction of the wave accumulators into out[0..5] (DD mean,
 // var, sigf sums, fixed order) and the run-wide max |w f n| / NaN flag.
 __device__ inline void block_close(TileAcc& acc, double* Lred, double* out, Scalars* sc) {
@@ -645,14 +742,28 @@ __device__ inline void block_close(TileAcc& acc, double* Lred, double* out, Scal
 template <int DIM, bool PER, class Fcn>
 __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedPtrs io, Fcn fcn) {
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_skip;
+    __shared__ uint32_t s_broken;
     Scalars* sc = io.sc;
-    if (sc->status) return;  // NaN seen earlier in this run: nothing more is done
+    // NaN seen earlier in this run: nothing more is done (read once, so that
+    // every wave of the workgroup takes the same decision)
+    if (threadIdx.x == 0) {
+        s_skip = sc->status;
+        s_broken = 0u;
+    }
+    __syncthreads();
+    if (s_skip) return;
     const TileGeom tg = tile_geom(g);
     const int dim = DIM > 0 ? DIM : g.dim;
     const uint32_t ntiles = sc->b_ntiles, p0 = sc->b_p0, p1 = sc->b_p1, hend = sc->b_h1;
     const int tot = dim * g.nb;
+    // the ordered LDS histogram is compiled into the integer stratum path (the
+    // fast FP32 sampler) only; the other paths train from their records
+    // after the launch
+    const bool lds_hist = PER && g.hist_lds && g.train_points;
 
-    // ---- LDS carve-up ----
+    // ---- LDS carve-up: grid | histogram sums, u16 pair counts, axis
+    // tickets (training in the fixed order) | offsets slices | reduction
     unsigned char* cur = smem;
     const AxisC* ax = io.axes;  // read by scalar loads (ldc), not staged
     double* Lg = reinterpret_cast<double*>(cur);
@@ -661,13 +772,17 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     cur += (size_t)dim * g.ld * sizeof(double);
     double* Ls = nullptr;
     uint32_t* Lc = nullptr;
-    if (g.hist_lds) {
+    uint32_t* tick = nullptr;
+    if (lds_hist) {
         Ls = reinterpret_cast<double*>(cur);
         cur += (size_t)tot * sizeof(double);
         Lc = reinterpret_cast<uint32_t*>(cur);
         cur += (size_t)((tot + 1) / 2) * sizeof(uint32_t);
+        tick = reinterpret_cast<uint32_t*>(cur);
+        cur += (size_t)dim * sizeof(uint32_t);
         for (int i = threadIdx.x; i < tot; i += blockDim.x) Ls[i] = 0.0;
         for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) Lc[i] = 0u;
+        for (int i = threadIdx.x; i < dim; i += blockDim.x) tick[i] = 0u;
     }
     // offsets from smem rounded up to 16 B (pointer arithmetic on smem keeps
     // the accesses ds_* instead of flat)
@@ -679,7 +794,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     double* Lred = reinterpret_cast<double*>(smem + o16);
     __syncthreads();
 
-    const TileIO tio{io.off, io.sigf, io.slots, io.errv, io.sum_f, io.n_f};
+    const TileIO tio{io.off, io.sigf, io.slots, io.errv, io.rec_bins, io.rec_c2, p0, io.rec_stride};
     const uint32_t chunk = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t tb = blockIdx.x * chunk;
     const uint32_t te = min(tb + chunk, ntiles);
@@ -688,33 +803,68 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
 
     TileAcc acc{{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, 0.0, 0};
     bool cnt_flushed = false;  // this workgroup's count row holds earlier rounds
+    // The histogram additions of round r are made while the next tile is
+    // computed, by the two halves of the workgroup at different times: waves
+    // nwave/2.. ("early") add round r - 1 before computing their tile of
+    // round r, waves 0..nwave/2-1 ("late") after it.  Each axis takes round
+    // q's additions in the order early waves, then late waves (its ticket
+    // position); an early wave's additions of round q wait only for round
+    // q - 1 to be complete, a late wave's only for the early waves of round
+    // q, so one half adds while the other computes and no wait depends on
+    // itself.  The records of the round wait in memory meanwhile.
+    const uint32_t half = (uint32_t)nwave / 2;
+    const bool early = (uint32_t)wave >= half;
+    const uint32_t pos = early ? (uint32_t)wave - half : (uint32_t)wave + half;
+    bool have_prev = false;
+    uint32_t t_prev = 0;
+    auto train_prev = [&](uint32_t q) {
+        const uint32_t ri0 = have_prev ? t_prev * (uint32_t)kT : 0u;
+        const uint32_t nvalid = have_prev ? min((uint32_t)kT, p1 - p0 - ri0) : 0u;
+        tile_train_ordered<DIM>(tg, tio, dim, have_prev, ri0, nvalid, q * (uint32_t)nwave + pos, tick, &s_broken, Ls,
+                                Lc, lane);
+    };
     for (uint32_t r = 0; r < nrounds; ++r) {
         const uint32_t t = tb + r * nwave + wave;
-        if (t < te) {
+        const bool have = t < te;
+        if constexpr (PER) {
+            if (lds_hist && early && r > 0) train_prev(r - 1);
+        }
+        if (have) {
             const uint32_t plo = p0 + t * kT;
             const TilePos tp{t, plo, min(plo + (uint32_t)kT, p1), (uint32_t)io.tile_h[t], hend, off_end, 0u, 0u};
             uint32_t ov[kOffRegs];
             load_offsets(io.off, tp.hf, hend, off_end, lane, ov);
             store_offsets(offs, lane, ov);
-            tile_body<DIM, PER>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
+            tile_body<DIM, PER>(tg, tio, fcn, ax, G, nullptr, nullptr, offs, tp, lane, acc);
         }
-        if (g.hist_lds && g.train_points && ((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
-            __syncthreads();
-            counts_to_row(Lc, tot, io.hpart_c + (size_t)blockIdx.x * tot, cnt_flushed, true);
-            cnt_flushed = true;
-            __syncthreads();
+        if constexpr (PER) {
+            if (lds_hist && r > 0) {
+                if (!early) train_prev(r - 1);
+                // rounds up to r - 1 are in the histogram here, for every wave
+                if (((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
+                    __syncthreads();
+                    counts_to_row(Lc, tot, io.hpart_c + (size_t)blockIdx.x * tot, cnt_flushed, true);
+                    cnt_flushed = true;
+                    __syncthreads();
+                }
+            }
         }
+        have_prev = have;
+        t_prev = t;
+    }
+    if constexpr (PER) {
+        if (lds_hist && nrounds > 0) train_prev(nrounds - 1);  // the last round, early then late
     }
 
     // ---- block epilogue ----
     __syncthreads();
-    if (g.hist_lds && g.train_points) {
-        // this workgroup's row of the histogram: plain stores (the rows are
-        // summed by hist_reduce_kernel; global atomics from every workgroup
-        // onto the same bins would serialise at the end of the kernel)
+    if (lds_hist) {
+        // this workgroup's row of the histogram: plain stores, summed in
+        // workgroup order after the launch
         double* srow = io.hpart_s + (size_t)blockIdx.x * tot;
         for (int i = threadIdx.x; i < tot; i += blockDim.x) srow[i] = Ls[i];
         counts_to_row(Lc, tot, io.hpart_c + (size_t)blockIdx.x * tot, cnt_flushed, false);
+        if (threadIdx.x == 0 && s_broken) atomicCAS(&sc->status, 0, (int32_t)VGX_ETIMEOUT);
     }
     block_close(acc, Lred, io.blk_part + 6 * (size_t)blockIdx.x, sc);
 }

@@ -33,7 +33,18 @@ struct HistArgs {
     uint32_t* start;   // ordered: dim x nb bin starts
     uint32_t* total;   // ordered: dim x nb bin totals
     double* sorted;    // ordered: dim x n
+    // training records of the fused sampler instead of y and f: the bin of
+    // point i on axis d in the (d & 1) half of rbins[(d >> 1) * rstride + i],
+    // the values in f, the point count read on the device as nrange[1] -
+    // nrange[0] (n is then the capacity)
+    const uint32_t* rbins;
+    int64_t rstride;
+    const uint32_t* nrange;
 };
+
+__device__ inline int64_t npoints(const HistArgs& a) {
+    return a.nrange ? (int64_t)(a.nrange[1] - a.nrange[0]) : a.n;
+}
 
 // Bin of y for axis d, or -1 when y is not strictly inside (0, 1).
 __device__ inline int train_bin(double yv, int ni) {
@@ -42,6 +53,15 @@ __device__ inline int train_bin(double yv, int ni) {
     // y*ninc can round up to ninc for y within an ulp of 1; the reference
     // would then write one slot past the row.  Keep it in the last bin.
     return k < ni ? k : ni - 1;
+}
+
+// Bin of point i on axis d: from its record, or from y (pyx:459-463).
+__device__ inline int point_bin(const HistArgs& a, int64_t i, int d, int ni) {
+    if (a.rbins) {
+        const uint32_t b = (a.rbins[(d >> 1) * a.rstride + i] >> (16 * (d & 1))) & 0xFFFFu;
+        return (b == kNoBin || (int)b >= ni) ? -1 : (int)b;  // (a bin >= ninc is never written)
+    }
+    return train_bin(a.y[i * a.dim + d], ni);
 }
 
 __global__ __launch_bounds__(256) void hist_fast_kernel(HistArgs a) {
@@ -88,11 +108,11 @@ __global__ __launch_bounds__(256) void hist_count_kernel(HistArgs a) {
     const int ni = a.ninc[d];
     for (int k = threadIdx.x; k < a.nb; k += blockDim.x) cl[k] = 0u;
     __syncthreads();
-    const int64_t p0 = (int64_t)tile * kHistTile;
+    const int64_t p0 = (int64_t)tile * kHistTile, n = npoints(a);
     for (int j = threadIdx.x; j < kHistTile; j += blockDim.x) {
         int64_t i = p0 + j;
-        if (i >= a.n) break;
-        int k = train_bin(a.y[i * a.dim + d], ni);
+        if (i >= n) break;
+        int k = point_bin(a, i, d, ni);
         if (k >= 0) atomicAdd(&cl[k], 1u);
     }
     __syncthreads();
@@ -148,13 +168,13 @@ __global__ __launch_bounds__(64) void hist_scatter_kernel(HistArgs a) {
     const uint32_t* tbase = a.cnt + ((int64_t)d * a.ntiles + tile) * a.nb;
     const uint32_t* bstart = a.start + d * a.nb;
     double* out = a.sorted + (int64_t)d * a.n;
-    const int64_t p0 = (int64_t)tile * kHistTile;
+    const int64_t p0 = (int64_t)tile * kHistTile, n = npoints(a);
     for (int g = 0; g < kHistTile; g += 64) {
         const int64_t i = p0 + g + lane;
         int key = -1;
         double fa = 0.0;
-        if (i < a.n) {
-            key = train_bin(a.y[i * a.dim + d], ni);
+        if (i < n) {
+            key = point_bin(a, i, d, ni);
             fa = fabs(a.f[i]);
         }
         uint64_t active = __ballot(key >= 0);
@@ -242,3 +262,49 @@ extern "C" int vgx_train_hist(vgx_ctx* ctx, int dim, const int32_t* ninc, int ld
     hipLaunchKernelGGL(hist_sum_kernel, dim3((maxn + 255) / 256, dim), dim3(256), 0, st, a);
     return check_launch();
 }
+
+namespace vgx {
+
+// The training histogram of a batch of the fused sampler, from its records,
+// bit for bit the reference's add_training_data on the batch's points in
+// order (pyx:457-463): the ordered mode above with the bins and values of
+// the records.  cap bounds the batch's point count, which is read on the
+// device from nrange.
+int train_hist_records(vgx_ctx* ctx, int dim, const int32_t* ninc, int nb, const uint32_t* rbins,
+                       int64_t rstride, const double* c2, const uint32_t* nrange, int64_t cap, double* sum_f,
+                       double* n_f, hipStream_t st) {
+    if (cap <= 0) return VGX_OK;
+    if ((size_t)nb * sizeof(uint32_t) > 64 * 1024 || dim > VGX_MAXDIM) return VGX_ERANGE;
+    HistArgs a{};
+    a.dim = dim;
+    a.nb = nb;
+    a.n = cap;
+    a.f = c2;
+    a.sum_f = sum_f;
+    a.n_f = n_f;
+    a.rbins = rbins;
+    a.rstride = rstride;
+    a.nrange = nrange;
+    int maxn = 0;
+    for (int d = 0; d < dim; ++d) {
+        a.ninc[d] = ninc[d];
+        maxn = ninc[d] > maxn ? ninc[d] : maxn;
+    }
+    a.ntiles = (int)((cap + kHistTile - 1) / kHistTile);
+    int rc = ctx->hist_cnt.ensure((size_t)dim * a.ntiles * a.nb * sizeof(uint32_t));
+    if (rc) return rc;
+    if ((rc = ctx->hist_start.ensure(2 * (size_t)dim * a.nb * sizeof(uint32_t)))) return rc;
+    if ((rc = ctx->hist_sorted.ensure((size_t)dim * cap * sizeof(double)))) return rc;
+    a.cnt = (uint32_t*)ctx->hist_cnt.ptr;
+    a.start = (uint32_t*)ctx->hist_start.ptr;
+    a.total = a.start + (size_t)dim * a.nb;
+    a.sorted = (double*)ctx->hist_sorted.ptr;
+    const size_t lds = (size_t)a.nb * sizeof(uint32_t);
+    hipLaunchKernelGGL(hist_count_kernel, dim3(a.ntiles, dim), dim3(256), lds, st, a);
+    hipLaunchKernelGGL(hist_scan_kernel, dim3(dim), dim3(1024), 0, st, a);
+    hipLaunchKernelGGL(hist_scatter_kernel, dim3(a.ntiles, dim), dim3(64), lds, st, a);
+    hipLaunchKernelGGL(hist_sum_kernel, dim3((maxn + 255) / 256, dim), dim3(256), 0, st, a);
+    return check_launch();
+}
+
+}  // namespace vgx

@@ -45,12 +45,23 @@ __global__ void iter_init_kernel(IterGeom g, double* sum_f, double* n_f, Scalars
 // ------------------------------------------------------------ allocation ----
 // n_h = min(int(sigf_h * neval_sigf) + nmin, max(nmax, nmin))   (pyx:1696-1698)
 // Per-workgroup total, min and max go to stat[3 * blockIdx.x + {0,1,2}].
+// The iteration's training data start empty (AdaptiveMap.clear, pyx:596-599):
+// they are cleared here rather than after the adapt reads them, so the last
+// iteration's histogram stays readable after a run.
 __global__ __launch_bounds__(kAllocThreads) void alloc_kernel(IterGeom g, double* sigf,
                                                               const double* sum_sigf, int32_t* nh,
-                                                              uint64_t* stat, const Scalars* sc) {
+                                                              uint64_t* stat, const Scalars* sc,
+                                                              double* sum_f, double* n_f) {
     __shared__ uint64_t red[kAllocThreads / 64];
     __shared__ int32_t rmin[kAllocThreads / 64], rmax[kAllocThreads / 64];
     if (sc->status) return;
+    if (g.train_points || g.train_errors) {
+        const int tot = g.dim * g.nb;
+        for (int i = blockIdx.x * kAllocThreads + threadIdx.x; i < tot; i += gridDim.x * kAllocThreads) {
+            sum_f[i] = 0.0;
+            n_f[i] = kTiny;
+        }
+    }
     const int reset = sc->reset_sigf;  // previous iteration found sum(sigf) == 0
     const double ss = *sum_sigf;
     const double neval_sigf = (g.beta > 0 && ss > 0 && g.adaptive_sampler) ? g.neval_frac * g.neval / ss : 0.0;
@@ -557,6 +568,110 @@ __device__ __forceinline__ void hist_reduce_block(const HistArgs& h, const Scala
     }
 }
 
+// Training histogram of a fast-mode batch from the fused sampler's records,
+// in a fixed order.  The batch's points are cut into kTrainBlocks contiguous
+// blocks (a multiple of 64 points each); workgroup (block b, axis d) is one
+// wave that adds its block's points into an LDS row of axis d (FP64 sums,
+// u32 counts) and writes that row to part d of row b, which
+// hist_reduce_block sums over the blocks in a fixed order.  Each bin is thus
+// only ever added to by one wave, in a fixed order (rounds of 256 points,
+// then the 4 points of a lane, then the lanes as the LDS orders one
+// instruction's lanes), so a run's histogram does not depend on timing.
+// A lane reads 4 consecutive points per round (one 16-byte load of their
+// bin words, two of their values), in batches of kTrainRounds rounds whose
+// loads are issued a batch ahead of their additions.  Workgroups are dealt to the 8 XCDs round-robin by id, so
+// the id keeps the XCD slot of the block (id mod 8 = b mod 8): the waves of
+// one block run on one XCD and share the records' lines in its L2.
+constexpr int kTrainBlocks = 256, kTrainRounds = 8, kXcds = 8;
+static_assert(kTrainBlocks % kXcds == 0, "blocks must fill the XCD slots evenly");
+
+// kTrainRounds rounds of 256 points of one axis: a lane's 4 bins and values
+struct TrainBatch {
+    uint4 w[kTrainRounds];
+    double2 c01[kTrainRounds], c23[kTrainRounds];
+};
+
+__device__ __forceinline__ void train_load(TrainBatch& t, const uint32_t* words, const double* rc2, uint32_t p,
+                                           uint32_t hi, int lane) {
+#pragma unroll
+    for (int r = 0; r < kTrainRounds; ++r) {
+        const uint32_t i = p + 256u * r + 4u * lane;  // a multiple of 4 (p is)
+        if (i + 3 < hi) {
+            t.w[r] = *reinterpret_cast<const uint4*>(words + i);
+            t.c01[r] = *reinterpret_cast<const double2*>(rc2 + i);
+            t.c23[r] = *reinterpret_cast<const double2*>(rc2 + i + 2);
+        } else {
+            const uint32_t none = kNoBin | (kNoBin << 16);
+            t.w[r] = uint4{i < hi ? words[i] : none, i + 1 < hi ? words[i + 1] : none,
+                           i + 2 < hi ? words[i + 2] : none, none};
+            t.c01[r] = double2{i < hi ? rc2[i] : 0.0, i + 1 < hi ? rc2[i + 1] : 0.0};
+            t.c23[r] = double2{i + 2 < hi ? rc2[i + 2] : 0.0, 0.0};
+        }
+    }
+}
+
+__device__ __forceinline__ void train_add(const TrainBatch& t, int sh, uint32_t ni, double* Ls, uint32_t* Lc) {
+#pragma unroll
+    for (int r = 0; r < kTrainRounds; ++r) {
+        const uint32_t k4[4] = {(t.w[r].x >> sh) & 0xFFFFu, (t.w[r].y >> sh) & 0xFFFFu,
+                                (t.w[r].z >> sh) & 0xFFFFu, (t.w[r].w >> sh) & 0xFFFFu};
+        const double v4[4] = {t.c01[r].x, t.c01[r].y, t.c23[r].x, t.c23[r].y};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (k4[j] < ni) {  // kNoBin and any bin past ninc are skipped
+                unsafeAtomicAdd(&Ls[k4[j]], v4[j]);
+                atomicAdd(&Lc[k4[j]], 1u);
+            }
+    }
+}
+
+__global__ __launch_bounds__(64) void train_rows_kernel(IterGeom g, const uint32_t* rbins, const double* rc2,
+                                                        uint32_t rstride, double* rows_s, uint32_t* rows_c,
+                                                        const Scalars* sc) {
+    extern __shared__ double trow[];
+    const int dim = g.dim, nb = g.nb;
+    const int xcd = (int)blockIdx.x % kXcds, rest = (int)blockIdx.x / kXcds;
+    const int d = rest % dim, b = (rest / dim) * kXcds + xcd;
+    const int lane = threadIdx.x;
+    double* Ls = trow;
+    uint32_t* Lc = reinterpret_cast<uint32_t*>(trow + nb);
+    for (int k = lane; k < nb; k += 64) {
+        Ls[k] = 0.0;
+        Lc[k] = 0u;
+    }
+    __syncthreads();
+    // the sampler did not run, or its records are void: the rows stay zero
+    const bool skip = sc->status || sc->overflow;
+    const uint32_t ni = (uint32_t)g.ninc[d];
+    const uint32_t* words = rbins + (size_t)(d >> 1) * rstride;
+    const int sh = 16 * (d & 1);
+    if (!skip) {
+        const uint32_t n = sc->b_p1 - sc->b_p0;
+        const uint32_t chunk = ((n + kTrainBlocks - 1) / kTrainBlocks + 63u) & ~63u;
+        const uint32_t lo = min((uint32_t)b * chunk, n), hi = min(lo + chunk, n);
+        constexpr uint32_t step = 256u * kTrainRounds;
+        // software pipelined: the loads of the next batch are in flight
+        // while the current one is added (the additions keep their order)
+        TrainBatch A, B;
+        if (lo < hi) train_load(A, words, rc2, lo, hi, lane);
+        for (uint32_t p = lo; p < hi; p += 2 * step) {
+            if (p + step < hi) train_load(B, words, rc2, p + step, hi, lane);
+            train_add(A, sh, ni, Ls, Lc);
+            if (p + step >= hi) break;
+            if (p + 2 * step < hi) train_load(A, words, rc2, p + 2 * step, hi, lane);
+            train_add(B, sh, ni, Ls, Lc);
+        }
+    }
+    __syncthreads();
+    const size_t tot = (size_t)dim * nb;
+    double* rs = rows_s + (size_t)b * tot + (size_t)d * nb;
+    uint32_t* rcn = rows_c + (size_t)b * tot + (size_t)d * nb;
+    for (int k = lane; k < nb; k += 64) {
+        rs[k] = Ls[k];
+        rcn[k] = Lc[k];
+    }
+}
+
 // Straddler workgroups 0..nstrad-1, histogram-row workgroups after them.
 __global__ __launch_bounds__(kStradThreads) void strad_hist_kernel(IterGeom g, StradArgs a, HistArgs h,
                                                                     int nstrad) {
@@ -699,22 +814,29 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
     g->seed_lo = (uint32_t)(c->seed & 0xffffffffu);
     g->seed_hi = (uint32_t)(c->seed >> 32);
     g->nitn = (uint32_t)c->nitn;
-    // LDS plan of the fused kernel: axis table + FP64 grid always, the
-    // privatised histogram when it also fits.
+    // LDS plan of the fused kernel: the FP64 grid, offsets slices and the
+    // reduction area, and in fast mode the training histogram (FP64 sums, u16
+    // pair counts, axis tickets) when it fits; otherwise (and in exact mode)
+    // the histogram is built from the training records after the launch
     const size_t tot = (size_t)c->dim * g->nb;
     const size_t fixed = (size_t)16 * kNoff * sizeof(uint32_t) + 16 * 6 * sizeof(double) + 64;
     const size_t gbytes = (size_t)c->dim * c->ld * sizeof(double);
-    const size_t hbytes = tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t) + 16;
+    const size_t hbytes = tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t) + c->dim * sizeof(uint32_t) + 32;
     g->grid_lds = 1;
     if (fixed + gbytes > info.max_lds) return VGX_ERANGE;
-    g->hist_lds = g->train_points && (fixed + gbytes + hbytes <= info.max_lds);
+    // (the ordered LDS histogram is in the integer stratum path's kernels)
+    g->hist_lds = g->train_points && g->per_all && !(c->flags & VGX_FLAG_EXACT_ADAPT) &&
+                  (fixed + gbytes + hbytes <= info.max_lds);
+    // the training-row kernel's LDS row of one axis
+    if ((size_t)g->nb * (sizeof(double) + sizeof(uint32_t)) > info.max_lds) return VGX_ERANGE;
     return VGX_OK;
 }
 
 static size_t fused_lds(const IterGeom& g) {
     const size_t tot = (size_t)g.dim * g.nb;
     size_t b = (size_t)g.dim * g.ld * sizeof(double);
-    if (g.hist_lds) b += tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t);
+    if (g.hist_lds && g.train_points)
+        b += tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t) + g.dim * sizeof(uint32_t);
     b = (b + 15) & ~(size_t)15;
     b += (size_t)16 * kNoff * sizeof(uint32_t);
     b = (b + 15) & ~(size_t)15;
@@ -745,7 +867,10 @@ struct Run {
     AxisC* axes;
     double* errv;
     double* sigf_part;
-    int hist_tot;  // dim * nb when the fused sampler keeps the histogram in LDS and trains
+    int hist_tot;  // dim * nb when the fused sampler trains (records and rows)
+    int exact_train;  // training histogram summed in point order from the records
+    double* rows_s;   // fast mode: [kTrainBlocks][dim * nb] rows of the training histogram
+    uint32_t* rows_c;
 };
 
 static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
@@ -791,15 +916,37 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     io.blk_part = (double*)ctx->blk_part.ptr;
     io.sum_f = st->sum_f;
     io.n_f = st->n_f;
+    io.rec_bins = nullptr;
+    io.rec_c2 = nullptr;
+    io.rec_stride = 0;
     io.hpart_s = nullptr;
     io.hpart_c = nullptr;
     R->hist_tot = 0;
-    if (g.hist_lds && g.train_points) {
+    R->exact_train = (c->flags & VGX_FLAG_EXACT_ADAPT) ? 1 : 0;
+    R->rows_s = nullptr;
+    R->rows_c = nullptr;
+    if (g.train_points) {
+        // one record per point of a batch (at most cap points): bins, then
+        // (w f n)^2; rows of the fast-mode histogram per block
         R->hist_tot = c->dim * g.nb;
-        const size_t cells = (size_t)R->fused_blocks * (size_t)R->hist_tot;
-        if ((rc = ctx->hist_part.ensure(cells * (sizeof(double) + sizeof(uint32_t))))) return rc;
-        io.hpart_s = (double*)ctx->hist_part.ptr;
-        io.hpart_c = (uint32_t*)((double*)ctx->hist_part.ptr + cells);
+        const size_t stride = (R->cap + 63) & ~(size_t)63;  // 256-byte rows
+        const size_t wbytes = stride * (size_t)((c->dim + 1) / 2) * sizeof(uint32_t);
+        if ((rc = ctx->trec.ensure(wbytes + stride * sizeof(double)))) return rc;
+        io.rec_bins = (uint32_t*)ctx->trec.ptr;
+        io.rec_c2 = (double*)((char*)ctx->trec.ptr + wbytes);
+        io.rec_stride = (uint32_t)stride;
+        if (g.hist_lds) {
+            // one row per fused workgroup
+            const size_t cells = (size_t)R->fused_blocks * (size_t)R->hist_tot;
+            if ((rc = ctx->hist_part.ensure(cells * (sizeof(double) + sizeof(uint32_t))))) return rc;
+            io.hpart_s = (double*)ctx->hist_part.ptr;
+            io.hpart_c = (uint32_t*)((double*)ctx->hist_part.ptr + cells);
+        } else if (!R->exact_train) {
+            const size_t cells = (size_t)kTrainBlocks * (size_t)R->hist_tot;
+            if ((rc = ctx->hist_part.ensure(cells * (sizeof(double) + sizeof(uint32_t))))) return rc;
+            R->rows_s = (double*)ctx->hist_part.ptr;
+            R->rows_c = (uint32_t*)((double*)ctx->hist_part.ptr + cells);
+        }
     }
     io.errv = R->errv;
     io.sc = R->sc;
@@ -836,7 +983,7 @@ static int run_init(Run* R, vgx_state* st, hipStream_t s, int new_run) {
 static int alloc_kernels(vgx_ctx* ctx, Run* R, vgx_state* st, hipStream_t s) {
     const IterGeom& g = R->g;
     hipLaunchKernelGGL(alloc_kernel, dim3(R->nblk_alloc), dim3(kAllocThreads), 0, s, g, st->sigf,
-                       (const double*)st->sum_sigf, R->nh, R->stat, (const Scalars*)R->sc);
+                       (const double*)st->sum_sigf, R->nh, R->stat, (const Scalars*)R->sc, st->sum_f, st->n_f);
     hipLaunchKernelGGL(offsets_kernel, dim3(R->nblk_alloc), dim3(kAllocThreads), 0, s, g,
                        (const int32_t*)R->nh, (const uint64_t*)R->stat, R->nblk_alloc,
                        (uint32_t*)ctx->off.ptr, (int32_t*)ctx->tile_h.ptr, R->sc, R->cap);
@@ -849,14 +996,35 @@ static int alloc_kernels(vgx_ctx* ctx, Run* R, vgx_state* st, hipStream_t s) {
 template <class Fcn>
 static int accumulate_kernels(vgx_ctx* ctx, Run* R, const Fcn& fcn, hipStream_t s, int timed, int reduce) {
     const IterGeom& g = R->g;
-    const int rec = timed && ctx->timing_on && ctx->ntimed < vgx_ctx::kMaxTimed;
-    int rc = rec ? launch_fused(g, R->io, fcn, R->fused_blocks, R->lds, s, ctx->ev[2 * ctx->ntimed],
-                                ctx->ev[2 * ctx->ntimed + 1])
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool rec = timed && timed_slot(ctx, kTimedFused, &e0, &e1);
+    int rc = rec ? launch_fused(g, R->io, fcn, R->fused_blocks, R->lds, s, e0, e1)
                  : launch_fused(g, R->io, fcn, R->fused_blocks, R->lds, s);
     if (rc) return rc;
-    if (rec) ctx->ntimed++;
     if (R->io.hpart_s) {
+        // the fused launch's workgroup rows, summed in workgroup order
         const HistArgs h{R->io.hpart_s, R->io.hpart_c, R->fused_blocks, R->hist_tot, R->io.sum_f, R->io.n_f};
+        const int nh = (R->hist_tot + kHrCols - 1) / kHrCols;
+        hipLaunchKernelGGL(strad_hist_kernel, dim3(R->strad_blocks + nh), dim3(kStradThreads), 0, s, g, R->sa, h,
+                           R->strad_blocks);
+    } else if (R->io.rec_c2 && R->exact_train) {
+        // the reference's order: every bin summed over the batch's points in order
+        rc = train_hist_records(ctx, g.dim, g.ninc, g.nb, R->io.rec_bins, (int64_t)R->io.rec_stride, R->io.rec_c2,
+                                &R->sc->b_p0, (int64_t)R->cap, R->io.sum_f, R->io.n_f, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(strad_kernel, dim3(R->strad_blocks), dim3(kStradThreads), 0, s, g, R->sa);
+    } else if (R->io.rec_c2) {
+        const size_t lds = (size_t)g.nb * (sizeof(double) + sizeof(uint32_t));
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (timed && timed_slot(ctx, kTimedTrainRows, &t0, &t1))
+            hipExtLaunchKernelGGL(train_rows_kernel, dim3(kTrainBlocks * g.dim), dim3(64), (uint32_t)lds, s, t0, t1,
+                                  0, g, (const uint32_t*)R->io.rec_bins, (const double*)R->io.rec_c2,
+                                  R->io.rec_stride, R->rows_s, R->rows_c, (const Scalars*)R->sc);
+        else
+            hipLaunchKernelGGL(train_rows_kernel, dim3(kTrainBlocks * g.dim), dim3(64), lds, s, g,
+                               (const uint32_t*)R->io.rec_bins, (const double*)R->io.rec_c2, R->io.rec_stride,
+                               R->rows_s, R->rows_c, (const Scalars*)R->sc);
+        const HistArgs h{R->rows_s, R->rows_c, kTrainBlocks, R->hist_tot, R->io.sum_f, R->io.n_f};
         const int nh = (R->hist_tot + kHrCols - 1) / kHrCols;
         hipLaunchKernelGGL(strad_hist_kernel, dim3(R->strad_blocks + nh), dim3(kStradThreads), 0, s, g, R->sa, h,
                            R->strad_blocks);

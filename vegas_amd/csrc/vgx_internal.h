@@ -28,6 +28,15 @@ struct DeviceInfo {
 
 int device_info(DeviceInfo* out);
 
+}  // namespace vgx
+struct vgx_ctx;
+namespace vgx {
+// vgx_hist.hip: the training histogram of a fused-sampler batch from its
+// records, in point order (bit for bit the reference's sequential sums).
+int train_hist_records(vgx_ctx* ctx, int dim, const int32_t* ninc, int nb, const uint32_t* rbins,
+                       int64_t rstride, const double* c2, const uint32_t* nrange, int64_t cap, double* sum_f,
+                       double* n_f, hipStream_t st);
+
 // A growable device buffer owned by a context.
 struct Scratch {
     void* ptr = nullptr;
@@ -63,11 +72,38 @@ struct vgx_ctx {
     vgx::Scratch nh, off, tile_h, blocksum, tile_slots, blk_part, strad_part, misc, errv;
     // multi-component integrands: per-workgroup partials
     vgx::Scratch cov_part;
-    vgx::Scratch hist_part;  // per-workgroup training-histogram rows of the fused sampler
+    vgx::Scratch hist_part;  // per-block training-histogram rows of the fused sampler
+    vgx::Scratch trec;       // training records of the fused sampler (bins, (w f n)^2)
     // adapt
     vgx::Scratch adapt_tmp;
     static constexpr int kMaxTimed = 256;
     hipEvent_t ev[2 * kMaxTimed] = {};
+    int32_t tags[kMaxTimed] = {};  // which kernel each event pair brackets (vgx::Timed*)
     int timing_on = 0;
     int ntimed = 0;
 };
+
+namespace vgx {
+// Kernels whose launches can be timed with HIP events (vgx_set_timing).
+enum Timed : int32_t {
+    kTimedFused = 1,      // fused_kernel (single integral)
+    kTimedBatchFused = 2, // b_fused_kernel (batched integrals, eager calls)
+    kTimedFlowBwd = 3,    // flow_backward_kernel
+    kTimedFlowWgrad = 4,  // flow_wgrad_kernel
+    kTimedTrainRows = 5,  // train_rows_kernel
+    kTimedFlowSample = 6, // flow_sample_kernel
+    kTimedCov = 7         // the multi-component covariance pass
+};
+// The context whose timing is on (vgx_set_timing), for code without one.
+extern vgx_ctx* g_timing_ctx;
+// Events for the next launch of kernel `tag` when timing is on and the log
+// has room (then true); the launch goes through hipExtLaunchKernelGGL.
+inline bool timed_slot(vgx_ctx* c, int32_t tag, hipEvent_t* e0, hipEvent_t* e1) {
+    if (!c || !c->timing_on || c->ntimed >= vgx_ctx::kMaxTimed) return false;
+    *e0 = c->ev[2 * c->ntimed];
+    *e1 = c->ev[2 * c->ntimed + 1];
+    c->tags[c->ntimed] = tag;
+    c->ntimed++;
+    return true;
+}
+}  // namespace vgx

@@ -91,8 +91,13 @@ struct FusedPtrs {
     double* blk_part;   // [gridDim.x][6]
     double* sum_f;
     double* n_f;
-    double* hpart_s;    // LDS histogram: [gridDim.x][dim * nb] sums and counts of each
-    uint32_t* hpart_c;  //   workgroup, summed in row order by hist_reduce_kernel
+    double* hpart_s;    // LDS histogram (fast mode): [gridDim.x][dim * nb] sums and counts
+    uint32_t* hpart_c;  //   of each workgroup, summed in workgroup order after the launch
+    uint32_t* rec_bins; // training records, one per point of the batch (index i = point
+    double* rec_c2;     //   index - b_p0): the bins of axes 2k, 2k+1 as the u16 halves of
+                        //   rec_bins[k * rec_stride + i] (kNoBin where the point is not
+                        //   trained on the axis, pyx:459-463), (w f n)^2 (pyx:2173) at rec_c2[i]
+    uint32_t rec_stride; // a multiple of 4 (16-byte rows)
     double* errv;       // adapt_to_errors: |variance| per hcube
     Scalars* sc;
 };

@@ -129,7 +129,10 @@ class Integrator:
             self.nstrat = np.array(map.nstrat)
             self._sigf = None if map._sigf is None else map._sigf.clone()
             self.sum_sigf = map.sum_sigf
-            self._rng_iter = map._rng_iter
+            # a copy samples independently of its original, as copies that
+            # share the reference's global gvar.RNG do: a fresh seed unless
+            # seed= is given (pickling keeps the seed and stream position)
+            args['seed'] = None
             if 'nstrat' not in kargs:
                 kargs['nstrat'] = map.nstrat
                 if 'neval' not in kargs:
@@ -410,6 +413,7 @@ class Integrator:
             sum_f=hist[:nb],
             n_f=hist[nb:],
         )
+        self._train_hist = hist
         st = _lib.VgxState()
         st.grid = self.map._grid.data_ptr()
         st.sigf = self._sigf.data_ptr()
@@ -487,6 +491,14 @@ class Integrator:
         if self.beta > 0 and not self.adapt_to_errors and self.adapt:
             self.sum_sigf = float(res['sum_sigf'][0])
         self._last_device_ravg = res['ravg']
+
+    def _last_training(self):
+        """The training histogram (sum_f, n_f), each [dim, max(ninc)], of the
+        last iteration run on the device, as the adapt after it read it
+        (before AdaptiveMap.clear in the reference, pyx:2216-2217, 596-599)."""
+        hist = self._train_hist.cpu().numpy()
+        nb = len(hist) // 2
+        return hist[:nb].reshape(self.dim, -1), hist[nb:].reshape(self.dim, -1)
 
     def _run_device(self, torch, fcn, nitn, prec, result):
         st, bufs = self._state(torch, nitn)

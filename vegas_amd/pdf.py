@@ -512,7 +512,7 @@ class PDFIntegrator(Integrator):
                 vs[key] = _view(vsdev, a, b, shp)
                 for key2, shp2, a2, b2 in lay:
                     rows = (np.arange(a, b)[:, None] * k + np.arange(a2, b2)[None, :]).reshape(-1)
-                    blk = (b - a, b2 - a2) if (shp, shp2) != ((), ()) else ()
+                    blk = tuple(shp) + tuple(shp2)  # gvar.evalcov's block shape
                     vc[key, key2] = (GVar(cov[a, a2], math.sqrt(max(cov_err[rows[0], rows[0]], 0.0)))
                                      if blk == () else
                                      GVarArray(cov[a:b, a2:b2].reshape(-1), cov_err[np.ix_(rows, rows)], blk))

@@ -35,7 +35,7 @@ def ridge_8d(diagonal=False):
     B = np.full(1000, 1.0 / (2 * sigma ** 2))
     f = DiagonalRidge(A, B, t, 8) if diagonal else GaussianMixture(A, B, t[:, None] * np.ones(8))
     return dict(name="8-D ridge (1000 Gaussians)", dim=8, f=f, neval=2e7, nitn=20, exact=1.0,
-                precision="f32", ncomp=1000)
+                precision="f32", ncomp=1000, sigma=sigma)
 
 
 def ridge_9d(diagonal=False):
@@ -47,7 +47,7 @@ def ridge_9d(diagonal=False):
     B = np.full(500, 1.0 / (2 * sigma ** 2))
     f = DiagonalRidge(A, B, t, 9) if diagonal else GaussianMixture(A, B, t[:, None] * np.ones(9))
     return dict(name="9-D ridge (500 Gaussians)", dim=9, f=f, neval=2e7, nitn=20, exact=1.0,
-                precision="f32", ncomp=500)
+                precision="f32", ncomp=500, sigma=sigma)
 
 
 def flops_per_eval(dim, ncomp):
