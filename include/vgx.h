@@ -77,6 +77,18 @@ int vgx_train_hist(vgx_ctx* ctx, int dim, const int32_t* ninc, int ld,
                    const double* y, const double* f, int64_t n,
                    double* sum_f, double* n_f, int ordered, void* stream);
 
+/* No counterpart in the reference: vgx_train_hist's ordered mode for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments
+ * and results (bit for bit) without a context: its temporaries live in a
+ * static device array of the library, 128 MiB per GPU, held until the
+ * process ends; VGX_ERANGE when the call needs more (about
+ * dim*(8 + 4*ld/1024) bytes per point plus 8*dim*ld).  Every call on one
+ * GPU shares that array, so calls that overlap on one GPU race.  Allocates
+ * nothing and launches on `stream`.                                        */
+int vgx_train_hist_noalloc(int dim, const int32_t* ninc, int ld, const double* y, const double* f,
+                           int64_t n, double* sum_f, double* n_f, void* stream);
+
 /* AdaptiveMap.adapt(alpha, ninc)              -- _vegas.pyx:467-594
  * has_training=0 is the reference's sum_f None.  new_grid: device
  * double[dim][new_ld], new_ld >= max(new_ninc)+1.  exact=1 runs the

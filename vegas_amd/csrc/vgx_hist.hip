@@ -12,6 +12,8 @@
 //   2 hist_scan     exclusive scan over tiles per bin, then over bins
 //   3 hist_scatter  one wave per (tile, axis); stable ranks from ballots
 //   4 hist_sum      sequential per-bin sums in point order
+#include <mutex>
+
 #include "vgx_internal.h"
 
 namespace vgx {
@@ -308,3 +310,84 @@ int train_hist_records(vgx_ctx* ctx, int dim, const int32_t* ninc, int nb, const
 }
 
 }  // namespace vgx
+
+// ---- the ordered histogram without allocation -----------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, tight serving loops): the ordered mode of vgx_train_hist with its
+// temporaries (per-tile counts, bin starts and totals, the grouped values)
+// in this file-scope array, one copy per GPU, held until the process ends.
+// Its device address is looked up once per device (kernels take that
+// address, never the array's host-side name).
+namespace vgx {
+constexpr size_t kNoAllocScratch = (size_t)128 << 20;  // bytes per device
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+constexpr int kNoAllocDevices = 64;
+static void* g_scratch_dev[kNoAllocDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The current device's copy of `scratch` (its device address).
+static int scratch_for_current_device(unsigned char** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return VGX_EHIP;
+    if (dev < 0 || dev >= kNoAllocDevices) return VGX_ERANGE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return VGX_EHIP;
+        g_scratch_dev[dev] = p;
+    }
+    *out = (unsigned char*)g_scratch_dev[dev];
+    return VGX_OK;
+}
+
+// Bytes of scratch the ordered histogram of n points needs, laid out as
+// per-tile counts | bin starts and totals | grouped values (256-B aligned).
+static size_t noalloc_layout(int dim, int nb, int64_t n, size_t* o_start, size_t* o_sorted) {
+    const int64_t ntiles = (n + kHistTile - 1) / kHistTile;
+    const size_t cnt = (((size_t)dim * ntiles * nb * sizeof(uint32_t)) + 255) & ~(size_t)255;
+    const size_t st = ((2 * (size_t)dim * nb * sizeof(uint32_t)) + 255) & ~(size_t)255;
+    *o_start = cnt;
+    *o_sorted = cnt + st;
+    return cnt + st + (size_t)dim * n * sizeof(double);
+}
+}  // namespace vgx
+
+extern "C" int vgx_train_hist_noalloc(int dim, const int32_t* ninc, int ld, const double* y, const double* f,
+                                      int64_t n, double* sum_f, double* n_f, void* stream) {
+    if (dim < 1 || dim > VGX_MAXDIM || ld < 2 || n < 0 || !ninc) return VGX_EINVAL;
+    if (n == 0) return VGX_OK;
+    if (n > 0xFFFFFFFFll) return VGX_ERANGE;
+    HistArgs a{};
+    a.dim = dim;
+    a.nb = ld - 1;
+    a.y = y;
+    a.f = f;
+    a.n = n;
+    a.sum_f = sum_f;
+    a.n_f = n_f;
+    int maxn = 0;
+    for (int d = 0; d < dim; ++d) {
+        if (ninc[d] < 1 || ninc[d] > a.nb) return VGX_EINVAL;
+        a.ninc[d] = ninc[d];
+        maxn = ninc[d] > maxn ? ninc[d] : maxn;
+    }
+    if ((size_t)a.nb * sizeof(uint32_t) > 64 * 1024) return VGX_ERANGE;
+    size_t o_start = 0, o_sorted = 0;
+    if (noalloc_layout(dim, a.nb, n, &o_start, &o_sorted) > kNoAllocScratch) return VGX_ERANGE;
+    unsigned char* base = nullptr;
+    int rc = scratch_for_current_device(&base);
+    if (rc) return rc;
+    a.ntiles = (int)((n + kHistTile - 1) / kHistTile);
+    a.cnt = (uint32_t*)base;
+    a.start = (uint32_t*)(base + o_start);
+    a.total = a.start + (size_t)dim * a.nb;
+    a.sorted = (double*)(base + o_sorted);
+    hipStream_t st = (hipStream_t)stream;
+    const size_t lds = (size_t)a.nb * sizeof(uint32_t);
+    hipLaunchKernelGGL(hist_count_kernel, dim3(a.ntiles, dim), dim3(256), lds, st, a);
+    hipLaunchKernelGGL(hist_scan_kernel, dim3(dim), dim3(1024), 0, st, a);
+    hipLaunchKernelGGL(hist_scatter_kernel, dim3(a.ntiles, dim), dim3(64), lds, st, a);
+    hipLaunchKernelGGL(hist_sum_kernel, dim3((maxn + 255) / 256, dim), dim3(256), 0, st, a);
+    return check_launch();
+}
