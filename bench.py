@@ -536,6 +536,24 @@ def main():
                              "train_rows: the training histogram built from the sampler's records "
                              "(24 B per evaluation read)"},
     }
+    # the same iterations with the unordered training histogram (LDS atomics
+    # in timing order, Integrator(deterministic=False)): not part of value
+    ui = va.Integrator([[0, 1]] * wl["dim"], neval=wl["neval"], seed=2026, precision=wl["precision"],
+                       deterministic=False)
+    if a.warmup > 0:
+        ui(f, nitn=a.warmup)
+    torch.cuda.synchronize()
+    L.vgx_set_timing(cx, 1)
+    t0 = time.perf_counter()
+    ru = ui(f, nitn=a.steps)
+    torch.cuda.synchronize()
+    dtu = time.perf_counter() - t0
+    ums = kernel_ms("fused_kernel")
+    L.vgx_set_timing(cx, 0)
+    out["result"]["unordered_histogram"] = {
+        "evals_per_s": float(ru.sum_neval) / dtu, "ms_per_step": dtu * 1e3 / a.steps,
+        "fused_kernel_ms_avg": (sum(ums) / len(ums)) if ums else None, "mean": ru.mean, "sdev": ru.sdev,
+        "note": "Integrator(deterministic=False): training histogram by LDS atomics in timing order"}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_iters)
     else:

@@ -106,8 +106,11 @@ int vgx_adapt(vgx_ctx* ctx, int dim, const int32_t* ninc, const double* grid, in
 enum { VGX_PREC_F64 = 0, VGX_PREC_F32 = 1 };
 enum { VGX_FCN_GAUSSMIX = 1, VGX_FCN_DIAGRIDGE = 2, VGX_FCN_BUFFER = 3 };
 enum {
-    VGX_FLAG_EXACT_ADAPT = 1,  /* vgx_adapt(exact=1) inside the loop */
-    VGX_FLAG_EXACT_DIV = 2     /* y = (c+u)/s by IEEE division (as pyx:1741) */
+    VGX_FLAG_EXACT_ADAPT = 1,  /* vgx_adapt(exact=1) inside the loop, and the training
+                                  histogram summed in point order (pyx:457-463) */
+    VGX_FLAG_EXACT_DIV = 2,    /* y = (c+u)/s by IEEE division (as pyx:1741) */
+    VGX_FLAG_UNORDERED_HIST = 4 /* training histogram by LDS atomics in timing order:
+                                  faster, but not reproducible bit for bit */
 };
 
 typedef struct {

@@ -275,6 +275,22 @@ def test_training_histogram_fast_mode(va, orc):
     np.testing.assert_allclose(gs, sf, rtol=1e-12, atol=0)
 
 
+def test_unordered_histogram_option(va):
+    """deterministic=False (LDS atomics in timing order) trains on the same
+    points: the same first-iteration estimate, the same counts, sums to
+    rounding, and an estimate consistent with the default mode's."""
+    f = double_peak(va, dim=8, sigma=0.05)
+    a = va.Integrator([[0, 1]] * 8, neval=4e5, seed=77)
+    b = va.Integrator([[0, 1]] * 8, neval=4e5, seed=77, deterministic=False)
+    ra, rb = a(f, nitn=1), b(f, nitn=1)
+    assert ra.mean == rb.mean and ra.sdev == rb.sdev
+    (sa, na), (sb, nb) = a._last_training(), b._last_training()
+    assert np.array_equal(na, nb)
+    np.testing.assert_allclose(sb, sa, rtol=1e-12, atol=0)
+    ra, rb = a(f, nitn=3), b(f, nitn=3)
+    assert abs(ra.mean - rb.mean) < 5 * math.hypot(ra.sdev, rb.sdev)
+
+
 def test_y_zero_left_out_of_training(va, orc):
     """A point with y = 0 exactly (one stratum per axis, 24-bit uniform 0 on
     axis 0: seed 15, point 477932) stays out of axis 0's training histogram

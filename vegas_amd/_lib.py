@@ -17,7 +17,7 @@ VGX_OK, VGX_EINVAL, VGX_EHIP, VGX_ENAN, VGX_ENOMEM, VGX_ERANGE, VGX_ENODEV = 0, 
 VGX_ETIMEOUT = -7
 PREC_F64, PREC_F32 = 0, 1
 FCN_GAUSSMIX, FCN_DIAGRIDGE, FCN_BUFFER = 1, 2, 3
-FLAG_EXACT_ADAPT, FLAG_EXACT_DIV = 1, 2
+FLAG_EXACT_ADAPT, FLAG_EXACT_DIV, FLAG_UNORDERED_HIST = 1, 2, 4
 
 
 class VgxConfig(C.Structure):

@@ -316,18 +316,20 @@ template <int DIM>
 __device__ __forceinline__ void store_record(const TileIO& io, int dim, uint32_t ri, const int* kb,
                                              uint64_t zm, double c2) {
     constexpr int ND = nd_max(DIM);
-    uint32_t* dst = io.rec_bins + ri;
+    uint32_t h[2 * ((ND + 1) / 2)];
+#pragma unroll
+    for (int d = 0; d < 2 * ((ND + 1) / 2); ++d) h[d] = (d < ND && d < dim) ? (uint32_t)kb[d < ND ? d : 0] : kNoBin;
+    if (zm) {  // y = 0 exactly on some axis (rare): those axes are not trained on
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+            if ((zm >> d) & 1) h[d] = kNoBin;
+    }
 #pragma unroll
     for (int k = 0; k < (ND + 1) / 2; ++k) {
         if (DIM <= 0 && 2 * k >= dim) break;
-        uint32_t h[2];
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const int d = 2 * k + e;
-            h[e] = (d < ND && d < dim && kb[d < ND ? d : 0] >= 0 && !((zm >> d) & 1))
-                       ? (uint32_t)kb[d < ND ? d : 0] : kNoBin;
-        }
-        dst[(size_t)k * io.rec_stride] = h[0] | (h[1] << 16);
+        // a uniform row base and a 32-bit lane offset (no 64-bit address math per lane)
+        uint32_t* row = io.rec_bins + (size_t)k * io.rec_stride;
+        row[ri] = (h[2 * k] & 0xFFFFu) | (h[2 * k + 1] << 16);
     }
     io.rec_c2[ri] = c2;
 }
@@ -656,6 +658,7 @@ __device__ inline void counts_to_row(uint32_t* Lc, int tot, uint32_t* row, bool 
 constexpr uint32_t kTicketSpins = 1u << 20;
 
 __device__ __forceinline__ bool ticket_wait(uint32_t* tick, uint32_t want, uint32_t* broken) {
+#pragma unroll 1
     for (uint32_t spins = 0;; ++spins) {
         if (__hip_atomic_load(broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
         if (__hip_atomic_load(tick, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == want) return true;
@@ -678,31 +681,38 @@ __device__ __forceinline__ void tile_train_ordered(const TileGeom& g, const Tile
     constexpr int NW = (ND + 1) / 2;  // bin words per point (two axes each)
     // the record stores of this wave are complete before it reads them back
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-    const uint32_t r0 = ri0 + (uint32_t)lane * kK;  // the lane's first record
-    // every load of the round before the first ticket: one memory round trip
-    double cv[kK];
-    uint32_t wd[NW][kK];
-#pragma unroll
-    for (int j = 0; j < kK; ++j) {
-        const bool ok = have && (uint32_t)lane * kK + j < nvalid;
-        cv[j] = ok ? io.rec_c2[r0 + j] : 0.0;
+    // every load of the round before the first ticket (one memory round
+    // trip): the lane's 4 consecutive records as 16-byte loads (the rows are
+    // padded to whole tiles, so the loads stay inside them); records past
+    // the tile's last point are masked after the loads
+    const uint32_t r0 = ri0 + (uint32_t)lane * kK;
+    const uint32_t nmine = have && (uint32_t)lane * kK < nvalid ? min((uint32_t)kK, nvalid - (uint32_t)lane * kK) : 0u;
+    uint4 wd[NW];
+    double2 c01, c23;
+    if (have) {
 #pragma unroll
         for (int k = 0; k < NW; ++k) {
             if (DIM <= 0 && 2 * k >= dim) break;
-            wd[k][j] = ok ? io.rec_bins[(size_t)k * io.rec_stride + r0 + j] : (kNoBin | (kNoBin << 16));
+            const uint32_t* row = io.rec_bins + (size_t)k * io.rec_stride;
+            wd[k] = *reinterpret_cast<const uint4*>(row + r0);
         }
+        c01 = *reinterpret_cast<const double2*>(io.rec_c2 + r0);
+        c23 = *reinterpret_cast<const double2*>(io.rec_c2 + r0 + 2);
     }
+    const double cv[kK] = {c01.x, c01.y, c23.x, c23.y};
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         if (DIM <= 0 && d >= dim) break;
         if (ticket_wait(&tick[d], want, broken) && have) {
+            const uint4 w = wd[d >> 1];
+            const uint32_t ws[kK] = {w.x, w.y, w.z, w.w};
 #pragma unroll
             for (int j = 0; j < kK; ++j) {
-                const uint32_t kb = (wd[d >> 1][j] >> (16 * (d & 1))) & 0xFFFFu;
-                if (kb != kNoBin) {
+                const uint32_t kb = (ws[j] >> (16 * (d & 1))) & 0xFFFFu;
+                if ((uint32_t)j < nmine && kb != kNoBin) {
                     const int idx = d * g.nb + (int)kb;
                     unsafeAtomicAdd(&Ls[idx], cv[j]);
-                    atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
+                    atomicAdd(&Lc[idx >> 1], 1u << (16 * (idx & 1)));
                 }
             }
         }
@@ -794,7 +804,13 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     double* Lred = reinterpret_cast<double*>(smem + o16);
     __syncthreads();
 
-    const TileIO tio{io.off, io.sigf, io.slots, io.errv, io.rec_bins, io.rec_c2, p0, io.rec_stride};
+    // unordered histogram: the tile body adds each point into the LDS
+    // histogram at once (no records, no tickets)
+    const bool unordered = lds_hist && g.hist_unordered;
+    const TileIO tio{io.off, io.sigf, io.slots, io.errv, unordered ? nullptr : io.rec_bins,
+                     unordered ? nullptr : io.rec_c2, p0, io.rec_stride};
+    double* const Lsu = unordered ? Ls : nullptr;
+    uint32_t* const Lcu = unordered ? Lc : nullptr;
     const uint32_t chunk = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t tb = blockIdx.x * chunk;
     const uint32_t te = min(tb + chunk, ntiles);
@@ -815,6 +831,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     const uint32_t half = (uint32_t)nwave / 2;
     const bool early = (uint32_t)wave >= half;
     const uint32_t pos = early ? (uint32_t)wave - half : (uint32_t)wave + half;
+    const bool ordered = lds_hist && !unordered;
     bool have_prev = false;
     uint32_t t_prev = 0;
     auto train_prev = [&](uint32_t q) {
@@ -827,7 +844,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
         const uint32_t t = tb + r * nwave + wave;
         const bool have = t < te;
         if constexpr (PER) {
-            if (lds_hist && early && r > 0) train_prev(r - 1);
+            if (ordered && early && r > 0) train_prev(r - 1);
         }
         if (have) {
             const uint32_t plo = p0 + t * kT;
@@ -835,11 +852,11 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
             uint32_t ov[kOffRegs];
             load_offsets(io.off, tp.hf, hend, off_end, lane, ov);
             store_offsets(offs, lane, ov);
-            tile_body<DIM, PER>(tg, tio, fcn, ax, G, nullptr, nullptr, offs, tp, lane, acc);
+            tile_body<DIM, PER>(tg, tio, fcn, ax, G, Lsu, Lcu, offs, tp, lane, acc);
         }
         if constexpr (PER) {
-            if (lds_hist && r > 0) {
-                if (!early) train_prev(r - 1);
+            if (lds_hist && (r > 0 || unordered)) {
+                if (ordered && !early) train_prev(r - 1);
                 // rounds up to r - 1 are in the histogram here, for every wave
                 if (((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
                     __syncthreads();
@@ -853,7 +870,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
         t_prev = t;
     }
     if constexpr (PER) {
-        if (lds_hist && nrounds > 0) train_prev(nrounds - 1);  // the last round, early then late
+        if (ordered && nrounds > 0) train_prev(nrounds - 1);  // the last round, early then late
     }
 
     // ---- block epilogue ----

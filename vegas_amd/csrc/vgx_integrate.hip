@@ -827,6 +827,7 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
     // (the ordered LDS histogram is in the integer stratum path's kernels)
     g->hist_lds = g->train_points && g->per_all && !(c->flags & VGX_FLAG_EXACT_ADAPT) &&
                   (fixed + gbytes + hbytes <= info.max_lds);
+    g->hist_unordered = g->hist_lds && (c->flags & VGX_FLAG_UNORDERED_HIST) ? 1 : 0;
     // the training-row kernel's LDS row of one axis
     if ((size_t)g->nb * (sizeof(double) + sizeof(uint32_t)) > info.max_lds) return VGX_ERANGE;
     return VGX_OK;
@@ -929,7 +930,9 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
         // one record per point of a batch (at most cap points): bins, then
         // (w f n)^2; rows of the fast-mode histogram per block
         R->hist_tot = c->dim * g.nb;
-        const size_t stride = (R->cap + 63) & ~(size_t)63;  // 256-byte rows
+        // rows padded to whole 256-point tiles (the ordered histogram reads a
+        // lane's 4 records with one 16-byte load, also in a partial tile)
+        const size_t stride = (R->cap + kT - 1) / kT * kT;
         const size_t wbytes = stride * (size_t)((c->dim + 1) / 2) * sizeof(uint32_t);
         if ((rc = ctx->trec.ensure(wbytes + stride * sizeof(double)))) return rc;
         io.rec_bins = (uint32_t*)ctx->trec.ptr;

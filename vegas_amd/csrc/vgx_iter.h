@@ -30,6 +30,7 @@ struct IterGeom {
     int rng32;             // 24-bit uniforms (FP32 integrand) else 53-bit
     int exact_div;         // y = (c+u)/s by IEEE division
     int hist_lds, grid_lds;
+    int hist_unordered;    // hist_lds: LDS atomics in timing order (VGX_FLAG_UNORDERED_HIST)
     int weighted;
     int adapt;
     int fast_pow;          // sigf = |var|^(beta/2) in the sampler, FP32 log2/exp2
@@ -97,7 +98,7 @@ struct FusedPtrs {
     double* rec_c2;     //   index - b_p0): the bins of axes 2k, 2k+1 as the u16 halves of
                         //   rec_bins[k * rec_stride + i] (kNoBin where the point is not
                         //   trained on the axis, pyx:459-463), (w f n)^2 (pyx:2173) at rec_c2[i]
-    uint32_t rec_stride; // a multiple of 4 (16-byte rows)
+    uint32_t rec_stride; // a multiple of the tile size (whole tiles per row)
     double* errv;       // adapt_to_errors: |variance| per hcube
     Scalars* sc;
 };
