@@ -448,10 +448,22 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
             if (g.train_points) {
                 const double c2 = tn * tn;
                 if (io.rec_c2) {
-                    // a record per point; the histogram is built from them in
-                    // point order or in a fixed order (vgx_integrate.hip)
+                    // a record per point; the histogram sums are built from
+                    // them in a fixed order (here, in tile order, or after the
+                    // launch); the counts, integers, need no order and go
+                    // into the LDS at once when Lc is given
                     const uint64_t zm = (PER && zmin == 0) ? per_zero_axes<DIM>(g, pc, pj - tp.pshift) : 0ull;
                     store_record<DIM>(io, dim, pj - io.rec_p0, kb, zm, c2);
+                    if (Lc) {
+#pragma unroll
+                        for (int d = 0; d < ND; ++d) {
+                            if (DIM <= 0 && d >= dim) break;
+                            const int k = kb[d];
+                            if (k < 0 || ((zm >> d) & 1)) continue;
+                            const int idx = d * g.nb + k;
+                            atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
+                        }
+                    }
                 } else if (PER && zmin != 0) {
                     // LDS histogram, every axis inside (0,1): no per-axis test
 #pragma unroll
@@ -645,79 +657,159 @@ __device__ inline void counts_to_row(uint32_t* Lc, int tot, uint32_t* row, bool 
 }
 
 // ---- training histogram in a fixed order ----------------------------------
-// Each axis d has a ticket counter in LDS.  In round r wave w may add to the
-// bins of axis d only when the counter reads r * nwave + w, and moves it on
-// afterwards: every bin receives the additions of round 0 wave 0, round 0
-// wave 1, ..., round 1 wave 0, ..., each wave's in point order per lane
-// slot (the LDS resolves the lanes of one instruction that hit one bin in a
-// fixed order), so the histogram does not depend on timing.  The wait is
-// bounded: past kTicketSpins polls the workgroup is marked broken, every
-// later wait returns at once and the launch reports VGX_ETIMEOUT (a wave
-// always passes its tickets, with or without a tile, so this does not
-// happen in a correct launch).
-constexpr uint32_t kTicketSpins = 1u << 20;
+// The axes are taken in pairs (the two halves of one record word); each pair
+// k has its own adder lock and its own next tile.  A workgroup's tiles
+// n = 0, 1, ... (wave n % nwave computes tile n in round n / nwave) add
+// their (w f n)^2 to pair k's LDS sums strictly in tile order, each tile
+// axis by axis and lane slot by lane slot (the LDS resolves the lanes of one
+// instruction that hit one bin in a fixed order), so no bin's sum depends on
+// timing.  (The counts are integers: the tile body adds them at once.)  No
+// wave waits for another while it computes: a wave that has stored a tile's
+// records publishes its round count; then, for each pair in turn (starting
+// at a pair of its own), if the pair's next tile is complete and no other
+// wave holds the pair's lock, it adds that tile and the complete ones after
+// it (up to kHoldTiles) under the lock, kBatch tiles' records loaded at
+// once (the first batch before the lock is taken); different pairs are
+// added by different waves at once.  After
+// its last tile a wave keeps trying (up to 64 tiles a hold) until every
+// pair has every tile; that wait ends once the slowest wave's last tile is
+// stored.
+// It is bounded by wall-clock time (kDrainTicks of the 100 MHz clock), past
+// which the workgroup is marked broken and the launch reports VGX_ETIMEOUT.
+constexpr int kPairs = (VGX_MAXDIM + 1) / 2;
+struct HistSeq {
+    uint32_t prog[kFusedThreads / 64];  // rounds whose records each wave has stored
+    uint32_t word[kPairs];               // per pair: next tile << 1 | 1 while a wave adds
+};
+constexpr uint32_t kHoldTiles = 8;  // tiles per hold of a lock while the holder has tiles left to compute
+constexpr uint32_t kBatch = 4;      // tiles whose records a holder loads at once
+constexpr uint64_t kDrainTicks = 3000000000ull;  // 30 s
 
-__device__ __forceinline__ bool ticket_wait(uint32_t* tick, uint32_t want, uint32_t* broken) {
-#pragma unroll 1
-    for (uint32_t spins = 0;; ++spins) {
-        if (__hip_atomic_load(broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return false;
-        if (__hip_atomic_load(tick, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == want) return true;
-        if (spins >= kTicketSpins) {
-            __hip_atomic_store(broken, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            return false;
+// One lane's records of one tile for one axis pair: its 4 consecutive
+// points' bin words and (w f n)^2.
+struct PairRec {
+    uint4 wd;
+    double2 c01, c23;
+};
+
+// Loads the lane's records of axis pair k for the tile at batch point ri0
+// (16-byte loads: the rows are padded to whole tiles, so they stay inside
+// them).
+__device__ __forceinline__ void pair_load(const TileIO& io, int k, uint32_t ri0, int lane, PairRec& rc) {
+    const uint32_t r0 = ri0 + (uint32_t)lane * kK;
+    rc.wd = *reinterpret_cast<const uint4*>(io.rec_bins + (size_t)k * io.rec_stride + r0);
+    rc.c01 = *reinterpret_cast<const double2*>(io.rec_c2 + r0);
+    rc.c23 = *reinterpret_cast<const double2*>(io.rec_c2 + r0 + 2);
+}
+
+// The loaded records of a tile with nvalid points added to the LDS sums of
+// axes 2k and 2k+1 (< dim); records past the tile's last point are skipped.
+__device__ __forceinline__ void pair_add(const TileGeom& g, int dim, int k, const PairRec& rc, uint32_t nvalid,
+                                         double* Ls, int lane) {
+    const uint32_t nmine = (uint32_t)lane * kK < nvalid ? min((uint32_t)kK, nvalid - (uint32_t)lane * kK) : 0u;
+    const double cv[kK] = {rc.c01.x, rc.c01.y, rc.c23.x, rc.c23.y};
+    const uint32_t ws[kK] = {rc.wd.x, rc.wd.y, rc.wd.z, rc.wd.w};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int d = 2 * k + h;
+        if (d >= dim) break;
+#pragma unroll
+        for (int j = 0; j < kK; ++j) {
+            const uint32_t kb = (ws[j] >> (16 * h)) & 0xFFFFu;
+            if ((uint32_t)j < nmine && kb != kNoBin) unsafeAtomicAdd(&Ls[d * g.nb + (int)kb], cv[j]);
         }
-        __builtin_amdgcn_s_sleep(1);
     }
 }
 
-// Round r of wave `wave`: the records of its tile (if it had one) added to
-// the LDS histogram, axis by axis under the axis tickets.  The records were
-// written by these very lanes (each lane reads back its own 4 points).
-template <int DIM>
-__device__ __forceinline__ void tile_train_ordered(const TileGeom& g, const TileIO& io, int dim, bool have,
-                                                   uint32_t ri0, uint32_t nvalid, uint32_t want, uint32_t* tick,
-                                                   uint32_t* broken, double* Ls, uint32_t* Lc, int lane) {
-    constexpr int ND = nd_max(DIM);
-    constexpr int NW = (ND + 1) / 2;  // bin words per point (two axes each)
-    // the record stores of this wave are complete before it reads them back
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-    // every load of the round before the first ticket (one memory round
-    // trip): the lane's 4 consecutive records as 16-byte loads (the rows are
-    // padded to whole tiles, so the loads stay inside them); records past
-    // the tile's last point are masked after the loads
-    const uint32_t r0 = ri0 + (uint32_t)lane * kK;
-    const uint32_t nmine = have && (uint32_t)lane * kK < nvalid ? min((uint32_t)kK, nvalid - (uint32_t)lane * kK) : 0u;
-    uint4 wd[NW];
-    double2 c01, c23;
-    if (have) {
+// The number of consecutive complete tiles from tile n on, at most cap
+// (<= 64): lane i looks at tile n + i.
+__device__ __forceinline__ uint32_t ready_run(HistSeq& hs, uint32_t n, uint32_t ntile, uint32_t nwave, uint32_t cap,
+                                              int lane) {
+    const uint32_t t = n + (uint32_t)lane;
+    bool ok = false;
+    if ((uint32_t)lane < cap && t < ntile)
+        ok = __hip_atomic_load(&hs.prog[t % nwave], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > t / nwave;
+    const uint64_t run = ~__ballot(ok);
+    return run ? (uint32_t)__builtin_ctzll(run) : 64u;
+}
+
+// One try of the calling wave on axis pair k: from the pair's next tile on,
+// up to cap (<= 64) tiles in order while their records are complete.  The
+// pair's word holds its next tile and the lock bit, so taking the lock also
+// proves that no other wave has added those tiles since the snapshot.
+// ntile tiles in this workgroup from batch tile tb; npts points in the
+// batch.  Wave-uniform.
+__device__ __forceinline__ void pair_try_add(const TileGeom& g, const TileIO& io, int dim, int k, HistSeq& hs,
+                                             uint32_t tb, uint32_t ntile, uint32_t nwave, uint32_t npts,
+                                             double* Ls, int lane, uint32_t cap) {
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(&hs.word[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+    if (w0 & 1u) return;  // another wave adds
+    const uint32_t n = w0 >> 1;
+    const uint32_t m = __builtin_amdgcn_readfirstlane(ready_run(hs, n, ntile, nwave, cap, lane));
+    if (m == 0) return;
+    // the records of complete tiles never change: the first batch is loaded
+    // before the lock is taken
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    PairRec R[kBatch];
 #pragma unroll
-        for (int k = 0; k < NW; ++k) {
-            if (DIM <= 0 && 2 * k >= dim) break;
-            const uint32_t* row = io.rec_bins + (size_t)k * io.rec_stride;
-            wd[k] = *reinterpret_cast<const uint4*>(row + r0);
-        }
-        c01 = *reinterpret_cast<const double2*>(io.rec_c2 + r0);
-        c23 = *reinterpret_cast<const double2*>(io.rec_c2 + r0 + 2);
+    for (uint32_t q = 0; q < kBatch; ++q)
+        if (q < m) pair_load(io, k, (tb + n + q) * (uint32_t)kT, lane, R[q]);
+    uint32_t got = 0u;
+    if (lane == 0) {
+        uint32_t expect = w0;
+        got = __hip_atomic_compare_exchange_strong(&hs.word[k], &expect, w0 | 1u, __ATOMIC_ACQUIRE,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                  ? 1u
+                  : 0u;
     }
-    const double cv[kK] = {c01.x, c01.y, c23.x, c23.y};
+    got = __builtin_amdgcn_readfirstlane(got);
+    if (!got) return;
+#pragma unroll 1
+    for (uint32_t i = 0;;) {
+        const uint32_t nb = min(kBatch, m - i);
 #pragma unroll
-    for (int d = 0; d < ND; ++d) {
-        if (DIM <= 0 && d >= dim) break;
-        if (ticket_wait(&tick[d], want, broken) && have) {
-            const uint4 w = wd[d >> 1];
-            const uint32_t ws[kK] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-            for (int j = 0; j < kK; ++j) {
-                const uint32_t kb = (ws[j] >> (16 * (d & 1))) & 0xFFFFu;
-                if ((uint32_t)j < nmine && kb != kNoBin) {
-                    const int idx = d * g.nb + (int)kb;
-                    unsafeAtomicAdd(&Ls[idx], cv[j]);
-                    atomicAdd(&Lc[idx >> 1], 1u << (16 * (idx & 1)));
-                }
+        for (uint32_t q = 0; q < kBatch; ++q) {
+            if (q < nb) {
+                const uint32_t ri0 = (tb + n + i + q) * (uint32_t)kT;
+                pair_add(g, dim, k, R[q], min((uint32_t)kT, npts - ri0), Ls, lane);
             }
         }
-        __hip_atomic_store(&tick[d], want + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        i += nb;
+        if (i >= m) break;
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; ++q)
+            if (i + q < m) pair_load(io, k, (tb + n + i + q) * (uint32_t)kT, lane, R[q]);
     }
+    // the additions are complete before the next holder's
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_store(&hs.word[k], (n + m) << 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Whether the calling wave, at round count `mine`, trails the workgroup:
+// no wave has stored fewer rounds and some wave has stored more.
+__device__ __forceinline__ bool trails(HistSeq& hs, uint32_t mine, uint32_t nwave, int lane) {
+    uint32_t v = mine;
+    if ((uint32_t)lane < nwave) v = __hip_atomic_load(&hs.prog[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __ballot(v < mine) == 0 && __ballot(v > mine) != 0;
+}
+
+// One try on every axis pair, from pair `first` on; true when every pair
+// has every tile.
+__device__ __forceinline__ bool hist_try_add(const TileGeom& g, const TileIO& io, int dim, HistSeq& hs, uint32_t tb,
+                                             uint32_t ntile, uint32_t nwave, uint32_t npts, double* Ls, int lane,
+                                             int first, uint32_t cap) {
+    const int np = (dim + 1) / 2;
+    bool all = true;
+#pragma unroll 1
+    for (int i = 0; i < np; ++i) {
+        int k = first + i;
+        k = k >= np ? k - np : k;
+        pair_try_add(g, io, dim, k, hs, tb, ntile, nwave, npts, Ls, lane, cap);
+        all = all && (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&hs.word[k], __ATOMIC_RELAXED,
+                                                                        __HIP_MEMORY_SCOPE_WORKGROUP)) >> 1) >= ntile;
+    }
+    return all;
 }
 
 // Workgroup reduction of the wave accumulators into out[0..5] (DD mean,
@@ -754,6 +846,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_skip;
     __shared__ uint32_t s_broken;
+    __shared__ HistSeq hs;
     Scalars* sc = io.sc;
     // NaN seen earlier in this run: nothing more is done (read once, so that
     // every wave of the workgroup takes the same decision)
@@ -761,6 +854,8 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
         s_skip = sc->status;
         s_broken = 0u;
     }
+    if (threadIdx.x < kFusedThreads / 64) hs.prog[threadIdx.x] = 0u;
+    if (threadIdx.x < kPairs) hs.word[threadIdx.x] = 0u;
     __syncthreads();
     if (s_skip) return;
     const TileGeom tg = tile_geom(g);
@@ -772,8 +867,8 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     // after the launch
     const bool lds_hist = PER && g.hist_lds && g.train_points;
 
-    // ---- LDS carve-up: grid | histogram sums, u16 pair counts, axis
-    // tickets (training in the fixed order) | offsets slices | reduction
+    // ---- LDS carve-up: grid | histogram sums, u16 pair counts | offsets
+    // slices | reduction
     unsigned char* cur = smem;
     const AxisC* ax = io.axes;  // read by scalar loads (ldc), not staged
     double* Lg = reinterpret_cast<double*>(cur);
@@ -782,17 +877,13 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     cur += (size_t)dim * g.ld * sizeof(double);
     double* Ls = nullptr;
     uint32_t* Lc = nullptr;
-    uint32_t* tick = nullptr;
     if (lds_hist) {
         Ls = reinterpret_cast<double*>(cur);
         cur += (size_t)tot * sizeof(double);
         Lc = reinterpret_cast<uint32_t*>(cur);
         cur += (size_t)((tot + 1) / 2) * sizeof(uint32_t);
-        tick = reinterpret_cast<uint32_t*>(cur);
-        cur += (size_t)dim * sizeof(uint32_t);
         for (int i = threadIdx.x; i < tot; i += blockDim.x) Ls[i] = 0.0;
         for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) Lc[i] = 0u;
-        for (int i = threadIdx.x; i < dim; i += blockDim.x) tick[i] = 0u;
     }
     // offsets from smem rounded up to 16 B (pointer arithmetic on smem keeps
     // the accesses ds_* instead of flat)
@@ -805,12 +896,12 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     __syncthreads();
 
     // unordered histogram: the tile body adds each point into the LDS
-    // histogram at once (no records, no tickets)
+    // histogram at once (no records, no fixed order)
     const bool unordered = lds_hist && g.hist_unordered;
     const TileIO tio{io.off, io.sigf, io.slots, io.errv, unordered ? nullptr : io.rec_bins,
                      unordered ? nullptr : io.rec_c2, p0, io.rec_stride};
     double* const Lsu = unordered ? Ls : nullptr;
-    uint32_t* const Lcu = unordered ? Lc : nullptr;
+    uint32_t* const Lcu = lds_hist ? Lc : nullptr;
     const uint32_t chunk = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t tb = blockIdx.x * chunk;
     const uint32_t te = min(tb + chunk, ntiles);
@@ -819,33 +910,14 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
 
     TileAcc acc{{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, 0.0, 0};
     bool cnt_flushed = false;  // this workgroup's count row holds earlier rounds
-    // The histogram additions of round r are made while the next tile is
-    // computed, by the two halves of the workgroup at different times: waves
-    // nwave/2.. ("early") add round r - 1 before computing their tile of
-    // round r, waves 0..nwave/2-1 ("late") after it.  Each axis takes round
-    // q's additions in the order early waves, then late waves (its ticket
-    // position); an early wave's additions of round q wait only for round
-    // q - 1 to be complete, a late wave's only for the early waves of round
-    // q, so one half adds while the other computes and no wait depends on
-    // itself.  The records of the round wait in memory meanwhile.
-    const uint32_t half = (uint32_t)nwave / 2;
-    const bool early = (uint32_t)wave >= half;
-    const uint32_t pos = early ? (uint32_t)wave - half : (uint32_t)wave + half;
+    // ordered: the tiles enter the histogram in tile order through the adder
+    // lock (hist_try_add), from the records the tile body stores
     const bool ordered = lds_hist && !unordered;
-    bool have_prev = false;
-    uint32_t t_prev = 0;
-    auto train_prev = [&](uint32_t q) {
-        const uint32_t ri0 = have_prev ? t_prev * (uint32_t)kT : 0u;
-        const uint32_t nvalid = have_prev ? min((uint32_t)kT, p1 - p0 - ri0) : 0u;
-        tile_train_ordered<DIM>(tg, tio, dim, have_prev, ri0, nvalid, q * (uint32_t)nwave + pos, tick, &s_broken, Ls,
-                                Lc, lane);
-    };
+    const uint32_t ntile_wg = te > tb ? te - tb : 0u;
+    uint32_t* const crow = lds_hist ? io.hpart_c + (size_t)blockIdx.x * tot : nullptr;
     for (uint32_t r = 0; r < nrounds; ++r) {
         const uint32_t t = tb + r * nwave + wave;
         const bool have = t < te;
-        if constexpr (PER) {
-            if (ordered && early && r > 0) train_prev(r - 1);
-        }
         if (have) {
             const uint32_t plo = p0 + t * kT;
             const TilePos tp{t, plo, min(plo + (uint32_t)kT, p1), (uint32_t)io.tile_h[t], hend, off_end, 0u, 0u};
@@ -855,22 +927,48 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
             tile_body<DIM, PER>(tg, tio, fcn, ax, G, Lsu, Lcu, offs, tp, lane, acc);
         }
         if constexpr (PER) {
-            if (lds_hist && (r > 0 || unordered)) {
-                if (ordered && !early) train_prev(r - 1);
-                // rounds up to r - 1 are in the histogram here, for every wave
-                if (((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
-                    __syncthreads();
-                    counts_to_row(Lc, tot, io.hpart_c + (size_t)blockIdx.x * tot, cnt_flushed, true);
-                    cnt_flushed = true;
-                    __syncthreads();
+            if (ordered) {
+                if (have) {
+                    // the tile's records are stored before its round is published
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    if (lane == 0)
+                        __hip_atomic_store(&hs.prog[wave], r + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
+                // a trailing wave leaves the additions to the waves ahead of
+                // it (arbitration favours the older waves of a SIMD, so the
+                // same waves trail all along, and their time is the launch's)
+                if (!trails(hs, r + 1u, (uint32_t)nwave, lane))
+                    (void)hist_try_add(tg, tio, dim, hs, tb, ntile_wg, (uint32_t)nwave, p1 - p0, Ls, lane,
+                                       wave % ((dim + 1) / 2), kHoldTiles);
+            }
+            if (lds_hist && ((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
+                // the u16 counts into the workgroup's u32 row before they can overflow
+                __syncthreads();
+                counts_to_row(Lc, tot, crow, cnt_flushed, true);
+                cnt_flushed = true;
+                __syncthreads();
             }
         }
-        have_prev = have;
-        t_prev = t;
     }
     if constexpr (PER) {
-        if (ordered && nrounds > 0) train_prev(nrounds - 1);  // the last round, early then late
+        if (ordered) {
+            // every wave helps until the workgroup's last tile is in
+            const uint64_t t0 = wall_clock64();
+#pragma unroll 1
+            for (;;) {
+                if (__builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(&s_broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
+                    break;
+                if (hist_try_add(tg, tio, dim, hs, tb, ntile_wg, (uint32_t)nwave, p1 - p0, Ls, lane,
+                                 wave % ((dim + 1) / 2), 64u))
+                    break;
+                if (wall_clock64() - t0 > kDrainTicks) {
+                    if (lane == 0) __hip_atomic_store(&s_broken, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
     }
 
     // ---- block epilogue ----
@@ -880,7 +978,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
         // workgroup order after the launch
         double* srow = io.hpart_s + (size_t)blockIdx.x * tot;
         for (int i = threadIdx.x; i < tot; i += blockDim.x) srow[i] = Ls[i];
-        counts_to_row(Lc, tot, io.hpart_c + (size_t)blockIdx.x * tot, cnt_flushed, false);
+        counts_to_row(Lc, tot, crow, cnt_flushed, false);
         if (threadIdx.x == 0 && s_broken) atomicCAS(&sc->status, 0, (int32_t)VGX_ETIMEOUT);
     }
     block_close(acc, Lred, io.blk_part + 6 * (size_t)blockIdx.x, sc);

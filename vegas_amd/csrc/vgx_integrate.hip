@@ -816,12 +816,13 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
     g->nitn = (uint32_t)c->nitn;
     // LDS plan of the fused kernel: the FP64 grid, offsets slices and the
     // reduction area, and in fast mode the training histogram (FP64 sums, u16
-    // pair counts, axis tickets) when it fits; otherwise (and in exact mode)
-    // the histogram is built from the training records after the launch
+    // pair counts) when it fits; otherwise (and in exact mode) the histogram
+    // is built from the training records after the launch.  `fixed` includes
+    // the kernel's static LDS (flags and the adder state, under 512 bytes).
     const size_t tot = (size_t)c->dim * g->nb;
-    const size_t fixed = (size_t)16 * kNoff * sizeof(uint32_t) + 16 * 6 * sizeof(double) + 64;
+    const size_t fixed = (size_t)16 * kNoff * sizeof(uint32_t) + 16 * 6 * sizeof(double) + 512;
     const size_t gbytes = (size_t)c->dim * c->ld * sizeof(double);
-    const size_t hbytes = tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t) + c->dim * sizeof(uint32_t) + 32;
+    const size_t hbytes = tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t) + 32;
     g->grid_lds = 1;
     if (fixed + gbytes > info.max_lds) return VGX_ERANGE;
     // (the ordered LDS histogram is in the integer stratum path's kernels)
@@ -837,7 +838,7 @@ static size_t fused_lds(const IterGeom& g) {
     const size_t tot = (size_t)g.dim * g.nb;
     size_t b = (size_t)g.dim * g.ld * sizeof(double);
     if (g.hist_lds && g.train_points)
-        b += tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t) + g.dim * sizeof(uint32_t);
+        b += tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t);
     b = (b + 15) & ~(size_t)15;
     b += (size_t)16 * kNoff * sizeof(uint32_t);
     b = (b + 15) & ~(size_t)15;
