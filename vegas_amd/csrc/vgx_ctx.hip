@@ -63,7 +63,7 @@ extern "C" int vgx_ctx_destroy(vgx_ctx* c) {
     if (!c) return VGX_OK;
     (void)hipDeviceSynchronize();
     vgx::Scratch* all[] = {&c->hist_cnt, &c->hist_sorted, &c->hist_start, &c->nh, &c->off,
-                           &c->tile_h, &c->blocksum, &c->tile_slots, &c->blk_part,
+                           &c->tile_h, &c->blocksum, &c->tile_slots, &c->tile_part, &c->blk_part,
                            &c->strad_part, &c->misc, &c->adapt_tmp, &c->errv, &c->cov_part,
                            &c->hist_part, &c->trec};
     if (vgx::g_timing_ctx == c) vgx::g_timing_ctx = nullptr;

@@ -657,32 +657,36 @@ __device__ inline void counts_to_row(uint32_t* Lc, int tot, uint32_t* row, bool 
 }
 
 // ---- training histogram in a fixed order ----------------------------------
-// The axes are taken in pairs (the two halves of one record word); each pair
-// k has its own adder lock and its own next tile.  A workgroup's tiles
-// n = 0, 1, ... (wave n % nwave computes tile n in round n / nwave) add
-// their (w f n)^2 to pair k's LDS sums strictly in tile order, each tile
-// axis by axis and lane slot by lane slot (the LDS resolves the lanes of one
-// instruction that hit one bin in a fixed order), so no bin's sum depends on
-// timing.  (The counts are integers: the tile body adds them at once.)  No
-// wave waits for another while it computes: a wave that has stored a tile's
-// records publishes its round count; then, for each pair in turn (starting
+// The waves of a workgroup take its tiles n = 0, 1, ... in turn from an LDS
+// counter, so faster waves take more of them.  The axes are taken in pairs
+// (the two halves of one record word); each pair k has its own adder lock
+// and its own next tile.  The tiles add their (w f n)^2 to pair k's LDS sums
+// strictly in tile order, each tile axis by axis and lane slot by lane slot
+// (the LDS resolves the lanes of one instruction that hit one bin in a
+// fixed order), so no bin's sum depends on timing.  (The counts are
+// integers: the tile body adds them at once.)  No wave waits for another
+// while it computes: a wave that has stored a tile's records marks the tile
+// complete in a ring of kRing flags; then, for each pair in turn (starting
 // at a pair of its own), if the pair's next tile is complete and no other
 // wave holds the pair's lock, it adds that tile and the complete ones after
 // it (up to kHoldTiles) under the lock, kBatch tiles' records loaded at
 // once (the first batch before the lock is taken); different pairs are
-// added by different waves at once.  After
-// its last tile a wave keeps trying (up to 64 tiles a hold) until every
-// pair has every tile; that wait ends once the slowest wave's last tile is
-// stored.
-// It is bounded by wall-clock time (kDrainTicks of the 100 MHz clock), past
-// which the workgroup is marked broken and the launch reports VGX_ETIMEOUT.
+// added by different waves at once.  A wave about to take tile n >= kRing
+// first makes sure every pair has added tile n - kRing (whose flag slot it
+// reuses), adding meanwhile itself.  After its last tile a wave keeps
+// trying (up to 64 tiles a hold) until every pair has every tile.  Both
+// waits end once the tiles in flight are stored; they are bounded by
+// wall-clock time (kDrainTicks of the 100 MHz clock), past which the
+// workgroup is marked broken and the launch reports VGX_ETIMEOUT.
 constexpr int kPairs = (VGX_MAXDIM + 1) / 2;
+constexpr uint32_t kRing = 128;
 struct HistSeq {
-    uint32_t prog[kFusedThreads / 64];  // rounds whose records each wave has stored
-    uint32_t word[kPairs];               // per pair: next tile << 1 | 1 while a wave adds
+    uint32_t done[kRing];  // tile n complete (records stored): done[n % kRing] == n + 1
+    uint32_t word[kPairs];  // per pair: next tile << 1 | 1 while a wave adds
 };
 constexpr uint32_t kHoldTiles = 8;  // tiles per hold of a lock while the holder has tiles left to compute
 constexpr uint32_t kBatch = 4;      // tiles whose records a holder loads at once
+constexpr uint32_t kMinRun = 4;     // complete tiles a pair needs before a wave computing tiles takes its lock
 constexpr uint64_t kDrainTicks = 3000000000ull;  // 30 s
 
 // One lane's records of one tile for one axis pair: its 4 consecutive
@@ -723,12 +727,11 @@ __device__ __forceinline__ void pair_add(const TileGeom& g, int dim, int k, cons
 
 // The number of consecutive complete tiles from tile n on, at most cap
 // (<= 64): lane i looks at tile n + i.
-__device__ __forceinline__ uint32_t ready_run(HistSeq& hs, uint32_t n, uint32_t ntile, uint32_t nwave, uint32_t cap,
-                                              int lane) {
+__device__ __forceinline__ uint32_t ready_run(HistSeq& hs, uint32_t n, uint32_t ntile, uint32_t cap, int lane) {
     const uint32_t t = n + (uint32_t)lane;
     bool ok = false;
     if ((uint32_t)lane < cap && t < ntile)
-        ok = __hip_atomic_load(&hs.prog[t % nwave], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > t / nwave;
+        ok = __hip_atomic_load(&hs.done[t % kRing], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == t + 1u;
     const uint64_t run = ~__ballot(ok);
     return run ? (uint32_t)__builtin_ctzll(run) : 64u;
 }
@@ -740,13 +743,13 @@ __device__ __forceinline__ uint32_t ready_run(HistSeq& hs, uint32_t n, uint32_t 
 // ntile tiles in this workgroup from batch tile tb; npts points in the
 // batch.  Wave-uniform.
 __device__ __forceinline__ void pair_try_add(const TileGeom& g, const TileIO& io, int dim, int k, HistSeq& hs,
-                                             uint32_t tb, uint32_t ntile, uint32_t nwave, uint32_t npts,
-                                             double* Ls, int lane, uint32_t cap) {
+                                             uint32_t tb, uint32_t ntile, uint32_t npts, double* Ls, int lane,
+                                             uint32_t cap) {
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(
         __hip_atomic_load(&hs.word[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
     if (w0 & 1u) return;  // another wave adds
     const uint32_t n = w0 >> 1;
-    const uint32_t m = __builtin_amdgcn_readfirstlane(ready_run(hs, n, ntile, nwave, cap, lane));
+    const uint32_t m = __builtin_amdgcn_readfirstlane(ready_run(hs, n, ntile, cap, lane));
     if (m == 0) return;
     // the records of complete tiles never change: the first batch is loaded
     // before the lock is taken
@@ -786,30 +789,44 @@ __device__ __forceinline__ void pair_try_add(const TileGeom& g, const TileIO& io
     if (lane == 0) __hip_atomic_store(&hs.word[k], (n + m) << 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Whether the calling wave, at round count `mine`, trails the workgroup:
-// no wave has stored fewer rounds and some wave has stored more.
-__device__ __forceinline__ bool trails(HistSeq& hs, uint32_t mine, uint32_t nwave, int lane) {
-    uint32_t v = mine;
-    if ((uint32_t)lane < nwave) v = __hip_atomic_load(&hs.prog[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    return __ballot(v < mine) == 0 && __ballot(v > mine) != 0;
-}
-
-// One try on every axis pair, from pair `first` on; true when every pair
-// has every tile.
+// One try on every axis pair that is free and has its next tile complete
+// (and the one min_run - 1 tiles later: while tiles are still computed the
+// locks are taken for runs of several tiles), from pair `first` on; true
+// when every pair had every tile at the look.
 __device__ __forceinline__ bool hist_try_add(const TileGeom& g, const TileIO& io, int dim, HistSeq& hs, uint32_t tb,
-                                             uint32_t ntile, uint32_t nwave, uint32_t npts, double* Ls, int lane,
-                                             int first, uint32_t cap) {
+                                             uint32_t ntile, uint32_t npts, double* Ls, int lane, int first,
+                                             uint32_t cap, uint32_t min_run) {
     const int np = (dim + 1) / 2;
-    bool all = true;
+    // one look at every pair (lane k): finished, or free with its next tile
+    // complete (and tile min_run - 1 after it, or the last tile); only the
+    // latter are tried
+    bool fin = true, open = false;
+    if (lane < np) {
+        const uint32_t w = __hip_atomic_load(&hs.word[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t nx = w >> 1;
+        fin = nx >= ntile;
+        const uint32_t nl = min(nx + min_run, ntile) - 1u;  // needs !fin
+        open = !(w & 1u) && !fin &&
+               __hip_atomic_load(&hs.done[nx % kRing], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == nx + 1u &&
+               __hip_atomic_load(&hs.done[nl % kRing], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == nl + 1u;
+    }
+    if (__ballot(!fin) == 0) return true;
+    const uint64_t want = __ballot(open);
 #pragma unroll 1
     for (int i = 0; i < np; ++i) {
         int k = first + i;
         k = k >= np ? k - np : k;
-        pair_try_add(g, io, dim, k, hs, tb, ntile, nwave, npts, Ls, lane, cap);
-        all = all && (__builtin_amdgcn_readfirstlane(__hip_atomic_load(&hs.word[k], __ATOMIC_RELAXED,
-                                                                        __HIP_MEMORY_SCOPE_WORKGROUP)) >> 1) >= ntile;
+        if ((want >> k) & 1ull) pair_try_add(g, io, dim, k, hs, tb, ntile, npts, Ls, lane, cap);
     }
-    return all;
+    return false;
+}
+
+// Whether every pair has added every tile before tile `upto`.
+__device__ __forceinline__ bool added_before(HistSeq& hs, int dim, uint32_t upto, int lane) {
+    bool ok = true;
+    if (lane < (dim + 1) / 2)
+        ok = (__hip_atomic_load(&hs.word[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> 1) >= upto;
+    return __ballot(!ok) == 0;
 }
 
 // Workgroup reduction of the wave accumulators into out[0..5] (DD mean,
@@ -841,11 +858,44 @@ __device__ inline void block_close(TileAcc& acc, double* Lred, double* out, Scal
     }
 }
 
+// Workgroup close of the fused sampler: the DD sums of its tiles
+// tile_part[tb..te) in tile order into out[0..5] (mean, var, sigf sums; lane
+// l takes tiles tb + l, tb + l + 64, ..., then a fixed-order butterfly), and
+// the run-wide max |w f n| / NaN flag from the lanes' accumulators.
+__device__ inline void block_close_tiles(const TileAcc& acc, const double* tpart, uint32_t tb, uint32_t te,
+                                         double* out, Scalars* sc) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const double wmax = wave_max(acc.wmax);
+    const int anynan = __any(acc.nan_seen);
+    if (lane == 0) {
+        if (wmax > 0.0) atomicMax((unsigned long long*)&sc->wfn_max_bits,
+                                  (unsigned long long)__double_as_longlong(wmax));
+        if (anynan) atomicOr(&sc->status, (int32_t)VGX_ENAN);
+    }
+    __syncthreads();  // every tile's sums are stored
+    if (wave == 0) {
+        DD m{0, 0}, v{0, 0}, q{0, 0};
+        for (uint32_t t = tb + (uint32_t)lane; t < te; t += 64) {
+            const double* tp = tpart + 6 * (size_t)t;
+            m = dd_sum(m, DD{tp[0], tp[1]});
+            v = dd_sum(v, DD{tp[2], tp[3]});
+            q = dd_sum(q, DD{tp[4], tp[5]});
+        }
+        m = wave_sum_dd(m);
+        v = wave_sum_dd(v);
+        q = wave_sum_dd(q);
+        if (lane == 0) {
+            out[0] = m.hi; out[1] = m.lo; out[2] = v.hi; out[3] = v.lo; out[4] = q.hi; out[5] = q.lo;
+        }
+    }
+}
+
 template <int DIM, bool PER, class Fcn>
 __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedPtrs io, Fcn fcn) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_skip;
     __shared__ uint32_t s_broken;
+    __shared__ uint32_t s_tile;  // the workgroup's next tile to take
     __shared__ HistSeq hs;
     Scalars* sc = io.sc;
     // NaN seen earlier in this run: nothing more is done (read once, so that
@@ -853,8 +903,9 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     if (threadIdx.x == 0) {
         s_skip = sc->status;
         s_broken = 0u;
+        s_tile = 0u;
     }
-    if (threadIdx.x < kFusedThreads / 64) hs.prog[threadIdx.x] = 0u;
+    if (threadIdx.x < kRing) hs.done[threadIdx.x] = 0u;
     if (threadIdx.x < kPairs) hs.word[threadIdx.x] = 0u;
     __syncthreads();
     if (s_skip) return;
@@ -868,7 +919,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     const bool lds_hist = PER && g.hist_lds && g.train_points;
 
     // ---- LDS carve-up: grid | histogram sums, u16 pair counts | offsets
-    // slices | reduction
+    // slices
     unsigned char* cur = smem;
     const AxisC* ax = io.axes;  // read by scalar loads (ldc), not staged
     double* Lg = reinterpret_cast<double*>(cur);
@@ -891,9 +942,6 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nwave = blockDim.x >> 6;
     uint32_t* offs = reinterpret_cast<uint32_t*>(smem + o16) + wave * kNoff;
-    o16 = (o16 + (size_t)nwave * kNoff * sizeof(uint32_t) + 15) & ~(size_t)15;
-    double* Lred = reinterpret_cast<double*>(smem + o16);
-    __syncthreads();
 
     // unordered histogram: the tile body adds each point into the LDS
     // histogram at once (no records, no fixed order)
@@ -905,64 +953,104 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     const uint32_t chunk = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t tb = blockIdx.x * chunk;
     const uint32_t te = min(tb + chunk, ntiles);
-    const uint32_t nrounds = (chunk + nwave - 1) / nwave;
+    const uint32_t ntile_wg = te > tb ? te - tb : 0u;
     const uint32_t off_end = io.off[hend];
+    // ordered: the tiles enter the histogram in tile order through the pair
+    // locks (hist_try_add), from the records the tile body stores
+    const bool ordered = lds_hist && !unordered;
+    uint32_t* const crow = lds_hist ? io.hpart_c + (size_t)blockIdx.x * tot : nullptr;
+    // u16 LDS counts: a workgroup with more than kCntFlushTiles tiles moves
+    // them into its u32 row (zeroed here) every kCntFlushTiles tiles taken,
+    // before any can overflow: between two moves at most kCntFlushTiles + 15
+    // tiles (15 more waves' tiles in flight) add to a bin, 256 points each
+    const bool cnt_moves = lds_hist && ntile_wg > kCntFlushTiles;
+    if (cnt_moves) {
+        for (int i = threadIdx.x; i < tot; i += blockDim.x)
+            __hip_atomic_store(&crow[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+    }
+    __syncthreads();
 
     TileAcc acc{{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, 0.0, 0};
-    bool cnt_flushed = false;  // this workgroup's count row holds earlier rounds
-    // ordered: the tiles enter the histogram in tile order through the adder
-    // lock (hist_try_add), from the records the tile body stores
-    const bool ordered = lds_hist && !unordered;
-    const uint32_t ntile_wg = te > tb ? te - tb : 0u;
-    uint32_t* const crow = lds_hist ? io.hpart_c + (size_t)blockIdx.x * tot : nullptr;
-    for (uint32_t r = 0; r < nrounds; ++r) {
-        const uint32_t t = tb + r * nwave + wave;
-        const bool have = t < te;
-        if (have) {
-            const uint32_t plo = p0 + t * kT;
-            const TilePos tp{t, plo, min(plo + (uint32_t)kT, p1), (uint32_t)io.tile_h[t], hend, off_end, 0u, 0u};
-            uint32_t ov[kOffRegs];
-            load_offsets(io.off, tp.hf, hend, off_end, lane, ov);
-            store_offsets(offs, lane, ov);
-            tile_body<DIM, PER>(tg, tio, fcn, ax, G, Lsu, Lcu, offs, tp, lane, acc);
+#pragma unroll 1
+    for (;;) {
+        uint32_t n = 0u;
+        if (lane == 0) n = atomicAdd(&s_tile, 1u);
+        n = __builtin_amdgcn_readfirstlane(n);
+        if (n >= ntile_wg) break;
+        if constexpr (PER) {
+            if (ordered && n >= kRing) {
+                // tile n - kRing's flag slot is reused: every pair has added it
+                const uint64_t t_wait0 = wall_clock64();
+#pragma unroll 1
+                while (!added_before(hs, dim, n - kRing + 1u, lane)) {
+                    if (__builtin_amdgcn_readfirstlane(
+                            __hip_atomic_load(&s_broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
+                        break;
+                    (void)hist_try_add(tg, tio, dim, hs, tb, ntile_wg, p1 - p0, Ls, lane, wave % ((dim + 1) / 2),
+                                       64u, 1u);
+                    if (wall_clock64() - t_wait0 > kDrainTicks) {
+                        if (lane == 0)
+                            __hip_atomic_store(&s_broken, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                if (__builtin_amdgcn_readfirstlane(
+                        __hip_atomic_load(&s_broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
+                    break;
+            }
+            if (cnt_moves && n % kCntFlushTiles == kCntFlushTiles - 1) {
+                // each word taken atomically: counts added meanwhile stay
+#pragma unroll 1
+                for (int i = lane; i < (tot + 1) / 2; i += 64) {
+                    const uint32_t wv = atomicExch(&Lc[i], 0u);
+                    if (wv & 0xffffu) atomicAdd(&crow[2 * i], wv & 0xffffu);
+                    if ((wv >> 16) && 2 * i + 1 < tot) atomicAdd(&crow[2 * i + 1], wv >> 16);
+                }
+            }
+        }
+        const uint32_t t = tb + n;
+        const uint32_t plo = p0 + t * kT;
+        const TilePos tp{t, plo, min(plo + (uint32_t)kT, p1), (uint32_t)io.tile_h[t], hend, off_end, 0u, 0u};
+        uint32_t ov[kOffRegs];
+        load_offsets(io.off, tp.hf, hend, off_end, lane, ov);
+        store_offsets(offs, lane, ov);
+        tile_body<DIM, PER>(tg, tio, fcn, ax, G, Lsu, Lcu, offs, tp, lane, acc);
+        {
+            // the tile's DD sums (in lane 63), summed in tile order at the close
+            const DD m = wave_total_dd(acc.am), v = wave_total_dd(acc.av), q = wave_total_dd(acc.as);
+            if (lane == 63) {
+                double* o = io.tile_part + 6 * (size_t)t;
+                o[0] = m.hi; o[1] = m.lo; o[2] = v.hi; o[3] = v.lo; o[4] = q.hi; o[5] = q.lo;
+            }
+            acc.am = DD{0.0, 0.0};
+            acc.av = DD{0.0, 0.0};
+            acc.as = DD{0.0, 0.0};
         }
         if constexpr (PER) {
             if (ordered) {
-                if (have) {
-                    // the tile's records are stored before its round is published
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    if (lane == 0)
-                        __hip_atomic_store(&hs.prog[wave], r + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                }
-                // a trailing wave leaves the additions to the waves ahead of
-                // it (arbitration favours the older waves of a SIMD, so the
-                // same waves trail all along, and their time is the launch's)
-                if (!trails(hs, r + 1u, (uint32_t)nwave, lane))
-                    (void)hist_try_add(tg, tio, dim, hs, tb, ntile_wg, (uint32_t)nwave, p1 - p0, Ls, lane,
-                                       wave % ((dim + 1) / 2), kHoldTiles);
-            }
-            if (lds_hist && ((r + 1) % kCntFlushRounds == 0) && r + 1 < nrounds) {
-                // the u16 counts into the workgroup's u32 row before they can overflow
-                __syncthreads();
-                counts_to_row(Lc, tot, crow, cnt_flushed, true);
-                cnt_flushed = true;
-                __syncthreads();
+                // the tile's records are stored before it is marked complete
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (lane == 0)
+                    __hip_atomic_store(&hs.done[n % kRing], n + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                (void)hist_try_add(tg, tio, dim, hs, tb, ntile_wg, p1 - p0, Ls, lane, wave % ((dim + 1) / 2),
+                                   kHoldTiles, kMinRun);
             }
         }
     }
     if constexpr (PER) {
         if (ordered) {
             // every wave helps until the workgroup's last tile is in
-            const uint64_t t0 = wall_clock64();
+            const uint64_t t_wait0 = wall_clock64();
 #pragma unroll 1
             for (;;) {
                 if (__builtin_amdgcn_readfirstlane(
                         __hip_atomic_load(&s_broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
                     break;
-                if (hist_try_add(tg, tio, dim, hs, tb, ntile_wg, (uint32_t)nwave, p1 - p0, Ls, lane,
-                                 wave % ((dim + 1) / 2), 64u))
+                if (hist_try_add(tg, tio, dim, hs, tb, ntile_wg, p1 - p0, Ls, lane, wave % ((dim + 1) / 2), 64u, 1u))
                     break;
-                if (wall_clock64() - t0 > kDrainTicks) {
+                if (wall_clock64() - t_wait0 > kDrainTicks) {
                     if (lane == 0) __hip_atomic_store(&s_broken, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     break;
                 }
@@ -974,14 +1062,23 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     // ---- block epilogue ----
     __syncthreads();
     if (lds_hist) {
-        // this workgroup's row of the histogram: plain stores, summed in
+        // this workgroup's row of the histogram: plain stores (counts added
+        // to the row when they were moved there before), summed in
         // workgroup order after the launch
         double* srow = io.hpart_s + (size_t)blockIdx.x * tot;
         for (int i = threadIdx.x; i < tot; i += blockDim.x) srow[i] = Ls[i];
-        counts_to_row(Lc, tot, crow, cnt_flushed, false);
+        if (cnt_moves) {
+            for (int i = threadIdx.x; i < (tot + 1) / 2; i += blockDim.x) {
+                const uint32_t wv = Lc[i];
+                if (wv & 0xffffu) atomicAdd(&crow[2 * i], wv & 0xffffu);
+                if ((wv >> 16) && 2 * i + 1 < tot) atomicAdd(&crow[2 * i + 1], wv >> 16);
+            }
+        } else {
+            counts_to_row(Lc, tot, crow, false, false);
+        }
         if (threadIdx.x == 0 && s_broken) atomicCAS(&sc->status, 0, (int32_t)VGX_ETIMEOUT);
     }
-    block_close(acc, Lred, io.blk_part + 6 * (size_t)blockIdx.x, sc);
+    block_close_tiles(acc, io.tile_part, tb, te, io.blk_part + 6 * (size_t)blockIdx.x, sc);
 }
 
 template <int DIM, class Fcn>

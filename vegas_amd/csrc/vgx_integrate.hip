@@ -814,13 +814,13 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
     g->seed_lo = (uint32_t)(c->seed & 0xffffffffu);
     g->seed_hi = (uint32_t)(c->seed >> 32);
     g->nitn = (uint32_t)c->nitn;
-    // LDS plan of the fused kernel: the FP64 grid, offsets slices and the
-    // reduction area, and in fast mode the training histogram (FP64 sums, u16
-    // pair counts) when it fits; otherwise (and in exact mode) the histogram
-    // is built from the training records after the launch.  `fixed` includes
-    // the kernel's static LDS (flags and the adder state, under 512 bytes).
+    // LDS plan of the fused kernel: the FP64 grid, offsets slices, and in
+    // fast mode the training histogram (FP64 sums, u16 pair counts) when it
+    // fits; otherwise (and in exact mode) the histogram is built from the
+    // training records after the launch.  `fixed` includes the kernel's
+    // static LDS (flags, the tile counter and the adder state, under 1 KiB).
     const size_t tot = (size_t)c->dim * g->nb;
-    const size_t fixed = (size_t)16 * kNoff * sizeof(uint32_t) + 16 * 6 * sizeof(double) + 512;
+    const size_t fixed = (size_t)16 * kNoff * sizeof(uint32_t) + 1024;
     const size_t gbytes = (size_t)c->dim * c->ld * sizeof(double);
     const size_t hbytes = tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t) + 32;
     g->grid_lds = 1;
@@ -841,8 +841,6 @@ static size_t fused_lds(const IterGeom& g) {
         b += tot * sizeof(double) + ((tot + 1) / 2) * sizeof(uint32_t);
     b = (b + 15) & ~(size_t)15;
     b += (size_t)16 * kNoff * sizeof(uint32_t);
-    b = (b + 15) & ~(size_t)15;
-    b += 16 * 6 * sizeof(double);
     return b;
 }
 
@@ -892,6 +890,7 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     if ((rc = ctx->blocksum.ensure((size_t)R->nblk_alloc * 3 * 8))) return rc;
     if ((rc = ctx->tile_h.ensure((size_t)R->maxtiles * 4))) return rc;
     if ((rc = ctx->tile_slots.ensure((size_t)R->maxtiles * 2 * sizeof(Slot)))) return rc;
+    if ((rc = ctx->tile_part.ensure((size_t)R->maxtiles * 6 * sizeof(double)))) return rc;
     if ((rc = ctx->blk_part.ensure((size_t)R->fused_blocks * 6 * 8))) return rc;
     if ((rc = ctx->strad_part.ensure((size_t)R->strad_blocks * 6 * 8))) return rc;
     R->errv = nullptr;
@@ -915,6 +914,7 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     io.tile_h = (const int32_t*)ctx->tile_h.ptr;
     io.sigf = st->sigf;
     io.slots = (Slot*)ctx->tile_slots.ptr;
+    io.tile_part = (double*)ctx->tile_part.ptr;
     io.blk_part = (double*)ctx->blk_part.ptr;
     io.sum_f = st->sum_f;
     io.n_f = st->n_f;

@@ -69,7 +69,7 @@ struct vgx_ctx {
     // ordered training histogram
     vgx::Scratch hist_cnt, hist_sorted, hist_start;
     // integrator
-    vgx::Scratch nh, off, tile_h, blocksum, tile_slots, blk_part, strad_part, misc, errv;
+    vgx::Scratch nh, off, tile_h, blocksum, tile_slots, tile_part, blk_part, strad_part, misc, errv;
     // multi-component integrands: per-workgroup partials
     vgx::Scratch cov_part;
     vgx::Scratch hist_part;  // per-block training-histogram rows of the fused sampler

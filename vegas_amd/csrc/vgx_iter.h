@@ -11,7 +11,7 @@ constexpr int kFusedThreads = 1024; // 16 waves per workgroup, one workgroup per
 constexpr int kAllocThreads = 256;
 constexpr int kAllocItems = 8;
 constexpr int kAllocPerBlock = kAllocThreads * kAllocItems;
-constexpr int kCntFlushRounds = 15; // u16 LDS counts: 15 rounds x 16 tiles x 256 < 65536
+constexpr uint32_t kCntFlushTiles = 224; // u16 LDS counts: (224 + 15) tiles x 256 < 65536
 constexpr int kStradThreads = 256;  // straddler kernel: one lane per tile slot
 constexpr int kLaneParts = 8;       // cut hcubes of up to this many tiles: one lane; more: the wave
 constexpr int kKeySentinel = 0x7fff;
@@ -89,6 +89,7 @@ struct FusedPtrs {
     const int32_t* tile_h;
     double* sigf;
     Slot* slots;        // [2 * tile]: part of the hcube cut by the tile's start / end
+    double* tile_part;  // [tile][6]: the tile's DD sums (mean, variance, sigf), summed in tile order
     double* blk_part;   // [gridDim.x][6]
     double* sum_f;
     double* n_f;
