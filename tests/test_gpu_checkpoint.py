@@ -113,10 +113,8 @@ def test_save_and_saveall_then_extend(va, tmp_path):
 def test_resume_continues_exactly(va, tmp_path):
     """An integrator pickled after 3 iterations and restored continues with
     the same random stream, grid and stratification weights: the next
-    iteration gives bit-identical results to the original's.  Later ones
-    agree to rounding only: the fused sampler adds its per-workgroup
-    training histograms with FP64 atomics, in an order that varies from run
-    to run, so two runs of one integrator differ there as well."""
+    iterations give bit-identical results to the original's, and so does
+    the grid they adapt (the training histogram is built in a fixed order)."""
     f = va.GaussianMixture([1.0], [40.0], [[0.4, 0.6, 0.5]])
     a = va.Integrator([[0, 1]] * 3, neval=20000, seed=77)
     a(f, nitn=3)
@@ -131,8 +129,7 @@ def test_resume_continues_exactly(va, tmp_path):
     rb = b(f, nitn=2)
     assert ra.itn_results[0].mean == rb.itn_results[0].mean
     assert ra.itn_results[0].sdev == rb.itn_results[0].sdev
-    np.testing.assert_allclose(ra.itn_results[1].mean, rb.itn_results[1].mean, rtol=1e-12)
-    np.testing.assert_allclose(ra.itn_results[1].sdev, rb.itn_results[1].sdev, rtol=1e-9)
-    for ga, gb in zip(a.map.extract_grid(), b.map.extract_grid()):
-        np.testing.assert_allclose(ga, gb, rtol=1e-12, atol=1e-15)
+    assert ra.itn_results[1].mean == rb.itn_results[1].mean
+    assert ra.itn_results[1].sdev == rb.itn_results[1].sdev
+    assert a.map.extract_grid() == b.map.extract_grid()
     os.remove(fn)

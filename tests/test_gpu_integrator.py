@@ -291,6 +291,30 @@ def test_unordered_histogram_option(va):
     assert abs(ra.mean - rb.mean) < 5 * math.hypot(ra.sdev, rb.sdev)
 
 
+def test_training_histogram_large_batch(va):
+    """One batch of 2e7 points in 8-D (FP32 integrand, integer stratum path):
+    each workgroup takes more than 224 tiles, so the u16 LDS counts are moved
+    into its u32 row while the tiles are still computed.  The default
+    (tile-ordered) histogram is bit for bit the same on a second run with the
+    same seed; the unordered one has the same counts and sums to rounding;
+    every axis' counts add up to the points sampled (less the rare y = 0
+    exactly, which train no bin)."""
+    f = double_peak(va, dim=8, sigma=0.05)
+    a = va.Integrator([[0, 1]] * 8, neval=2e7, seed=31)
+    b = va.Integrator([[0, 1]] * 8, neval=2e7, seed=31)
+    c = va.Integrator([[0, 1]] * 8, neval=2e7, seed=31, deterministic=False)
+    ra, rb, rc = a(f, nitn=1), b(f, nitn=1), c(f, nitn=1)
+    assert ra.mean == rb.mean == rc.mean and ra.sdev == rb.sdev == rc.sdev
+    (sa, na), (sb, nb), (sc, nc) = a._last_training(), b._last_training(), c._last_training()
+    assert np.array_equal(sa, sb) and np.array_equal(na, nb)
+    assert np.array_equal(na, nc)
+    np.testing.assert_allclose(sc, sa, rtol=1e-12, atol=0)
+    npts = a.last_neval
+    assert npts > 224 * 256 * 256
+    tot = na.sum(axis=1)
+    assert np.all(tot <= npts) and np.all(tot >= npts - 200), (tot, npts)
+
+
 def test_y_zero_left_out_of_training(va, orc):
     """A point with y = 0 exactly (one stratum per axis, 24-bit uniform 0 on
     axis 0: seed 15, point 477932) stays out of axis 0's training histogram
