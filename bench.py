@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-iters", type=int, default=5)
+    p.add_argument("--no-unordered", action="store_true",
+                   help="double_peak_8d: skip the unordered-histogram comparison run (profiling)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 512 MiB device to device 4 times (HBM counter calibration)")
     p.add_argument("--workload", default="double_peak_8d",
@@ -538,6 +540,10 @@ def main():
     }
     # the same iterations with the unordered training histogram (LDS atomics
     # in timing order, Integrator(deterministic=False)): not part of value
+    if a.no_unordered:
+        out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(wl, a.cpu_iters)
+        print(json.dumps(out))
+        return
     ui = va.Integrator([[0, 1]] * wl["dim"], neval=wl["neval"], seed=2026, precision=wl["precision"],
                        deterministic=False)
     if a.warmup > 0:

@@ -182,25 +182,18 @@ __device__ __forceinline__ double dpp_d(double old, double x) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// DD value of the source lane (old where there is none or the row is masked).
-template <int CTRL, int ROWS = 0xF>
-__device__ __forceinline__ DD dpp_dd(DD old, DD x) {
-    return DD{dpp_d<CTRL, ROWS>(old.hi, x.hi), dpp_d<CTRL, ROWS>(old.lo, x.lo)};
-}
-
-// The DD sum of the 64 lanes' values in lane 63, through DPP moves (no LDS):
-// quads (xor 1, xor 2), halves and rows (mirrors), then rows 1 and 3 take
-// rows 0 and 2 (lane 15 broadcast) and rows 2 and 3 take row 1 (lane 31).
-// dd_sum is commutative bit for bit, so the lanes of one row agree and the
-// result is a fixed function of the 64 inputs.
-__device__ inline DD wave_total_dd(DD v) {
-    const DD z{0.0, 0.0};
-    v = dd_sum(v, dpp_dd<0xB1>(z, v));    // quad_perm [1,0,3,2]
-    v = dd_sum(v, dpp_dd<0x4E>(z, v));    // quad_perm [2,3,0,1]
-    v = dd_sum(v, dpp_dd<0x141>(z, v));   // row_half_mirror
-    v = dd_sum(v, dpp_dd<0x140>(z, v));   // row_mirror
-    v = dd_sum(v, dpp_dd<kDppRowBcast15, 0xA>(z, v));
-    v = dd_sum(v, dpp_dd<kDppRowBcast31, 0xC>(z, v));
+// The FP64 sum of the 64 lanes' values in lane 63, through DPP moves (no
+// LDS): quads (xor 1, xor 2), halves and rows (mirrors), then rows 1 and 3
+// take rows 0 and 2 (lane 15 broadcast) and rows 2 and 3 take row 1 (lane
+// 31).  FP64 addition is commutative, so the lanes of one row agree and the
+// result is a fixed function of the 64 inputs (a fixed summation tree).
+__device__ inline double wave_total(double v) {
+    v += dpp_d<0xB1>(0.0, v);    // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(0.0, v);    // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(0.0, v);   // row_half_mirror
+    v += dpp_d<0x140>(0.0, v);   // row_mirror
+    v += dpp_d<kDppRowBcast15, 0xA>(0.0, v);
+    v += dpp_d<kDppRowBcast31, 0xC>(0.0, v);
     return v;
 }
 

@@ -688,6 +688,9 @@ constexpr uint32_t kHoldTiles = 8;  // tiles per hold of a lock while the holder
 constexpr uint32_t kBatch = 4;      // tiles whose records a holder loads at once
 constexpr uint32_t kMinRun = 4;     // complete tiles a pair needs before a wave computing tiles takes its lock
 constexpr uint64_t kDrainTicks = 3000000000ull;  // 30 s
+// s_sleep argument between polls of a waiting wave (64 clocks each): a poll's
+// instructions and LDS reads take issue slots from the waves still computing
+constexpr int kPollSleep = 16;
 
 // One lane's records of one tile for one axis pair: its 4 consecutive
 // points' bin words and (w f n)^2.
@@ -858,9 +861,10 @@ __device__ inline void block_close(TileAcc& acc, double* Lred, double* out, Scal
     }
 }
 
-// Workgroup close of the fused sampler: the DD sums of its tiles
-// tile_part[tb..te) in tile order into out[0..5] (mean, var, sigf sums; lane
-// l takes tiles tb + l, tb + l + 64, ..., then a fixed-order butterfly), and
+// Workgroup close of the fused sampler: the sums of its tiles
+// tile_part[tb..te) in tile order, in DD, into out[0..5] (mean, var, sigf
+// sums; lane l takes tiles tb + l, tb + l + 64, ..., then a fixed-order
+// butterfly), and
 // the run-wide max |w f n| / NaN flag from the lanes' accumulators.
 __device__ inline void block_close_tiles(const TileAcc& acc, const double* tpart, uint32_t tb, uint32_t te,
                                          double* out, Scalars* sc) {
@@ -876,10 +880,10 @@ __device__ inline void block_close_tiles(const TileAcc& acc, const double* tpart
     if (wave == 0) {
         DD m{0, 0}, v{0, 0}, q{0, 0};
         for (uint32_t t = tb + (uint32_t)lane; t < te; t += 64) {
-            const double* tp = tpart + 6 * (size_t)t;
-            m = dd_sum(m, DD{tp[0], tp[1]});
-            v = dd_sum(v, DD{tp[2], tp[3]});
-            q = dd_sum(q, DD{tp[4], tp[5]});
+            const double* tp = tpart + 3 * (size_t)t;
+            dd_add(m, tp[0]);
+            dd_add(v, tp[1]);
+            dd_add(q, tp[2]);
         }
         m = wave_sum_dd(m);
         v = wave_sum_dd(v);
@@ -994,7 +998,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
                             __hip_atomic_store(&s_broken, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(2);
+                    __builtin_amdgcn_s_sleep(kPollSleep);
                 }
                 if (__builtin_amdgcn_readfirstlane(
                         __hip_atomic_load(&s_broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
@@ -1018,11 +1022,14 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
         store_offsets(offs, lane, ov);
         tile_body<DIM, PER>(tg, tio, fcn, ax, G, Lsu, Lcu, offs, tp, lane, acc);
         {
-            // the tile's DD sums (in lane 63), summed in tile order at the close
-            const DD m = wave_total_dd(acc.am), v = wave_total_dd(acc.av), q = wave_total_dd(acc.as);
+            // the tile's sums (in lane 63): a fixed FP64 tree over the lanes'
+            // values (each at most kK hcubes' terms); summed in DD in tile
+            // order at the close
+            const double m = wave_total(acc.am.hi + acc.am.lo), v = wave_total(acc.av.hi + acc.av.lo),
+                         q = wave_total(acc.as.hi + acc.as.lo);
             if (lane == 63) {
-                double* o = io.tile_part + 6 * (size_t)t;
-                o[0] = m.hi; o[1] = m.lo; o[2] = v.hi; o[3] = v.lo; o[4] = q.hi; o[5] = q.lo;
+                double* o = io.tile_part + 3 * (size_t)t;
+                o[0] = m; o[1] = v; o[2] = q;
             }
             acc.am = DD{0.0, 0.0};
             acc.av = DD{0.0, 0.0};
@@ -1054,7 +1061,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
                     if (lane == 0) __hip_atomic_store(&s_broken, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(kPollSleep);
             }
         }
     }

@@ -890,7 +890,7 @@ static int prepare(vgx_ctx* ctx, const vgx_config* c, vgx_state* st, Run* R) {
     if ((rc = ctx->blocksum.ensure((size_t)R->nblk_alloc * 3 * 8))) return rc;
     if ((rc = ctx->tile_h.ensure((size_t)R->maxtiles * 4))) return rc;
     if ((rc = ctx->tile_slots.ensure((size_t)R->maxtiles * 2 * sizeof(Slot)))) return rc;
-    if ((rc = ctx->tile_part.ensure((size_t)R->maxtiles * 6 * sizeof(double)))) return rc;
+    if ((rc = ctx->tile_part.ensure((size_t)R->maxtiles * 3 * sizeof(double)))) return rc;
     if ((rc = ctx->blk_part.ensure((size_t)R->fused_blocks * 6 * 8))) return rc;
     if ((rc = ctx->strad_part.ensure((size_t)R->strad_blocks * 6 * 8))) return rc;
     R->errv = nullptr;

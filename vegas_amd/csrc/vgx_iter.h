@@ -89,7 +89,7 @@ struct FusedPtrs {
     const int32_t* tile_h;
     double* sigf;
     Slot* slots;        // [2 * tile]: part of the hcube cut by the tile's start / end
-    double* tile_part;  // [tile][6]: the tile's DD sums (mean, variance, sigf), summed in tile order
+    double* tile_part;  // [tile][3]: the tile's sums (mean, variance, sigf), summed in tile order
     double* blk_part;   // [gridDim.x][6]
     double* sum_f;
     double* n_f;
