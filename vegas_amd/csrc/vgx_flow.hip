@@ -74,10 +74,31 @@ __device__ inline f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// Two k-steps in one gfx950 v_mfma_f32_16x16x32_bf16: lane group g supplies
+// 8 of the 32 products' operands for A and B alike, so any assignment of
+// k to those 8 slots that A and B share gives the same sum.  Here slots
+// 0..3 are k-step kt's k = 4g..4g+3 and slots 4..7 k-step kt+1's: the B
+// operand is two C-layout tiles side by side (no data movement), the A
+// operand two 8-byte pieces of a weight row.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+__device__ inline bf16x8_t cat8(bf16x4_t lo, bf16x4_t hi) {
+    typedef short s16x8_t __attribute__((ext_vector_type(8)));
+    const s16x8_t v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_bit_cast(bf16x8_t, v);
+}
+__device__ inline f32x4_t mfma32(bf16x8_t a, bf16x8_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 // A operand fragment of W (bf16 row-major, row stride `ld`): rows 16 mt ..,
 // columns 16 kt + (l/16)*4 .. +3.
 __device__ inline bf16x4_t frag_a(const uint16_t* W, int ld, int mt, int kt, int lane) {
     return *reinterpret_cast<const bf16x4_t*>(W + (16 * mt + (lane & 15)) * ld + 16 * kt + (lane >> 4) * 4);
+}
+// The A operand of k-steps kt and kt+1 together (mfma32).
+__device__ inline bf16x8_t frag_a2(const uint16_t* W, int ld, int mt, int kt, int lane) {
+    const uint16_t* p = W + (16 * mt + (lane & 15)) * ld + 16 * kt + (lane >> 4) * 4;
+    return cat8(*reinterpret_cast<const bf16x4_t*>(p), *reinterpret_cast<const bf16x4_t*>(p + 16));
 }
 
 __device__ inline float logsigp(float u) {
@@ -166,9 +187,10 @@ __device__ inline void mlp(const LayerLds* s, bf16x4_t a, int lane, f32x4_t* o, 
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[mt][i] = s->b2[16 * mt + (lane >> 4) * 4 + i];
 #pragma unroll
-    for (int kt = 0; kt < 8; ++kt) {
+    for (int kt = 0; kt < 8; kt += 2) {
+        const bf16x8_t b = cat8(h1[kt], h1[kt + 1]);
 #pragma unroll
-        for (int mt = 0; mt < 8; ++mt) acc[mt] = mfma16(frag_a(s->W + kHW2, kS2, mt, kt, lane), h1[kt], acc[mt]);
+        for (int mt = 0; mt < 8; ++mt) acc[mt] = mfma32(frag_a2(s->W + kHW2, kS2, mt, kt, lane), b, acc[mt]);
         __builtin_amdgcn_sched_barrier(0);
     }
     bf16x4_t h2[8];
@@ -188,9 +210,10 @@ __device__ inline void mlp(const LayerLds* s, bf16x4_t a, int lane, f32x4_t* o, 
         o1[i] = s->b3[16 + (lane >> 4) * 4 + i];
     }
 #pragma unroll
-    for (int kt = 0; kt < 8; ++kt) {
-        o0 = mfma16(frag_a(s->W + kHW3, kS3, 0, kt, lane), h2[kt], o0);
-        o1 = mfma16(frag_a(s->W + kHW3, kS3, 1, kt, lane), h2[kt], o1);
+    for (int kt = 0; kt < 8; kt += 2) {
+        const bf16x8_t b = cat8(h2[kt], h2[kt + 1]);
+        o0 = mfma32(frag_a2(s->W + kHW3, kS3, 0, kt, lane), b, o0);
+        o1 = mfma32(frag_a2(s->W + kHW3, kS3, 1, kt, lane), b, o1);
     }
     o[0] = o0;
     o[1] = o1;
@@ -531,6 +554,21 @@ __global__ void flow_lossw_sum_kernel(const float* w, const float* fv, const flo
     }
 }
 
+// The sum of the loss weights' partial sums, in a fixed order (thread t
+// adds parts t, t + 256, ... in order; then the threads' sums in order).
+__global__ __launch_bounds__(256) void flow_csum_kernel(const double* part, int nparts, double* out) {
+    __shared__ double red[256];
+    double t = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) t += part[i];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double u = 0.0;
+        for (int i = 0; i < 256; ++i) u += red[i];
+        *out = u;
+    }
+}
+
 // ------------------------------------------------------------- backward ----
 struct BwdArgs {
     uint64_t n;
@@ -538,17 +576,29 @@ struct BwdArgs {
     const float* saved;       // [L][n][16]
     const float* u0;          // [n][16]
     const float* c;           // [n] unnormalised weights
-    const double* csum_part;  // [ncpart] partial sums of c
-    int ncpart;
+    const double* csum;       // the sum of c (flow_csum_kernel)
     float* G;                 // [n][16] dL/du (in: layer output; out: layer input)
     uint16_t* ops;            // [ntiles][kRows][16] GEMM operands (bf16)
 };
 
-// Writes a C-layout tile (rows rt*16 + (l/16)*4 + i, sample l%16) into the
-// operand block of the sample tile: element (row, sample) at row*16 + sample.
-__device__ inline void put_rows(uint16_t* tileops, int row0, const bf16x4_t v, int lane) {
+// Writes a C-layout tile (rows row0 + (l/16)*4 + i, sample l%16) into the
+// operand block of the sample tile (element (row, sample) at row*16 +
+// sample), transposed through the wave's LDS block so that each lane stores
+// 4 consecutive samples of one row (8 bytes; the 64 lanes one contiguous
+// 512-byte run) instead of 4 scattered 2-byte values.  The block's row
+// stride of kTrStride elements (10 dwords) puts the four lane groups' writes
+// in different banks.
+constexpr int kTrStride = 20;
+__device__ inline void put_rows(uint16_t* tileops, int row0, const bf16x4_t v, int lane, uint16_t* tr) {
+    const int g = lane >> 4, c = lane & 15;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) tileops[(row0 + (lane >> 4) * 4 + i) * 16 + (lane & 15)] = (uint16_t)v[i];
+    for (int i = 0; i < 4; ++i) tr[(4 * g + i) * kTrStride + c] = (uint16_t)v[i];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const bf16x4_t r = *reinterpret_cast<const bf16x4_t*>(tr + c * kTrStride + 4 * g);
+    *reinterpret_cast<bf16x4_t*>(tileops + (row0 + c) * 16 + 4 * g) = r;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
 constexpr int kBwdThreads = 512;
@@ -557,17 +607,13 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
     extern __shared__ __align__(16) unsigned char smem[];
     LayerLds* L = reinterpret_cast<LayerLds*>(smem);
     uint16_t* Wt = reinterpret_cast<uint16_t*>(smem + sizeof(LayerLds));
-    __shared__ double csum_s;
+    uint16_t* tr = Wt + kHLayer + (threadIdx.x >> 6) * 16 * kTrStride;  // the wave's transpose block
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const int k = a.k;
     load_layer(f, k, L, true, Wt);
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int i = 0; i < a.ncpart; ++i) t += a.csum_part[i];
-        csum_s = t;
-    }
     __syncthreads();
-    const float inv_c = csum_s > 0 ? (float)(1.0 / csum_s) : 0.f;
+    const double csum = *a.csum;
+    const float inv_c = csum > 0 ? (float)(1.0 / csum) : 0.f;
     const uint32_t am = f.amask[k];
     const uint64_t ntiles = (a.n + 15) / 16;
     const int nw = blockDim.x >> 6;
@@ -609,11 +655,11 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
         // the forward operands of the weight gradients go out now, so h1 and
         // h2 are not live through the backward GEMMs (only their masks are)
         uint16_t* to = a.ops + (size_t)tile * kRows * 16;
-        put_rows(to, kRa, ain, lane);
+        put_rows(to, kRa, ain, lane, tr);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            put_rows(to, kRh1 + 16 * q, kp.h1[q], lane);
-            put_rows(to, kRh2 + 16 * q, kp.h2[q], lane);
+            put_rows(to, kRh1 + 16 * q, kp.h1[q], lane, tr);
+            put_rows(to, kRh2 + 16 * q, kp.h2[q], lane, tr);
         }
         // coupling backward (inverse direction)
         f32x4_t dout0, dout1, Gin;
@@ -635,15 +681,14 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
             }
         }
         const bf16x4_t do0 = pack_bf(dout0), do1 = pack_bf(dout1);
-        put_rows(to, kRdo, do0, lane);
-        put_rows(to, kRdo + 16, do1, lane);
+        put_rows(to, kRdo, do0, lane, tr);
+        put_rows(to, kRdo + 16, do1, lane, tr);
         // dh2 = W3^T dout (relu mask h2); dh1 = W2^T dh2 (mask h1); da = W1^T dh1
         const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
         bf16x4_t dh2[8];
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
-            f32x4_t cc = mfma16(frag_a(Wt + kTW3, kT3, mt, 0, lane), do0, zero);
-            cc = mfma16(frag_a(Wt + kTW3, kT3, mt, 1, lane), do1, cc);
+            f32x4_t cc = mfma32(frag_a2(Wt + kTW3, kT3, mt, 0, lane), cat8(do0, do1), zero);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 if (!((kp.m2 >> (4 * mt + i)) & 1u)) cc[i] = 0.f;
@@ -653,13 +698,14 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) acc[mt] = zero;
 #pragma unroll
-        for (int kt = 0; kt < 8; ++kt) {
+        for (int kt = 0; kt < 8; kt += 2) {
+            const bf16x8_t b = cat8(dh2[kt], dh2[kt + 1]);
 #pragma unroll
-            for (int mt = 0; mt < 8; ++mt) acc[mt] = mfma16(frag_a(Wt + kTW2, kT2, mt, kt, lane), dh2[kt], acc[mt]);
+            for (int mt = 0; mt < 8; ++mt) acc[mt] = mfma32(frag_a2(Wt + kTW2, kT2, mt, kt, lane), b, acc[mt]);
             __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) put_rows(to, kRdh2 + 16 * q, dh2[q], lane);
+        for (int q = 0; q < 8; ++q) put_rows(to, kRdh2 + 16 * q, dh2[q], lane, tr);
         bf16x4_t dh1[8];
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
@@ -670,7 +716,7 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
         }
         f32x4_t da = zero;
 #pragma unroll
-        for (int kt = 0; kt < 8; ++kt) da = mfma16(frag_a(Wt + kTW1, kT1, 0, kt, lane), dh1[kt], da);
+        for (int kt = 0; kt < 8; kt += 2) da = mfma32(frag_a2(Wt + kTW1, kT1, 0, kt, lane), cat8(dh1[kt], dh1[kt + 1]), da);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int d = 4 * g + i;
@@ -679,7 +725,7 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
         if (live) *reinterpret_cast<f32x4_t*>(a.G + sl * 16 + 4 * g) = Gin;
         // the backward operands of the weight gradients
 #pragma unroll
-        for (int q = 0; q < 8; ++q) put_rows(to, kRdh1 + 16 * q, dh1[q], lane);
+        for (int q = 0; q < 8; ++q) put_rows(to, kRdh1 + 16 * q, dh1[q], lane, tr);
     }
 }
 
@@ -941,6 +987,10 @@ static FlowArgs flow_args(const vgx_flow* fl) {
 }
 
 static size_t lds_layer() { return (sizeof(LayerLds) + 15) & ~(size_t)15; }
+// backward: the layer, the transposed weights, one transpose block per wave
+static size_t lds_bwd() {
+    return lds_layer() + sizeof(uint16_t) * kHLayer + (size_t)(kBwdThreads / 64) * 16 * kTrStride * sizeof(uint16_t);
+}
 
 static int flow_cast(vgx_flow* fl, hipStream_t s) {
     hipLaunchKernelGGL(flow_cast_kernel, dim3(256), dim3(256), 0, s, (const float*)fl->P, fl->H, fl->HT, fl->L);
@@ -969,7 +1019,7 @@ extern "C" int vgx_flow_create(int dim, int nlayers, const uint32_t* amask, vgx_
         return VGX_EHIP;
     }
     if (hipFuncSetAttribute((const void*)flow_backward_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(lds_layer() + sizeof(uint16_t) * kHLayer)) != hipSuccess ||
+                            (int)lds_bwd()) != hipSuccess ||
         hipFuncSetAttribute((const void*)flow_inverse_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(lds_layer() + kStateLds)) != hipSuccess) {
         vgx_flow_destroy(fl);
@@ -1153,10 +1203,12 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
                        (uint64_t)n, pf, pq, mx);
     hipLaunchKernelGGL(flow_lossw_sum_kernel, dim3(nc), dim3(256), 0, s, w, f, (const float*)fl->logq.ptr,
                        (uint64_t)n, pf, pq, (const uint32_t*)mx, (float*)fl->c.ptr, (double*)fl->cpart.ptr);
+    hipLaunchKernelGGL(flow_csum_kernel, dim3(1), dim3(256), 0, s, (const double*)fl->cpart.ptr, nc,
+                       (double*)fl->misc.ptr + 1);
     if ((rc = check_launch())) return rc;
     DeviceInfo info;
     if ((rc = device_info(&info))) return rc;
-    const size_t blds = lds_layer() + sizeof(uint16_t) * kHLayer;
+    const size_t blds = lds_bwd();
     for (int k = 0; k < fl->L; ++k) {
         BwdArgs b{};
         b.n = (uint64_t)n;
@@ -1164,8 +1216,7 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
         b.saved = (const float*)fl->saved.ptr;
         b.u0 = (const float*)fl->u0.ptr;
         b.c = (const float*)fl->c.ptr;
-        b.csum_part = (const double*)fl->cpart.ptr;
-        b.ncpart = nc;
+        b.csum = (const double*)fl->misc.ptr + 1;
         b.G = (float*)fl->G.ptr;
         b.ops = (uint16_t*)fl->ops.ptr;
         // one workgroup per CU: the layer's weights fill most of the LDS, so a
