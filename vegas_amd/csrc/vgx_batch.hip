@@ -84,6 +84,7 @@ struct BArgs {
     int32_t* tile_h;
     Slot* slots;
     double* cont;          // [2 maxtiles][3]
+    double* tile_part;     // [maxtiles][3]: each tile's sums (mean, variance, sigf), summed in tile order
     BPlan* plan;
     BItem* items;
     double* item_part;     // [maxitems][6]
@@ -300,12 +301,12 @@ constexpr size_t kBLdsGrid = (size_t)kBD * kBLd * sizeof(double);
 constexpr size_t kBLdsSums = (size_t)kBHist * sizeof(double);
 constexpr size_t kBLdsCnts = (size_t)(kBHist / 2) * sizeof(uint32_t);
 constexpr size_t kBLdsOffs = (size_t)16 * kNoff * sizeof(uint32_t);
-constexpr size_t kBLds = kBLdsGrid + kBLdsSums + kBLdsCnts + kBLdsOffs + 16 * 6 * sizeof(double);
+constexpr size_t kBLds = kBLdsGrid + kBLdsSums + kBLdsCnts + kBLdsOffs;
 
 template <bool PER>
 __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it, int train) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ uint32_t s_item;
+    __shared__ uint32_t s_item, s_tile;
     if (a.sc->status || a.sc->overflow) return;
     double* G = reinterpret_cast<double*>(smem);
     double* Ls = reinterpret_cast<double*>(smem + kBLdsGrid);
@@ -313,11 +314,13 @@ __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it,
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t* offs = reinterpret_cast<uint32_t*>(smem + kBLdsGrid + kBLdsSums + kBLdsCnts) +
                      wave * kNoff;
-    double* Lred = reinterpret_cast<double*>(smem + kBLdsGrid + kBLdsSums + kBLdsCnts + kBLdsOffs);
     const uint32_t nitems = a.bs->nitems;
     const uint32_t iteration = a.bs->iter0 + (uint32_t)it;
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&a.bs->work, 1u);
+        if (threadIdx.x == 0) {
+            s_item = atomicAdd(&a.bs->work, 1u);
+            s_tile = 0u;
+        }
         __syncthreads();
         const uint32_t item = s_item;
         if (item >= nitems) break;
@@ -355,18 +358,33 @@ __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it,
         const TileIO tio{a.off, a.sigf, a.slots, nullptr, nullptr, nullptr, 0u, 0u};  // LDS histogram
         TileAcc acc{{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, 0.0, 0};
         const uint32_t hend = q.hbase + (uint32_t)q.nhcube, off_end = pl.P + pl.T;
-        const uint32_t nr = (itm.t1 - itm.t0 + 15) / 16;
-        for (uint32_t r = 0; r < nr; ++r) {
-            const uint32_t t = itm.t0 + r * 16 + wave;
-            if (t < itm.t1) {
-                const uint32_t plo = t * kT;
-                const TilePos tp{t, plo, min(plo + (uint32_t)kT, off_end), (uint32_t)a.tile_h[t], hend, off_end,
-                                 q.hbase, pl.P};
-                uint32_t ov[kOffRegs];
-                load_offsets(a.off, tp.hf, hend, off_end, lane, ov);
-                store_offsets(offs, lane, ov);
-                tile_body<-kBD, PER>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
+        // the waves take the item's tiles from an LDS counter (faster waves
+        // take more); each tile's sums go to tile_part and are summed in tile
+        // order at the item's close
+        const uint32_t ntl = itm.t1 - itm.t0;
+#pragma unroll 1
+        for (;;) {
+            uint32_t n = 0u;
+            if (lane == 0) n = atomicAdd(&s_tile, 1u);
+            n = __builtin_amdgcn_readfirstlane(n);
+            if (n >= ntl) break;
+            const uint32_t t = itm.t0 + n;
+            const uint32_t plo = t * kT;
+            const TilePos tp{t, plo, min(plo + (uint32_t)kT, off_end), (uint32_t)a.tile_h[t], hend, off_end,
+                             q.hbase, pl.P};
+            uint32_t ov[kOffRegs];
+            load_offsets(a.off, tp.hf, hend, off_end, lane, ov);
+            store_offsets(offs, lane, ov);
+            tile_body<-kBD, PER>(tg, tio, fcn, ax, G, Ls, Lc, offs, tp, lane, acc);
+            const double m = wave_total(acc.am.hi + acc.am.lo), v = wave_total(acc.av.hi + acc.av.lo),
+                         qs = wave_total(acc.as.hi + acc.as.lo);
+            if (lane == 63) {
+                double* o = a.tile_part + 3 * (size_t)t;
+                o[0] = m; o[1] = v; o[2] = qs;
             }
+            acc.am = DD{0.0, 0.0};
+            acc.av = DD{0.0, 0.0};
+            acc.as = DD{0.0, 0.0};
         }
         __syncthreads();
         if (train) {
@@ -377,7 +395,7 @@ __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it,
                 hc[k] = (Lc[k >> 1] >> (16 * (k & 1))) & 0xffffu;
             }
         }
-        block_close(acc, Lred, a.item_part + 6 * (size_t)item, a.sc);
+        block_close_tiles(acc, a.tile_part, itm.t0, itm.t1, a.item_part + 6 * (size_t)item, a.sc);
         __syncthreads();
     }
 }
@@ -632,6 +650,7 @@ extern "C" int vgx_batch_create(vgx_ctx* ctx, const vgx_batch_config* c, vgx_bat
     BA(tile_h, int32_t, maxtiles);
     BA(slots, Slot, 2 * maxtiles);
     BA(cont, double, 6 * maxtiles);
+    BA(tile_part, double, 3 * maxtiles);
     BA(plan, BPlan, n);
     BA(items, BItem, maxitems);
     BA(item_part, double, 6 * maxitems);
