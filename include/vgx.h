@@ -323,9 +323,14 @@ int vgx_flow_logq(vgx_flow* fl, const float* x, int64_t n, float* logq, float* u
  *   VGX_FLOW_LOSS_VARIANCE  c_i ~ w_i f_i^2 / q(x_i): the gradient of
  *                           sum_i w_i f_i^2 / q(x_i) = Var(f/q) + I^2 (est.),
  *                           up to a positive factor;
- *   VGX_FLOW_LOSS_KL        c_i ~ w_i |f_i|: forward KL from |f|/I to q.
+ *   VGX_FLOW_LOSS_KL        c_i ~ w_i |f_i|: forward KL from |f|/I to q;
+ *   VGX_FLOW_LOSS_SQUARED   c_i ~ w_i f_i^2: the variance loss for samples of
+ *                           the flow itself, given f J (vgx_flow_sample's
+ *                           fout) as f and log w = 0 (then w f^2 / q is
+ *                           (f J)^2 / n: no log q is needed; likewise the KL
+ *                           loss takes f J and log w = 0 for them).
  * Kept on the device for vgx_flow_adam (vgx_flow_get_params reads it).   */
-enum { VGX_FLOW_LOSS_VARIANCE = 0, VGX_FLOW_LOSS_KL = 1 };
+enum { VGX_FLOW_LOSS_VARIANCE = 0, VGX_FLOW_LOSS_KL = 1, VGX_FLOW_LOSS_SQUARED = 2 };
 int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const float* logw, int64_t n, int loss,
                   void* stream);
 /* One Adam step with the last gradient; bf16 copies refreshed.            */

@@ -1176,7 +1176,7 @@ extern "C" int vgx_flow_logq(vgx_flow* fl, const float* x, int64_t n, float* log
 extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const float* logw, int64_t n,
                              int loss, void* stream) {
     if (!fl || !x || !f || !logw || n < 1) return VGX_EINVAL;
-    if (loss != VGX_FLOW_LOSS_VARIANCE && loss != VGX_FLOW_LOSS_KL) return VGX_EINVAL;
+    if (loss != VGX_FLOW_LOSS_VARIANCE && loss != VGX_FLOW_LOSS_KL && loss != VGX_FLOW_LOSS_SQUARED) return VGX_EINVAL;
     const float* w = logw;
     hipStream_t s = (hipStream_t)stream;
     int rc = flow_inverse(fl, x, n, s);
@@ -1198,7 +1198,7 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
     if ((rc = fl->wpart.ensure((size_t)nparts * kPLayer * 4))) return rc;
     uint32_t* mx = (uint32_t*)fl->misc.ptr;
     hipLaunchKernelGGL(flow_zero_u32_kernel, dim3(1), dim3(64), 0, s, mx, 1);
-    const float pf = loss == VGX_FLOW_LOSS_KL ? 1.f : 2.f, pq = loss == VGX_FLOW_LOSS_KL ? 0.f : 1.f;
+    const float pf = loss == VGX_FLOW_LOSS_KL ? 1.f : 2.f, pq = loss == VGX_FLOW_LOSS_VARIANCE ? 1.f : 0.f;
     hipLaunchKernelGGL(flow_lossw_max_kernel, dim3(256), dim3(256), 0, s, w, f, (const float*)fl->logq.ptr,
                        (uint64_t)n, pf, pq, mx);
     hipLaunchKernelGGL(flow_lossw_sum_kernel, dim3(nc), dim3(256), 0, s, w, f, (const float*)fl->logq.ptr,

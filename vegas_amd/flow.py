@@ -173,14 +173,16 @@ class CouplingFlow:
         return lq, u0[:, :self.dim]
 
     def grad(self, x, f, logw, loss='variance'):
-        """Device gradient of the loss ('variance' or 'kl') on points x
-        (float [n][16]), values f and log sampling weights logw."""
+        """Device gradient of the loss ('variance', 'kl', or 'squared': the
+        variance loss for the flow's own samples, given f J as f and log w =
+        0) on points x (float [n][16]), values f and log sampling weights
+        logw."""
         torch = self._torch
         n = x.shape[0]
         f = torch.as_tensor(f, dtype=torch.float32, device='cuda').contiguous()
         w = torch.as_tensor(logw, dtype=torch.float32, device='cuda').contiguous()
         _lib.check(self._L.vgx_flow_grad(self._h, _lib.ptr(x), _lib.ptr(f), _lib.ptr(w), n,
-                                         1 if loss == 'kl' else 0, _lib.stream_ptr()),
+                                         {'variance': 0, 'kl': 1, 'squared': 2}[loss], _lib.stream_ptr()),
                    'vgx_flow_grad')
 
     def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, clip=0.0):
@@ -270,15 +272,17 @@ class FlowMap:
                 self.flow.grad(x[idx].contiguous(), f[idx], logw[idx], loss)
                 self.flow.adam(lr=lr, clip=clip)
         status = torch.zeros(1, dtype=torch.int32, device='cuda')
+        zero_w = torch.zeros(self_batch, dtype=torch.float32, device='cuda') if self_batches else None
         for it in range(self_batches):
-            # the flow's own samples: weights 1/(n q) = J/n, f J / n summed;
-            # no read-back inside the loop (NaN is checked once at the end)
+            # the flow's own samples (weights 1/(n q) = J/n): the loss
+            # weights w f^2 / q and w |f| are (f J)^2 / n and |f J| / n, so
+            # the sampler's f J is passed with log w = 0 and no log q or
+            # second integrand evaluation is needed; no read-back inside the
+            # loop (NaN is checked once at the end)
             self._self_it = getattr(self, '_self_it', 0) + 1
             _, _, xo, fo = self.flow.sample(fcn, self_batch, seed=self.seed + 7919, iteration=1000 + self._self_it,
                                             keep_points=True, status=status)
-            fx = self.flow.eval_fcn(fcn, xo)
-            lq, _ = self.flow.logq(xo)
-            self.flow.grad(xo, fx, -lq - math.log(self_batch), loss)
+            self.flow.grad(xo, fo, zero_w, 'squared' if loss == 'variance' else 'kl')
             self.flow.adam(lr=lr, clip=clip)
         torch.cuda.synchronize()
         if self_batches and int(status.item()) == _lib.VGX_ENAN:
