@@ -18,6 +18,21 @@ def va():
     return vegas_amd
 
 
+def test_batch_first_iteration_reproducible(va):
+    """Same problem, same seeds, two fresh integrators: the first
+    iteration's estimates and errors are bit for bit the same (the waves
+    take tiles in timing order, but each tile's sums are summed in tile
+    order); later iterations agree to rounding (the item histograms are
+    added by LDS atomics in timing order)."""
+    from vegas_amd.batch import BatchIntegrator, gaussian_params
+    dims, cs, sig, nev = _problem(48, 11)
+    A, B = gaussian_params(dims, sig)
+    r = [BatchIntegrator(dims, nev, tiles_per_item=16)(A, B, cs, nitn=3) for _ in range(2)]
+    np.testing.assert_array_equal(r[0].itn_mean[0], r[1].itn_mean[0])
+    np.testing.assert_array_equal(r[0].itn_sdev[0], r[1].itn_sdev[0])
+    np.testing.assert_allclose(r[0].itn_mean[2], r[1].itn_mean[2], rtol=1e-6)
+
+
 def _exact(c, s):
     out = 1.0
     for cd in c:

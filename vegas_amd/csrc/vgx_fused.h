@@ -455,13 +455,23 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
                     const uint64_t zm = (PER && zmin == 0) ? per_zero_axes<DIM>(g, pc, pj - tp.pshift) : 0ull;
                     store_record<DIM>(io, dim, pj - io.rec_p0, kb, zm, c2);
                     if (Lc) {
+                        if (PER && zm == 0) {
+                            // every axis inside (0,1): no per-axis test
 #pragma unroll
-                        for (int d = 0; d < ND; ++d) {
-                            if (DIM <= 0 && d >= dim) break;
-                            const int k = kb[d];
-                            if (k < 0 || ((zm >> d) & 1)) continue;
-                            const int idx = d * g.nb + k;
-                            atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
+                            for (int d = 0; d < ND; ++d) {
+                                if (DIM <= 0 && d >= dim) break;
+                                const int idx = d * g.nb + kb[d];
+                                atomicAdd(&Lc[idx >> 1], 1u + (uint32_t)(idx & 1) * 65535u);
+                            }
+                        } else {
+#pragma unroll
+                            for (int d = 0; d < ND; ++d) {
+                                if (DIM <= 0 && d >= dim) break;
+                                const int k = kb[d];
+                                if (k < 0 || ((zm >> d) & 1)) continue;
+                                const int idx = d * g.nb + k;
+                                atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
+                            }
                         }
                     }
                 } else if (PER && zmin != 0) {
@@ -678,7 +688,8 @@ __device__ inline void counts_to_row(uint32_t* Lc, int tot, uint32_t* row, bool 
 // waits end once the tiles in flight are stored; they are bounded by
 // wall-clock time (kDrainTicks of the 100 MHz clock), past which the
 // workgroup is marked broken and the launch reports VGX_ETIMEOUT.
-constexpr int kPairs = (VGX_MAXDIM + 1) / 2;
+constexpr int kGroupAxes = 2;  // axes per adder lock (the two halves of a record word; one axis per lock was slower)
+constexpr int kPairs = (VGX_MAXDIM + kGroupAxes - 1) / kGroupAxes;
 constexpr uint32_t kRing = 128;
 struct HistSeq {
     uint32_t done[kRing];  // tile n complete (records stored): done[n % kRing] == n + 1
@@ -704,7 +715,7 @@ struct PairRec {
 // them).
 __device__ __forceinline__ void pair_load(const TileIO& io, int k, uint32_t ri0, int lane, PairRec& rc) {
     const uint32_t r0 = ri0 + (uint32_t)lane * kK;
-    rc.wd = *reinterpret_cast<const uint4*>(io.rec_bins + (size_t)k * io.rec_stride + r0);
+    rc.wd = *reinterpret_cast<const uint4*>(io.rec_bins + (size_t)((k * kGroupAxes) >> 1) * io.rec_stride + r0);
     rc.c01 = *reinterpret_cast<const double2*>(io.rec_c2 + r0);
     rc.c23 = *reinterpret_cast<const double2*>(io.rec_c2 + r0 + 2);
 }
@@ -717,12 +728,12 @@ __device__ __forceinline__ void pair_add(const TileGeom& g, int dim, int k, cons
     const double cv[kK] = {rc.c01.x, rc.c01.y, rc.c23.x, rc.c23.y};
     const uint32_t ws[kK] = {rc.wd.x, rc.wd.y, rc.wd.z, rc.wd.w};
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int d = 2 * k + h;
+    for (int h = 0; h < kGroupAxes; ++h) {
+        const int d = kGroupAxes * k + h;
         if (d >= dim) break;
 #pragma unroll
         for (int j = 0; j < kK; ++j) {
-            const uint32_t kb = (ws[j] >> (16 * h)) & 0xFFFFu;
+            const uint32_t kb = (ws[j] >> (16 * (d & 1))) & 0xFFFFu;
             if ((uint32_t)j < nmine && kb != kNoBin) unsafeAtomicAdd(&Ls[d * g.nb + (int)kb], cv[j]);
         }
     }
@@ -799,7 +810,7 @@ __device__ __forceinline__ void pair_try_add(const TileGeom& g, const TileIO& io
 __device__ __forceinline__ bool hist_try_add(const TileGeom& g, const TileIO& io, int dim, HistSeq& hs, uint32_t tb,
                                              uint32_t ntile, uint32_t npts, double* Ls, int lane, int first,
                                              uint32_t cap, uint32_t min_run) {
-    const int np = (dim + 1) / 2;
+    const int np = (dim + kGroupAxes - 1) / kGroupAxes;
     // one look at every pair (lane k): finished, or free with its next tile
     // complete (and tile min_run - 1 after it, or the last tile); only the
     // latter are tried
@@ -827,7 +838,7 @@ __device__ __forceinline__ bool hist_try_add(const TileGeom& g, const TileIO& io
 // Whether every pair has added every tile before tile `upto`.
 __device__ __forceinline__ bool added_before(HistSeq& hs, int dim, uint32_t upto, int lane) {
     bool ok = true;
-    if (lane < (dim + 1) / 2)
+    if (lane < (dim + kGroupAxes - 1) / kGroupAxes)
         ok = (__hip_atomic_load(&hs.word[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> 1) >= upto;
     return __ballot(!ok) == 0;
 }
@@ -962,6 +973,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
     // ordered: the tiles enter the histogram in tile order through the pair
     // locks (hist_try_add), from the records the tile body stores
     const bool ordered = lds_hist && !unordered;
+    const int ngroups = (dim + kGroupAxes - 1) / kGroupAxes;  // adder locks (axis groups)
     uint32_t* const crow = lds_hist ? io.hpart_c + (size_t)blockIdx.x * tot : nullptr;
     // u16 LDS counts: a workgroup with more than kCntFlushTiles tiles moves
     // them into its u32 row (zeroed here) every kCntFlushTiles tiles taken,
@@ -991,7 +1003,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
                     if (__builtin_amdgcn_readfirstlane(
                             __hip_atomic_load(&s_broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
                         break;
-                    (void)hist_try_add(tg, tio, dim, hs, tb, ntile_wg, p1 - p0, Ls, lane, wave % ((dim + 1) / 2),
+                    (void)hist_try_add(tg, tio, dim, hs, tb, ntile_wg, p1 - p0, Ls, lane, wave % ngroups,
                                        64u, 1u);
                     if (wall_clock64() - t_wait0 > kDrainTicks) {
                         if (lane == 0)
@@ -1041,7 +1053,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                 if (lane == 0)
                     __hip_atomic_store(&hs.done[n % kRing], n + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                (void)hist_try_add(tg, tio, dim, hs, tb, ntile_wg, p1 - p0, Ls, lane, wave % ((dim + 1) / 2),
+                (void)hist_try_add(tg, tio, dim, hs, tb, ntile_wg, p1 - p0, Ls, lane, wave % ngroups,
                                    kHoldTiles, kMinRun);
             }
         }
@@ -1055,7 +1067,7 @@ __global__ __launch_bounds__(kFusedThreads) void fused_kernel(IterGeom g, FusedP
                 if (__builtin_amdgcn_readfirstlane(
                         __hip_atomic_load(&s_broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)))
                     break;
-                if (hist_try_add(tg, tio, dim, hs, tb, ntile_wg, p1 - p0, Ls, lane, wave % ((dim + 1) / 2), 64u, 1u))
+                if (hist_try_add(tg, tio, dim, hs, tb, ntile_wg, p1 - p0, Ls, lane, wave % ngroups, 64u, 1u))
                     break;
                 if (wall_clock64() - t_wait0 > kDrainTicks) {
                     if (lane == 0) __hip_atomic_store(&s_broken, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
