@@ -265,8 +265,12 @@ class FlowMap:
         torch = _lib.torch_cuda()
         n = x.shape[0] if x is not None else 0
         t0 = time.perf_counter()
+        # batch order from a generator seeded by the flow's seed: the
+        # training, and so the learned map, repeat for a given seed
+        gen = torch.Generator(device='cuda')
+        gen.manual_seed(int(self.seed) * 1000003 + 17)
         for ep in range(epochs if n else 0):
-            perm = torch.randperm(n, device='cuda')
+            perm = torch.randperm(n, device='cuda', generator=gen)
             for b0 in range(0, n, batch):
                 idx = perm[b0:b0 + batch]
                 self.flow.grad(x[idx].contiguous(), f[idx], logw[idx], loss)
@@ -291,7 +295,7 @@ class FlowMap:
         return self
 
     def fit(self, fcn, integ, ntrain=20000000, batch=250000, kl_epochs=3, self_kl=200, self_var=200,
-            lr=1e-3, clip=1.0):
+            lr=1e-3, clip=1.0, rtol=None, chunk=100, max_var=3000, probe=4000000):
         """The training recipe: ntrain samples from the adapted vegas map of
         ``integ``; Adam on the forward KL loss over them (stable when the
         samples see only part of the peak), then on the flow's own samples
@@ -300,7 +304,16 @@ class FlowMap:
         unit of training time than batches of 1e6.  The step counts are the
         ones that minimised training plus sampling time to 1e-4 there (3
         epochs, 200 + 200 self-training steps: 1.7 s, against 2.2 s for 6
-        epochs and 300 + 150)."""
+        epochs and 300 + 150).
+
+        With ``rtol`` the variance-loss phase continues past self_var steps,
+        ``chunk`` steps at a time (at most max_var in all): after each, the
+        per-sample variance on ``probe`` samples gives the sampling time
+        still needed for rtol, and training stops after two chunks in a row
+        that did not lower the projected time to the target (training so far
+        plus that sampling time).  On the narrower 8-D ridge the fixed 200
+        steps leave 2-3 times the variance that 1000 more remove.  The
+        flow's weights then depend on these timings (not on the seed alone).""" 
         x, fx, lw = vegas_training_data(integ, fcn, self.flow, int(ntrain))
         self.train(fcn, x, fx, lw, epochs=kl_epochs, batch=batch, lr=lr, clip=clip, loss='kl')
         del x, fx, lw
@@ -310,6 +323,31 @@ class FlowMap:
         if self_var:
             self.train(fcn, None, None, None, epochs=0, self_batches=self_var, self_batch=batch, lr=lr * 0.3,
                        clip=clip, loss='variance')
+        if rtol is None:
+            return self
+
+        def remaining():
+            t0 = time.perf_counter()
+            mean, var = self.per_sample_variance(fcn, n=probe, iteration=9000 + len(self.history))
+            per_sample = (time.perf_counter() - t0) / probe
+            need = var / max((rtol * mean) ** 2, 1e-300)
+            return need * per_sample
+        torch = _lib.torch_cuda()
+        t_start = time.perf_counter()
+        best = remaining()  # projected time from here: further training plus sampling
+        done, stale = self_var, 0
+        while done < max_var and stale < 2:
+            self.train(fcn, None, None, None, epochs=0, self_batches=chunk, self_batch=batch, lr=lr * 0.3,
+                       clip=clip, loss='variance')
+            now = remaining()
+            torch.cuda.synchronize()
+            total = (time.perf_counter() - t_start) + now
+            done += chunk
+            self.history.append((done, now))
+            if total < best:
+                best, stale = total, 0
+            else:
+                stale += 1
         return self
 
     def per_sample_variance(self, fcn, n=1000000, iteration=0):
