@@ -134,6 +134,13 @@ typedef struct {
     int32_t flags;                   /* VGX_FLAG_* */
     int32_t weighted;                /* RAvg weighted average (adapt) */
     int32_t reserved;
+    /* Caller-supplied uniforms (Integrator.ran_array_generator, pyx:1676-1680,
+     * 1732): device double[total][dim], row p holding the dim numbers in [0,1)
+     * of point p of the iteration (p as vgx_iter_begin orders the points), so
+     * y = (stratum + u)/nstrat (pyx:1741) in FP64.  Staged iterations only
+     * (vgx_iter_sample/accumulate/end; precision VGX_PREC_F64, nitn = 1);
+     * vgx_integrate rejects it.  NULL: the library's Philox4x32-10 stream.  */
+    const double* uniforms;
 } vgx_config;
 
 /* Device integrands.  GAUSSMIX: f(x) = sum_k A[k] exp(-B[k] |x - c[k]|^2)

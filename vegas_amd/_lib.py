@@ -28,7 +28,7 @@ class VgxConfig(C.Structure):
                 ("min_neval_hcube", C.c_int64), ("max_neval_hcube", C.c_int64),
                 ("seed", C.c_uint64), ("iteration", C.c_uint32), ("nitn", C.c_int32),
                 ("precision", C.c_int32), ("flags", C.c_int32), ("weighted", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("uniforms", C.c_void_p)]
 
 
 class VgxIntegrand(C.Structure):

@@ -348,6 +348,7 @@ __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it,
         double jn = 1.0;
         for (int d = 0; d < dim; ++d) jn *= (double)ldc(ax, d).ninc;
         tg.jn = jn;
+        tg.uext = nullptr;
         const double* grow = a.grid + (size_t)i * kBD * kBLd;
         for (int k = threadIdx.x; k < dim * kBLd; k += blockDim.x) G[k] = grow[k];
         const int tot = dim * kBNb;

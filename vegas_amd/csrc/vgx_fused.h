@@ -24,6 +24,7 @@ struct TileGeom {
     uint32_t iteration, seed_lo, seed_hi;
     double beta_half, dv_y;
     double jn;  // product of ninc over the axes (the integer stratum path's Jacobian factor)
+    const double* uext;  // caller's uniforms [total][dim] for the 53-bit path, or null
 };
 
 __host__ __device__ inline TileGeom tile_geom(const IterGeom& g) {
@@ -43,6 +44,7 @@ __host__ __device__ inline TileGeom tile_geom(const IterGeom& g) {
     t.beta_half = g.beta_half;
     t.dv_y = g.dv_y;
     t.jn = g.jn;
+    t.uext = g.uext;
     return t;
 }
 
@@ -131,7 +133,9 @@ struct CoordWalk {
 //       (the map of the exact y = (c+u)/nstrat); k < ninc always;
 //   otherwise (53-bit uniforms, or unaligned axes, or exact_div) y is formed
 //       in FP64 as the reference does, y = (c+u)/nstrat (pyx:1741), and the
-//       map computes t = y*ninc (pyx:351).
+//       map computes t = y*ninc (pyx:351); the 53-bit uniforms are the
+//       caller's row p of g.uext when it is set (ran_array_generator,
+//       pyx:1732), else Philox4x32-10 words;
 //   k = ninc (y = 1) takes x = grid[ninc] and the last increment
 //   (pyx:355-359).
 //   kbin[d]: the training bin of axis d, -1 where y is not inside (0,1)
@@ -178,8 +182,14 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
                 kv = (yv > 0 && yv < 1) ? (k < ni ? k : ni - 1) : -1;
             }
         } else {
-            if ((d & 1) == 0) w = philox4x32_10_dev(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
-            const double u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);
+            double u;
+            if (g.uext) {
+                u = g.uext[(size_t)p * dim + d];
+            } else {
+                if ((d & 1) == 0)
+                    w = philox4x32_10_dev(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
+                u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);
+            }
             const double yv = g.exact_div ? ((double)c + u) / (double)a.nstrat : ((double)c + u) * a.inv_s;
             const double tt = fmin(yv * (double)ni, (double)ni);
             k = (int)tt;
@@ -243,6 +253,8 @@ __device__ inline void point_y(const TileGeom& g, const AxisC* __restrict__ ax, 
             if ((d & 3) == 0) w = philox4x32_10_dev(U4{p, 0u, g.iteration, (uint32_t)(d >> 2)}, g.seed_lo, g.seed_hi);
             const int e = d & 3;
             u = u24(e == 0 ? w.x : e == 1 ? w.y : e == 2 ? w.z : w.w);
+        } else if (g.uext) {
+            u = g.uext[(size_t)p * g.dim + d];
         } else {
             if ((d & 1) == 0) w = philox4x32_10_dev(U4{p, 0u, g.iteration, (uint32_t)(d >> 1)}, g.seed_lo, g.seed_hi);
             u = (d & 1) ? u53(w.z, w.w) : u53(w.x, w.y);

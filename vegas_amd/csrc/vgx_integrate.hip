@@ -814,6 +814,10 @@ static int build_geom(const vgx_config* c, const DeviceInfo& info, IterGeom* g) 
     g->seed_lo = (uint32_t)(c->seed & 0xffffffffu);
     g->seed_hi = (uint32_t)(c->seed >> 32);
     g->nitn = (uint32_t)c->nitn;
+    // the caller's uniforms take the place of the 53-bit Philox stream for one
+    // staged iteration
+    if (c->uniforms && (c->precision != VGX_PREC_F64 || c->nitn != 1)) return VGX_EINVAL;
+    g->uext = c->uniforms;
     // LDS plan of the fused kernel: the FP64 grid, offsets slices, and in
     // fast mode the training histogram (FP64 sums, u16 pair counts) when it
     // fits; otherwise (and in exact mode) the histogram is built from the
@@ -1146,7 +1150,7 @@ using namespace vgx;
 extern "C" int vgx_integrate(vgx_ctx* ctx, const vgx_config* c, const vgx_integrand* f, vgx_state* st,
                              void* stream) {
     if (!ctx || !c || !f || !st) return VGX_EINVAL;
-    if (f->kind == VGX_FCN_BUFFER) return VGX_EINVAL;
+    if (f->kind == VGX_FCN_BUFFER || c->uniforms) return VGX_EINVAL;
     hipStream_t s = (hipStream_t)stream;
     Run R;
     int rc = prepare(ctx, c, st, &R);

@@ -54,6 +54,7 @@ struct IterGeom {
     uint32_t itn;        // index within the run
     uint32_t nitn;
     uint32_t pad3;
+    const double* uext;  // caller's uniforms [total][dim] (53-bit path), or null: Philox
 };
 
 // Device-resident scalars of one run.

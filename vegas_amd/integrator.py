@@ -238,8 +238,6 @@ class Integrator:
                 raise AttributeError('no parameter named "%s"' % str(k))
         if self.nproc not in (1, None) or self.mpi:
             warnings.warn('nproc/mpi are ignored: the iteration runs on one GPU')
-        if self.ran_array_generator is not None:
-            raise NotImplementedError('ran_array_generator: samples come from the device Philox stream')
         # 2) sanity checks
         if nstrat is not None:
             if len(nstrat) != self.map.dim:
@@ -368,7 +366,32 @@ class Integrator:
             p = 'f32' if device_fcn else 'f64'
         if p not in ('f32', 'f64'):
             raise ValueError("precision must be 'f32' or 'f64'")
+        if p == 'f32' and self.ran_array_generator is not None:
+            raise ValueError("ran_array_generator gives float64 uniforms: use precision='f64'")
         return _lib.PREC_F32 if p == 'f32' else _lib.PREC_F64
+
+    def _uniforms(self, torch, cfg, total):
+        """Device buffer [total, dim] for the caller's uniforms of one
+        iteration (ran_array_generator, pyx:1676-1680), attached to cfg; None
+        when the samples come from the device Philox stream."""
+        if self.ran_array_generator is None:
+            return None
+        u = torch.empty((max(int(total), 1), self.dim), dtype=torch.float64, device='cuda')
+        cfg.uniforms = u.data_ptr()
+        return u
+
+    def _draw(self, torch, u, p0, p1):
+        """ran_array_generator((n, dim)) for the batch of points [p0, p1)
+        (pyx:1732), into rows p0..p1 of the iteration's uniform buffer."""
+        if u is None:
+            return
+        n = p1 - p0
+        r = self.ran_array_generator((n, self.dim))
+        r = torch.as_tensor(np.asarray(r, dtype=np.float64) if not torch.is_tensor(r) else r,
+                            dtype=torch.float64)
+        if tuple(r.shape) != (n, self.dim):
+            raise ValueError('ran_array_generator((%d, %d)) returned shape %s' % (n, self.dim, tuple(r.shape)))
+        u[p0:p1].copy_(r)
 
     def _config(self, nitn, iteration, prec):
         c = _lib.VgxConfig()
@@ -453,6 +476,10 @@ class Integrator:
         device_fcn = isinstance(fcn, DeviceIntegrand)
         if device_fcn and fcn.dim != self.dim:
             raise ValueError('integrand dimension %d != integrator dimension %d' % (fcn.dim, self.dim))
+        if self.ran_array_generator is not None:
+            # the caller's uniforms feed the staged iteration; a device
+            # integrand is then called through its batch form
+            device_fcn = False
         prec = self._precision(device_fcn)
         std = None if device_fcn else self._std_integrand(fcn)
         layout = None if device_fcn else self._probe_layout(torch, std)
@@ -613,8 +640,10 @@ class Integrator:
             cstat = torch.zeros(1, dtype=torch.int32, device='cuda')
         _lib.check(L.vgx_iter_begin(cx, C.byref(cfg), C.byref(st), 0, _lib.ptr(nh), C.byref(total),
                                     sp), 'iter_begin')
+        ubuf = self._uniforms(torch, cfg, total.value)
         for h0, h1, p0, p1 in _geometry.batches(nh.cpu().numpy(), self.min_neval_batch):
             n = p1 - p0
+            self._draw(torch, ubuf, p0, p1)
             x = torch.empty((n, self.dim), dtype=torch.float64, device='cuda')
             y = torch.empty((n, self.dim), dtype=torch.float64, device='cuda') if std.uses_jac else None
             w = torch.empty(n, dtype=torch.float64, device='cuda') if multi else None
@@ -674,8 +703,10 @@ class Integrator:
         self._rng_iter += 1
         nhh = nh.cpu().numpy()
         self.last_neval = int(total.value)
+        ubuf = self._uniforms(torch, cfg, total.value)
         for h0, h1, p0, p1 in _geometry.batches(nhh, self.min_neval_batch):
             n = p1 - p0
+            self._draw(torch, ubuf, p0, p1)
             x = torch.empty((n, self.dim), dtype=torch.float64, device='cuda')
             y = torch.empty_like(x) if yield_y else None
             w = torch.empty(n, dtype=torch.float64, device='cuda')
