@@ -419,8 +419,8 @@ def flow_ridge(a):
     for k, ms in tl:
         kt.setdefault(k, []).append(ms)
     lf = flow_layer_flops(D)
-    work = {"flow_backward_kernel": nb * 2 * lf,   # recomputed MLP + backpropagation per layer
-            "flow_wgrad_kernel": nb * lf,          # weight gradients per layer
+    # per layer: recomputed MLP, backpropagation and the weight gradients
+    work = {"flow_backward_kernel": nb * 3 * lf,
             "flow_sample_kernel": nb * 6 * lf}     # 6 layers forward (the integrand's flops aside)
     kern = {k: {"ms_avg": sum(v) / len(v), "launches": len(v),
                 "TFLOPs": work[k] / (sum(v) / len(v) * 1e-3) / 1e12} for k, v in kt.items() if k in work}
@@ -449,8 +449,8 @@ def flow_ridge(a):
                      "kernels": kern,
                      "note": "BF16 MFMA kernels of a training step on 2.5e5 samples per layer; algorithmic "
                              "flops 2(D*128 + 128*128 + 128*2D) per sample and layer for the forward MLP "
-                             "(backward kernel: twice that, weight gradients: once); peak is the BF16 dense "
-                             "MFMA peak"},
+                             "(backward kernel, which also forms the weight gradients: three times that); peak is the "
+                             "BF16 dense MFMA peak"},
         "cpu_baseline": None if a.no_cpu else cpu_oracle_rate(
             (workloads.ridge_8d if D == 8 else workloads.ridge_9d)(diagonal=False)["f"], D, 1e5, 1,
             "the %s (per-evaluation rate: plain vegas+ does not reach 1e-4 on this integrand in the "

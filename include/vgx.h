@@ -224,7 +224,7 @@ int vgx_iter_accumulate_cov(vgx_ctx* ctx, const vgx_config* cfg, int64_t h0, int
  * launches (up to 256 in all) of the timed kernels with HIP events on their
  * stream: the fused sampler (tag 1), the batched sampler (2; while timing
  * is on, vgx_batch_integrate launches its kernels directly instead of
- * through its graph), the flow's backward (3), weight-gradient (4) and
+ * through its graph), the flow's backward with its weight gradients (3) and
  * sampling (6) kernels, and the training-histogram rows (5).
  * vgx_timed_launches writes the durations in ms and the tags in launch
  * order and returns how many were recorded; vgx_fused_times writes those

@@ -22,8 +22,9 @@
 //                 estimator; layers streamed through LDS, samples in registers
 //   flow_inverse  x -> z, log q(x), each layer's input saved (training)
 //   flow_lossw    variance-loss weights c_i ~ w_i f_i^2 / q(x_i), normalised
-//   flow_backward one layer: recompute, backprop, write the GEMM operands
-//   flow_wgrad    weight gradients of one layer: MFMA over the sample axis
+//   flow_backward one layer: recompute, backprop, and the weight gradients
+//                 (MFMA over the sample axis, operands staged in LDS)
+//   flow_wgrad_reduce  the workgroups' weight-gradient sums, in a fixed order
 //   flow_adam     Adam on the FP32 master weights, bf16 copies refreshed
 #include <math.h>
 #include <string.h>
@@ -54,9 +55,6 @@ constexpr int kT1 = 132, kT2 = 132, kT3 = 36;          // W1t [16][128], W2t [12
 constexpr int kHW1 = 0, kHW2 = kHW1 + kFH * kS1, kHW3 = kHW2 + kFH * kS2, kHLayer = kHW3 + kFO * kS3;
 constexpr int kTW1 = 0, kTW2 = kTW1 + kFI * kT1, kTW3 = kTW2 + kFH * kT2;
 static_assert(kTW3 + kFH * kT3 <= kHLayer, "transposed layout fits the layer block");
-// backward GEMM operands of one 16-sample tile: rows of 16 bf16
-constexpr int kRa = 0, kRh1 = 16, kRh2 = kRh1 + kFH, kRdo = kRh2 + kFH, kRdh2 = kRdo + kFO,
-              kRdh1 = kRdh2 + kFH, kRows = kRdh1 + kFH;  // 560 rows
 
 // FP32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
 __device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
@@ -157,18 +155,12 @@ __device__ inline bf16x4_t cond_input(const f32x4_t u, uint32_t amask, int D, in
     return pack_bf(a);
 }
 
-// The conditioner MLP of one tile: o[0] = raw s, o[1] = t rows (l/16)*4+i.
-// Optionally keeps h1, h2 (bf16 fragments) and their ReLU masks.
-struct MlpKeep {
-    bf16x4_t h1[8], h2[8];
-    uint32_t m1, m2;   // bit 4 mt + i: pre-activation > 0
-};
-
-template <bool KEEP>
-__device__ inline void mlp(const LayerLds* s, bf16x4_t a, int lane, f32x4_t* o, MlpKeep* keep) {
+// Layer 1 of the conditioner MLP: h1 = relu(W1 a) (bf16 fragments, output
+// tile mt = k-step mt of layer 2) and its ReLU mask (bit 4 mt + i:
+// pre-activation > 0).
+__device__ inline uint32_t layer1(const LayerLds* s, bf16x4_t a, int lane, bf16x4_t* h1) {
     const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
-    bf16x4_t h1[8];
-    uint32_t m1 = 0, m2 = 0;
+    uint32_t m1 = 0;
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
         f32x4_t c = mfma16(frag_a(s->W + kHW1, kS1, mt, 0, lane), a, zero);
@@ -179,6 +171,21 @@ __device__ inline void mlp(const LayerLds* s, bf16x4_t a, int lane, f32x4_t* o, 
         }
         h1[mt] = pack_bf(c);
     }
+    return m1;
+}
+
+// The conditioner MLP of one tile: o[0] = raw s, o[1] = t rows (l/16)*4+i.
+// Optionally keeps h2 (bf16 fragments) and the ReLU masks of h1 and h2.
+struct MlpKeep {
+    bf16x4_t h2[8];
+    uint32_t m1, m2;   // bit 4 mt + i: pre-activation > 0
+};
+
+template <bool KEEP>
+__device__ inline void mlp(const LayerLds* s, bf16x4_t a, int lane, f32x4_t* o, MlpKeep* keep) {
+    bf16x4_t h1[8];
+    uint32_t m2 = 0;
+    const uint32_t m1 = layer1(s, a, lane, h1);
     // h2: the eight output tiles accumulate side by side (independent MFMA
     // chains), k-step by k-step
     f32x4_t acc[8];
@@ -219,10 +226,7 @@ __device__ inline void mlp(const LayerLds* s, bf16x4_t a, int lane, f32x4_t* o, 
     o[1] = o1;
     if (KEEP) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            keep->h1[q] = h1[q];
-            keep->h2[q] = h2[q];
-        }
+        for (int q = 0; q < 8; ++q) keep->h2[q] = h2[q];
         keep->m1 = m1;
         keep->m2 = m2;
     }
@@ -578,37 +582,63 @@ struct BwdArgs {
     const float* c;           // [n] unnormalised weights
     const double* csum;       // the sum of c (flow_csum_kernel)
     float* G;                 // [n][16] dL/du (in: layer output; out: layer input)
-    uint16_t* ops;            // [ntiles][kRows][16] GEMM operands (bf16)
+    float* part;              // [gridDim.x][kPLayer] each workgroup's weight-gradient sums
 };
 
-// Writes a C-layout tile (rows row0 + (l/16)*4 + i, sample l%16) into the
-// operand block of the sample tile (element (row, sample) at row*16 +
-// sample), transposed through the wave's LDS block so that each lane stores
-// 4 consecutive samples of one row (8 bytes; the 64 lanes one contiguous
-// 512-byte run) instead of 4 scattered 2-byte values.  The block's row
-// stride of kTrStride elements (10 dwords) puts the four lane groups' writes
-// in different banks.
-constexpr int kTrStride = 20;
-__device__ inline void put_rows(uint16_t* tileops, int row0, const bf16x4_t v, int lane, uint16_t* tr) {
+// Backward of one layer together with its weight gradients.  Per round each
+// wave takes one 16-sample tile (tile (round * gridDim.x + block) * 8 +
+// wave), recomputes the layer's MLP, backpropagates through the coupling and
+// the MLP, and the eight waves stage their GEMM operands in LDS (element
+// (row, sample) of a wave's slot at row * 16 + sample) in three phases:
+//   1: dout (32 rows) | h2 (128)   -> dW3, b3
+//   2: dh2 (128) | h1 (128)        -> dW2, b2 (h1 recomputed from a: 8 small MFMAs
+//                                     instead of 16 registers held through the
+//                                     backward GEMMs)
+//   3: dh1 (128) | a (16)          -> dW1
+// After each staging every wave adds its share of dW over the eight staged
+// tiles, two tiles per v_mfma_f32_16x16x32_bf16 (k = the samples of both);
+// an all-ones operand gives the bias sums.  The sums stay in registers over
+// the rounds and are written once per workgroup: the operands never leave
+// the CU.  The tile -> workgroup assignment is static, so the sums are
+// reproducible.  Tiles past the end run on zero inputs: their dout is 0, so
+// they add nothing.
+constexpr int kBwdThreads = 512;
+constexpr int kBwdWaves = kBwdThreads / 64;
+constexpr int kSlotRows = 2 * kFH;         // the largest phase: dh2 | h1
+constexpr int kSlot = kSlotRows * 16;      // bf16 elements per wave slot
+constexpr int kNo3 = 2 * 9;                // dW3 output tiles: 2 row tiles x (8 column tiles + bias)
+
+// The C-layout tile v (rows row0 + (l/16)*4 + i, sample l%16) into a slot.
+__device__ inline void stage(uint16_t* slot, int row0, const bf16x4_t v, int lane) {
     const int g = lane >> 4, c = lane & 15;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) tr[(4 * g + i) * kTrStride + c] = (uint16_t)v[i];
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    const bf16x4_t r = *reinterpret_cast<const bf16x4_t*>(tr + c * kTrStride + 4 * g);
-    *reinterpret_cast<bf16x4_t*>(tileops + (row0 + c) * 16 + 4 * g) = r;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int i = 0; i < 4; ++i) slot[(row0 + 4 * g + i) * 16 + c] = (uint16_t)v[i];
+}
+// Operand fragment of slot rows row0.. (A and B alike: row row0 + l%16,
+// samples (l/16)*4 .. +3).
+__device__ inline bf16x4_t frag_op(const uint16_t* slot, int row0, int lane) {
+    return *reinterpret_cast<const bf16x4_t*>(slot + (row0 + (lane & 15)) * 16 + (lane >> 4) * 4);
+}
+// Rows row0.. of staged tiles j and j+1 as one k = 32 operand (A and B
+// alike: slots 0..3 tile j's samples (l/16)*4.., slots 4..7 tile j+1's).
+__device__ inline bf16x8_t frag_pair(const uint16_t* S, int j, int row0, int lane) {
+    const uint16_t* s0 = S + j * kSlot;
+    return cat8(frag_op(s0, row0, lane), frag_op(s0 + kSlot, row0, lane));
 }
 
-constexpr int kBwdThreads = 512;
+constexpr size_t kLdsLayer = (sizeof(LayerLds) + 15) & ~(size_t)15;
+constexpr size_t kLdsBwd = kLdsLayer + sizeof(uint16_t) * kHLayer + sizeof(uint16_t) * (size_t)kBwdWaves * kSlot;
+static_assert(kLdsBwd <= 160 * 1024, "backward LDS plan fits a CU");
 
 __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, BwdArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     LayerLds* L = reinterpret_cast<LayerLds*>(smem);
-    uint16_t* Wt = reinterpret_cast<uint16_t*>(smem + sizeof(LayerLds));
-    uint16_t* tr = Wt + kHLayer + (threadIdx.x >> 6) * 16 * kTrStride;  // the wave's transpose block
-    const int lane = threadIdx.x & 63, g = lane >> 4;
+    uint16_t* Wt = reinterpret_cast<uint16_t*>(smem + kLdsLayer);
+    uint16_t* S = Wt + kHLayer;
+    // the wave index as a wave-uniform (SGPR) value: the output tiles a wave
+    // owns and its slot address are then scalar
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, g = lane >> 4;
+    uint16_t* my = S + w * kSlot;
     const int k = a.k;
     load_layer(f, k, L, true, Wt);
     __syncthreads();
@@ -616,15 +646,23 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
     const float inv_c = csum > 0 ? (float)(1.0 / csum) : 0.f;
     const uint32_t am = f.amask[k];
     const uint64_t ntiles = (a.n + 15) / 16;
-    const int nw = blockDim.x >> 6;
+    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+    f32x4_t acc3[3], acc2[9], acc1 = zero;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc2[q] = zero;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc3[q] = zero;
+    // all-ones operand (1 in column 0): row sums, the bias gradients
+    bf16x4_t ones;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ones[j] = (lane & 15) == 0 ? (short)0x3F80 : (short)0;
+    const bf16x8_t ones8 = cat8(ones, ones);
     // The inputs of a wave's next tile (saved u, c, and G or u0) are loaded
     // while the current tile computes.
-    const f32x4_t zero4 = {0.f, 0.f, 0.f, 0.f};
-    const uint64_t stride = (uint64_t)gridDim.x * nw;
     auto fetch = [&](uint64_t tl, f32x4_t& u, f32x4_t& gr, float& c) {
         const uint64_t s2 = tl * 16 + (lane & 15);
-        u = zero4;
-        gr = zero4;
+        u = zero;
+        gr = zero;
         c = 0.f;
         if (tl < ntiles && s2 < a.n) {
             u = *reinterpret_cast<const f32x4_t*>(a.saved + ((size_t)k * a.n + s2) * 16 + 4 * g);
@@ -632,16 +670,18 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
             gr = *reinterpret_cast<const f32x4_t*>((k == 0 ? a.u0 : a.G) + s2 * 16 + 4 * g);
         }
     };
-    uint64_t tile = (uint64_t)blockIdx.x * nw + (threadIdx.x >> 6);
+    const uint64_t per_round = (uint64_t)gridDim.x * kBwdWaves;
+    const uint64_t rounds = (ntiles + per_round - 1) / per_round;
+    uint64_t tile = (uint64_t)blockIdx.x * kBwdWaves + w;
     f32x4_t nx_u, nx_g;
     float nx_c;
     fetch(tile, nx_u, nx_g, nx_c);
-    for (; tile < ntiles; tile += stride) {
+    for (uint64_t rd = 0; rd < rounds; ++rd, tile += per_round) {
         const uint64_t sl = tile * 16 + (lane & 15);
         const bool live = sl < a.n;
         const f32x4_t uin = nx_u, graw = nx_g;
         const float cw = nx_c * inv_c;
-        fetch(tile + stride, nx_u, nx_g, nx_c);
+        fetch(tile + per_round, nx_u, nx_g, nx_c);
         f32x4_t G = graw;
         if (k == 0) {
 #pragma unroll
@@ -652,15 +692,6 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
         f32x4_t o[2];
         MlpKeep kp;
         mlp<true>(L, ain, lane, o, &kp);
-        // the forward operands of the weight gradients go out now, so h1 and
-        // h2 are not live through the backward GEMMs (only their masks are)
-        uint16_t* to = a.ops + (size_t)tile * kRows * 16;
-        put_rows(to, kRa, ain, lane, tr);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            put_rows(to, kRh1 + 16 * q, kp.h1[q], lane, tr);
-            put_rows(to, kRh2 + 16 * q, kp.h2[q], lane, tr);
-        }
         // coupling backward (inverse direction)
         f32x4_t dout0, dout1, Gin;
 #pragma unroll
@@ -681,10 +712,31 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
             }
         }
         const bf16x4_t do0 = pack_bf(dout0), do1 = pack_bf(dout1);
-        put_rows(to, kRdo, do0, lane, tr);
-        put_rows(to, kRdo + 16, do1, lane, tr);
+        // The staging and operand addresses of the phases below are formed
+        // from this copy of the lane index when used (immediate offsets of the
+        // LDS instructions): held from round to round they would take about
+        // forty registers.
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        // ---- phase 1: dout | h2 -> dW3, b3 (output tile o = wave + 8 j)
+        stage(my, 0, do0, ln);
+        stage(my, 16, do1, ln);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) stage(my, 32 + 16 * q, kp.h2[q], ln);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kBwdWaves; j += 2) {
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const int ot = w + kBwdWaves * e;
+                if (ot < kNo3) {
+                    const int q = ot % 9;
+                    acc3[e] = mfma32(frag_pair(S, j, 16 * (ot / 9), ln),
+                                     q < 8 ? frag_pair(S, j, 32 + 16 * q, ln) : ones8, acc3[e]);
+                }
+            }
+        }
         // dh2 = W3^T dout (relu mask h2); dh1 = W2^T dh2 (mask h1); da = W1^T dh1
-        const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
         bf16x4_t dh2[8];
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
@@ -704,8 +756,6 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
             for (int mt = 0; mt < 8; ++mt) acc[mt] = mfma32(frag_a2(Wt + kTW2, kT2, mt, kt, lane), b, acc[mt]);
             __builtin_amdgcn_sched_barrier(0);
         }
-#pragma unroll
-        for (int q = 0; q < 8; ++q) put_rows(to, kRdh2 + 16 * q, dh2[q], lane, tr);
         bf16x4_t dh1[8];
 #pragma unroll
         for (int mt = 0; mt < 8; ++mt) {
@@ -723,121 +773,55 @@ __global__ __launch_bounds__(kBwdThreads) void flow_backward_kernel(FlowArgs f, 
             if (d < f.D && ((am >> d) & 1u)) Gin[i] += da[i];
         }
         if (live) *reinterpret_cast<f32x4_t*>(a.G + sl * 16 + 4 * g) = Gin;
-        // the backward operands of the weight gradients
+        __syncthreads();
+        // ---- phase 2: dh2 | h1 -> dW2, b2
+        {
+            bf16x4_t h1[8];
+            layer1(L, ain, lane, h1);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) put_rows(to, kRdh1 + 16 * q, dh1[q], lane, tr);
-    }
-}
-
-// ---------------------------------------------------------- weight grads ----
-// dW[ra][rb] = sum_samples A[ra][s] B[rb][s] over the operand tiles, with a
-// virtual all-ones B row tile for the biases.  8 waves: wave w accumulates
-// dW2 rows 16w.. (8 tiles + bias), dW1 rows 16w.. (1 tile) and, for w < 2,
-// dW3 rows 16w.. (8 tiles + bias).  Partial sums per workgroup.
-// The operand tiles (kRows x 16 bf16, 17.5 KiB) stream from HBM: the
-// workgroup copies each tile into one of two LDS buffers with 16-byte loads
-// that are issued kWgAhead tiles ahead (a register ring), so several tiles
-// are in flight while the MFMAs run on the current one.
-constexpr int kWgThreads = 512;
-constexpr int kWgChunkMax = 256;  // sample tiles per workgroup (at most; fewer for small batches)
-constexpr int kWgAhead = 4;     // tiles in flight per workgroup
-constexpr int kTileChunks = kRows * 16 * 2 / 16;                          // 16-byte pieces per tile
-constexpr int kChunksPerThread = (kTileChunks + kWgThreads - 1) / kWgThreads;
-
-typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
-
-__device__ inline bf16x4_t frag_op(const uint16_t* tileops, int row0, int lane) {
-    return *reinterpret_cast<const bf16x4_t*>(tileops + (row0 + (lane & 15)) * 16 + (lane >> 4) * 4);
-}
-
-struct WgRegs {
-    u32x4_t v[kChunksPerThread];
-};
-
-__device__ inline void wg_fetch(const uint16_t* ops, uint64_t t, uint64_t t1, WgRegs& r) {
-    if (t >= t1) return;
-    const u32x4_t* src = reinterpret_cast<const u32x4_t*>(ops + (size_t)t * kRows * 16);
-#pragma unroll
-    for (int j = 0; j < kChunksPerThread; ++j) {
-        const int c = threadIdx.x + kWgThreads * j;
-        if (c < kTileChunks) r.v[j] = __builtin_nontemporal_load(src + c);
-    }
-}
-
-__device__ inline void wg_put(uint16_t* buf, const WgRegs& r) {
-    u32x4_t* dst = reinterpret_cast<u32x4_t*>(buf);
-#pragma unroll
-    for (int j = 0; j < kChunksPerThread; ++j) {
-        const int c = threadIdx.x + kWgThreads * j;
-        if (c < kTileChunks) dst[c] = r.v[j];
-    }
-}
-
-struct WgAcc {
-    f32x4_t acc2[9], acc3[9], acc1;
-};
-
-__device__ inline void wg_mfma(const uint16_t* to, int w, int lane, const bf16x4_t ones, WgAcc& A) {
-    const bf16x4_t a2 = frag_op(to, kRdh2 + 16 * w, lane);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) A.acc2[q] = mfma16(a2, frag_op(to, kRh1 + 16 * q, lane), A.acc2[q]);
-    A.acc2[8] = mfma16(a2, ones, A.acc2[8]);
-    A.acc1 = mfma16(frag_op(to, kRdh1 + 16 * w, lane), frag_op(to, kRa, lane), A.acc1);
-    if (w < 2) {
-        const bf16x4_t a3 = frag_op(to, kRdo + 16 * w, lane);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) A.acc3[q] = mfma16(a3, frag_op(to, kRh2 + 16 * q, lane), A.acc3[q]);
-        A.acc3[8] = mfma16(a3, ones, A.acc3[8]);
-    }
-}
-
-__global__ __launch_bounds__(kWgThreads) void flow_wgrad_kernel(const uint16_t* ops, uint64_t ntiles, int chunk,
-                                                                 float* part) {
-    __shared__ __align__(16) uint16_t Lt[2][kRows * 16];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
-    WgAcc A;
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        A.acc2[q] = zero;
-        A.acc3[q] = zero;
-    }
-    A.acc1 = zero;
-    bf16x4_t ones;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) ones[j] = (lane & 15) == 0 ? (short)0x3F80 : (short)0;
-    const uint64_t t0 = (uint64_t)blockIdx.x * chunk, t1 = min(t0 + (uint64_t)chunk, ntiles);
-    WgRegs R[kWgAhead];
-#pragma unroll
-    for (int k = 0; k < kWgAhead; ++k) wg_fetch(ops, t0 + k, t1, R[k]);
-    // tile t: its registers R[(t - t0) % kWgAhead] go to Lt[(t - t0) & 1];
-    // one barrier per tile (the other buffer's readers finished before it)
-    for (uint64_t tb = t0; tb < t1; tb += kWgAhead) {
-#pragma unroll
-        for (int k = 0; k < kWgAhead; ++k) {
-            const uint64_t t = tb + k;
-            if (t < t1) {
-                uint16_t* buf = Lt[k & 1];
-                wg_put(buf, R[k]);
-                __syncthreads();
-                wg_fetch(ops, t + kWgAhead, t1, R[k]);
-                wg_mfma(buf, w, lane, ones, A);
+            for (int q = 0; q < 8; ++q) {
+                stage(my, 16 * q, dh2[q], ln);
+                stage(my, kFH + 16 * q, h1[q], ln);
             }
         }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kBwdWaves; j += 2) {
+            const bf16x8_t av = frag_pair(S, j, 16 * w, ln);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc2[q] = mfma32(av, frag_pair(S, j, kFH + 16 * q, ln), acc2[q]);
+            acc2[8] = mfma32(av, ones8, acc2[8]);
+        }
+        __syncthreads();
+        // ---- phase 3: dh1 | a -> dW1
+#pragma unroll
+        for (int q = 0; q < 8; ++q) stage(my, 16 * q, dh1[q], ln);
+        stage(my, kFH, ain, ln);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kBwdWaves; j += 2) acc1 = mfma32(frag_pair(S, j, 16 * w, ln), frag_pair(S, j, kFH, ln), acc1);
+        __syncthreads();
     }
-    // C layout: lane l, i -> dW[16 w + (l/16)*4 + i][16 q + l%16]
-    float* o = part + (size_t)blockIdx.x * kPLayer;
+    // C layout: lane l, i -> dW[16 r + (l/16)*4 + i][16 q + l%16]
+    float* o = a.part + (size_t)blockIdx.x * kPLayer;
+    const int cc = lane & 15;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int r = 16 * w + (lane >> 4) * 4 + i;
+        const int rr = 4 * g + i, r2 = 16 * w + rr;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) o[kPW2 + r * kFH + 16 * q + (lane & 15)] = A.acc2[q][i];
-        if ((lane & 15) == 0) o[kPB2 + r] = A.acc2[8][i];
-        o[kPW1 + r * kFI + (lane & 15)] = A.acc1[i];
-        if (w < 2) {
+        for (int q = 0; q < 8; ++q) o[kPW2 + r2 * kFH + 16 * q + cc] = acc2[q][i];
+        if (cc == 0) o[kPB2 + r2] = acc2[8][i];
+        o[kPW1 + r2 * kFI + cc] = acc1[i];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) o[kPW3 + r * kFH + 16 * q + (lane & 15)] = A.acc3[q][i];
-            if ((lane & 15) == 0) o[kPB3 + r] = A.acc3[8][i];
+        for (int j = 0; j < 3; ++j) {
+            const int ot = w + kBwdWaves * j;
+            if (ot < kNo3) {
+                const int q = ot % 9, row = 16 * (ot / 9) + rr;
+                if (q < 8)
+                    o[kPW3 + row * kFH + 16 * q + cc] = acc3[j][i];
+                else if (cc == 0)
+                    o[kPB3 + row] = acc3[j][i];
+            }
         }
     }
 }
@@ -971,7 +955,7 @@ struct vgx_flow {
     uint16_t* H = nullptr;
     uint16_t* HT = nullptr;
     // per-batch scratch (grown on demand)
-    Scratch saved, u0, logq, c, G, ops, wpart, cpart, spart, misc, fpar;
+    Scratch saved, u0, logq, c, G, wpart, cpart, spart, misc, fpar;
     int step = 0;
 };
 
@@ -986,11 +970,9 @@ static FlowArgs flow_args(const vgx_flow* fl) {
     return f;
 }
 
-static size_t lds_layer() { return (sizeof(LayerLds) + 15) & ~(size_t)15; }
-// backward: the layer, the transposed weights, one transpose block per wave
-static size_t lds_bwd() {
-    return lds_layer() + sizeof(uint16_t) * kHLayer + (size_t)(kBwdThreads / 64) * 16 * kTrStride * sizeof(uint16_t);
-}
+static size_t lds_layer() { return kLdsLayer; }
+// backward: the layer, the transposed weights, the operand slots of the waves
+static size_t lds_bwd() { return kLdsBwd; }
 
 static int flow_cast(vgx_flow* fl, hipStream_t s) {
     hipLaunchKernelGGL(flow_cast_kernel, dim3(256), dim3(256), 0, s, (const float*)fl->P, fl->H, fl->HT, fl->L);
@@ -1035,7 +1017,7 @@ extern "C" int vgx_flow_destroy(vgx_flow* fl) {
     void* ps[] = {fl->P, fl->m, fl->v, fl->grad, fl->H, fl->HT};
     for (void* p : ps)
         if (p) (void)hipFree(p);
-    Scratch* ss[] = {&fl->saved, &fl->u0, &fl->logq, &fl->c, &fl->G, &fl->ops, &fl->wpart, &fl->cpart, &fl->spart,
+    Scratch* ss[] = {&fl->saved, &fl->u0, &fl->logq, &fl->c, &fl->G, &fl->wpart, &fl->cpart, &fl->spart,
                      &fl->misc, &fl->fpar};
     for (Scratch* q : ss) q->release();
     delete fl;
@@ -1186,15 +1168,13 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
     const int nc = 1024;
     if ((rc = fl->c.ensure(nn * 4))) return rc;
     if ((rc = fl->G.ensure(nn * 64))) return rc;
-    if ((rc = fl->ops.ensure((size_t)ntiles * kRows * 16 * 2))) return rc;
     if ((rc = fl->cpart.ensure(nc * 8))) return rc;
     if ((rc = fl->misc.ensure(256))) return rc;
-    // tiles per workgroup: two workgroups per CU, between 16 and 256 tiles
-    DeviceInfo dinfo;
-    if ((rc = device_info(&dinfo))) return rc;
-    const int wg_chunk = (int)std::max<uint64_t>(
-        16, std::min<uint64_t>(kWgChunkMax, (ntiles + 2 * (uint64_t)dinfo.cus - 1) / (2 * (uint64_t)dinfo.cus)));
-    const int nparts = (int)((ntiles + wg_chunk - 1) / wg_chunk);
+    DeviceInfo info;
+    if ((rc = device_info(&info))) return rc;
+    // one workgroup per CU (the layer's weights and the operand slots fill
+    // the LDS), each writing its own weight-gradient sums
+    const int nparts = (int)std::min<uint64_t>((uint64_t)info.cus, (ntiles + kBwdWaves - 1) / kBwdWaves);
     if ((rc = fl->wpart.ensure((size_t)nparts * kPLayer * 4))) return rc;
     uint32_t* mx = (uint32_t*)fl->misc.ptr;
     hipLaunchKernelGGL(flow_zero_u32_kernel, dim3(1), dim3(64), 0, s, mx, 1);
@@ -1206,8 +1186,6 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
     hipLaunchKernelGGL(flow_csum_kernel, dim3(1), dim3(256), 0, s, (const double*)fl->cpart.ptr, nc,
                        (double*)fl->misc.ptr + 1);
     if ((rc = check_launch())) return rc;
-    DeviceInfo info;
-    if ((rc = device_info(&info))) return rc;
     const size_t blds = lds_bwd();
     for (int k = 0; k < fl->L; ++k) {
         BwdArgs b{};
@@ -1218,22 +1196,13 @@ extern "C" int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const
         b.c = (const float*)fl->c.ptr;
         b.csum = (const double*)fl->misc.ptr + 1;
         b.G = (float*)fl->G.ptr;
-        b.ops = (uint16_t*)fl->ops.ptr;
-        // one workgroup per CU: the layer's weights fill most of the LDS, so a
-        // second workgroup per CU would only run after the first
-        const int bb = (int)std::min<uint64_t>((uint64_t)info.cus, (ntiles + 7) / 8);
+        b.part = (float*)fl->wpart.ptr;
         hipEvent_t e0, e1;
         if (timed_slot(g_timing_ctx, kTimedFlowBwd, &e0, &e1))
-            hipExtLaunchKernelGGL(flow_backward_kernel, dim3(bb), dim3(kBwdThreads), (uint32_t)blds, s, e0, e1, 0,
+            hipExtLaunchKernelGGL(flow_backward_kernel, dim3(nparts), dim3(kBwdThreads), (uint32_t)blds, s, e0, e1, 0,
                                   flow_args(fl), b);
         else
-            hipLaunchKernelGGL(flow_backward_kernel, dim3(bb), dim3(kBwdThreads), blds, s, flow_args(fl), b);
-        if (timed_slot(g_timing_ctx, kTimedFlowWgrad, &e0, &e1))
-            hipExtLaunchKernelGGL(flow_wgrad_kernel, dim3(nparts), dim3(kWgThreads), 0, s, e0, e1, 0,
-                                  (const uint16_t*)fl->ops.ptr, ntiles, wg_chunk, (float*)fl->wpart.ptr);
-        else
-            hipLaunchKernelGGL(flow_wgrad_kernel, dim3(nparts), dim3(kWgThreads), 0, s, (const uint16_t*)fl->ops.ptr,
-                               ntiles, wg_chunk, (float*)fl->wpart.ptr);
+            hipLaunchKernelGGL(flow_backward_kernel, dim3(nparts), dim3(kBwdThreads), blds, s, flow_args(fl), b);
         hipLaunchKernelGGL(flow_wgrad_reduce_kernel, dim3((kPLayer + kRedCols - 1) / kRedCols),
                            dim3(kRedCols * kRedGroups), 0, s,
                            (const float*)fl->wpart.ptr, nparts,

@@ -89,7 +89,7 @@ enum Timed : int32_t {
     kTimedFused = 1,      // fused_kernel (single integral)
     kTimedBatchFused = 2, // b_fused_kernel (batched integrals, eager calls)
     kTimedFlowBwd = 3,    // flow_backward_kernel
-    kTimedFlowWgrad = 4,  // flow_wgrad_kernel
+    kTimedFlowWgrad = 4,  // (unused: the weight gradients are part of flow_backward_kernel)
     kTimedTrainRows = 5,  // train_rows_kernel
     kTimedFlowSample = 6, // flow_sample_kernel
     kTimedCov = 7         // the multi-component covariance pass
