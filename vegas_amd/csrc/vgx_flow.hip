@@ -609,10 +609,28 @@ constexpr int kSlot = kSlotRows * 16;      // bf16 elements per wave slot
 constexpr int kNo3 = 2 * 9;                // dW3 output tiles: 2 row tiles x (8 column tiles + bias)
 
 // The C-layout tile v (rows row0 + (l/16)*4 + i, sample l%16) into a slot.
+// Lane (g, 4m + t) holds rows 4g..4g+3 of sample 4m + t; a 4x4 transpose of
+// the 16-bit values within the lane quad (DPP) leaves it row 4g + t of
+// samples 4m..4m+3, which it stores as one 8-byte write: the wave writes 512
+// contiguous bytes.
 __device__ inline void stage(uint16_t* slot, int row0, const bf16x4_t v, int lane) {
-    const int g = lane >> 4, c = lane & 15;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) slot[(row0 + 4 * g + i) * 16 + c] = (uint16_t)v[i];
+    typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+    const u32x2_t d = __builtin_bit_cast(u32x2_t, v);
+    const int t = lane & 3;
+    // dwords with lane t^2: lanes 0,1 keep (x0,x1) and take (x0,x1) of lane
+    // t+2; lanes 2,3 take (x2,x3) of lane t-2 and keep (x2,x3)
+    const uint32_t p0 = (uint32_t)__builtin_amdgcn_mov_dpp((int)d[0], 0x4E, 0xF, 0xF, false);
+    const uint32_t p1 = (uint32_t)__builtin_amdgcn_mov_dpp((int)d[1], 0x4E, 0xF, 0xF, false);
+    const uint32_t a0 = t < 2 ? d[0] : p1, a1 = t < 2 ? p0 : d[1];
+    // 16-bit halves with lane t^1: even lanes lo(own) | lo(odd) << 16, odd
+    // lanes hi(even) | hi(own) << 16
+    const uint32_t q0 = (uint32_t)__builtin_amdgcn_mov_dpp((int)a0, 0xB1, 0xF, 0xF, false);
+    const uint32_t q1 = (uint32_t)__builtin_amdgcn_mov_dpp((int)a1, 0xB1, 0xF, 0xF, false);
+    const uint32_t sel = (t & 1) ? 0x03020706u : 0x05040100u;
+    u32x2_t r;
+    r[0] = __builtin_amdgcn_perm(q0, a0, sel);
+    r[1] = __builtin_amdgcn_perm(q1, a1, sel);
+    *reinterpret_cast<u32x2_t*>(slot + (row0 + 4 * (lane >> 4) + t) * 16 + 4 * ((lane >> 2) & 3)) = r;
 }
 // Operand fragment of slot rows row0.. (A and B alike: row row0 + l%16,
 // samples (l/16)*4 .. +3).

@@ -311,9 +311,12 @@ class FlowMap:
         per-sample variance on ``probe`` samples gives the sampling time
         still needed for rtol, and training stops after two chunks in a row
         that did not lower the projected time to the target (training so far
-        plus that sampling time).  On the narrower 8-D ridge the fixed 200
-        steps leave 2-3 times the variance that 1000 more remove.  The
-        flow's weights then depend on these timings (not on the seed alone).""" 
+        plus that sampling time), and the weights of the best projection
+        are restored (the variance loss is noisy on heavy-tailed weights: a
+        chunk can undo the previous ones).  On the narrower 8-D ridge the
+        fixed 200 steps leave 2-3 times the variance that 1000 more remove.
+        The flow's weights then depend on these timings (not on the seed
+        alone).""" 
         x, fx, lw = vegas_training_data(integ, fcn, self.flow, int(ntrain))
         self.train(fcn, x, fx, lw, epochs=kl_epochs, batch=batch, lr=lr, clip=clip, loss='kl')
         del x, fx, lw
@@ -335,6 +338,7 @@ class FlowMap:
         torch = _lib.torch_cuda()
         t_start = time.perf_counter()
         best = remaining()  # projected time from here: further training plus sampling
+        best_p = self.flow.params()
         done, stale = self_var, 0
         while done < max_var and stale < 2:
             self.train(fcn, None, None, None, epochs=0, self_batches=chunk, self_batch=batch, lr=lr * 0.3,
@@ -346,8 +350,13 @@ class FlowMap:
             self.history.append((done, now))
             if total < best:
                 best, stale = total, 0
+                best_p = self.flow.params()
             else:
                 stale += 1
+        if stale:
+            # the last chunks did not improve on the best projection: its
+            # weights (Adam's state restarts with them)
+            self.flow.set_params(best_p)
         return self
 
     def per_sample_variance(self, fcn, n=1000000, iteration=0):
