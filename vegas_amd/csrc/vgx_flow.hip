@@ -611,8 +611,8 @@ constexpr int kNo3 = 2 * 9;                // dW3 output tiles: 2 row tiles x (8
 // The C-layout tile v (rows row0 + (l/16)*4 + i, sample l%16) into a slot.
 // Lane (g, 4m + t) holds rows 4g..4g+3 of sample 4m + t; a 4x4 transpose of
 // the 16-bit values within the lane quad (DPP) leaves it row 4g + t of
-// samples 4m..4m+3, which it stores as one 8-byte write: the wave writes 512
-// contiguous bytes.
+// samples 4m..4m+3, which it stores as one 8-byte write (at the permuted
+// chunk position frag_op reads).
 __device__ inline void stage(uint16_t* slot, int row0, const bf16x4_t v, int lane) {
     typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
     const u32x2_t d = __builtin_bit_cast(u32x2_t, v);
@@ -630,12 +630,18 @@ __device__ inline void stage(uint16_t* slot, int row0, const bf16x4_t v, int lan
     u32x2_t r;
     r[0] = __builtin_amdgcn_perm(q0, a0, sel);
     r[1] = __builtin_amdgcn_perm(q1, a1, sel);
-    *reinterpret_cast<u32x2_t*>(slot + (row0 + 4 * (lane >> 4) + t) * 16 + 4 * ((lane >> 2) & 3)) = r;
+    const int row = row0 + 4 * (lane >> 4) + t;
+    *reinterpret_cast<u32x2_t*>(slot + row * 16 + 4 * (((lane >> 2) & 3) ^ ((row >> 2) & 3))) = r;
 }
 // Operand fragment of slot rows row0.. (A and B alike: row row0 + l%16,
-// samples (l/16)*4 .. +3).
+// samples (l/16)*4 .. +3).  In a slot row the four 4-sample chunks are
+// permuted, chunk c at position c ^ ((row >> 2) & 3): the 16 rows that one
+// fragment read (and the 16 that one staging store) touches then fall in
+// distinct LDS banks, where the plain 32-byte rows put every fourth row in
+// the same banks.
 __device__ inline bf16x4_t frag_op(const uint16_t* slot, int row0, int lane) {
-    return *reinterpret_cast<const bf16x4_t*>(slot + (row0 + (lane & 15)) * 16 + (lane >> 4) * 4);
+    const int row = row0 + (lane & 15);
+    return *reinterpret_cast<const bf16x4_t*>(slot + row * 16 + 4 * ((lane >> 4) ^ ((row >> 2) & 3)));
 }
 // Rows row0.. of staged tiles j and j+1 as one k = 32 operand (A and B
 // alike: slots 0..3 tile j's samples (l/16)*4.., slots 4..7 tile j+1's).
