@@ -127,6 +127,40 @@ def test_grad_vs_oracle(torch):
             assert err < 2e-2, (k, name, err)
 
 
+@pytest.mark.parametrize("n", [1, 37, 300, 70000])
+def test_grad_sizes_and_reproducible(torch, n):
+    """The backward kernel forms the weight gradients in LDS, eight tiles of
+    16 samples per workgroup round: a batch smaller than one tile, one that
+    leaves most waves of the round without a tile, and one of several rounds
+    per workgroup all give the oracle's gradients, and a second call gives
+    the same bits (tiles go to workgroups in a fixed assignment)."""
+    from vegas_amd.flow import CouplingFlow, pack, unpack
+    dim = 7
+    P = _params(dim, 3, scale=0.1)
+    fl = CouplingFlow(dim, 6, params=pack(P))
+    rng = np.random.default_rng(n)
+    x = rng.uniform(0.05, 0.95, (n, dim)).astype(np.float32).astype(np.float64)
+    f = rng.uniform(0.1, 2.0, n).astype(np.float32).astype(np.float64)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32).astype(np.float64) / n
+    M = fo.masks(dim, 6)
+    _, lq = fo.inverse(P, M, x, True)
+    lc = np.log(w) + 2 * np.log(f) - lq
+    c = np.exp(lc - lc.max())
+    c /= c.sum()
+    g_o, _ = fo.backprop(P, M, x, c, bf=True)
+    xd = fl.padded(x)
+    fl.grad(xd, f, np.log(w))
+    G1 = fl.last_grad().copy()
+    fl.grad(xd, f, np.log(w))
+    assert np.array_equal(G1.view(np.uint32), fl.last_grad().view(np.uint32))
+    g = unpack(G1)
+    for k in range(6):
+        for name in ('W1', 'W2', 'b2', 'W3', 'b3'):
+            a, b = g[k][name], g_o[k][name]
+            err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            assert err < 2e-2, (n, k, name, err)
+
+
 def _ridge(dim, ncomp=50, sigma=0.03):
     import vegas_amd as va
     t = 0.3 + 0.4 * np.arange(ncomp) / (ncomp - 1)

@@ -340,10 +340,8 @@ __global__ __launch_bounds__(kFusedThreads) void b_fused_kernel(BArgs a, int it,
         tg.fast_pow = 1;
         tg.train_errors = 0;
         tg.iteration = iteration;
-        // one item per workgroup: its key is uniform (and the sampler's
-        // Philox keeps it in SGPRs)
-        tg.seed_lo = __builtin_amdgcn_readfirstlane(q.seed_lo);
-        tg.seed_hi = __builtin_amdgcn_readfirstlane(q.seed_hi);
+        tg.seed_lo = q.seed_lo;
+        tg.seed_hi = q.seed_hi;
         tg.beta_half = a.beta_half;
         tg.dv_y = q.dv_y;
         const AxisC* ax = a.axes + (size_t)i * kBD;

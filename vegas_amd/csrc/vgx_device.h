@@ -43,12 +43,7 @@ __host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 // The same function for device code: each round's two three-input XORs
 // (product half ^ counter word ^ key) are one v_bitop3_b32 each (truth
 // table 0x96 = a ^ b ^ c), half the XOR instructions of the plain form.
-// The key is wave-uniform at every call site and is pinned to SGPRs here, so
-// the round keys k + r*W are formed by scalar adds at each call: hoisted out
-// of a point loop they would hold 20 SGPRs for the whole loop and, in the
-// SGPR-bound samplers, be spilled to VGPR lanes and read back per point.
 __device__ __forceinline__ U4 philox4x32_10_dev(U4 c, uint32_t k0, uint32_t k1) {
-    asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
