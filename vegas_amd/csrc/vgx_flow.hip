@@ -235,7 +235,8 @@ __device__ inline void mlp(const LayerLds* s, bf16x4_t a, int lane, f32x4_t* o, 
 // ------------------------------------------------------------- sampling ----
 constexpr int kFlowThreads = 1024;
 constexpr int kFlowTiles = 4;        // sample tiles (16 samples) per wave per pass
-constexpr int kFlowPass = (kFlowThreads / 64) * kFlowTiles * 16;  // 1024 samples
+constexpr int kFlowWaves = kFlowThreads / 64;
+constexpr int kFlowPass = kFlowWaves * kFlowTiles * 16;  // samples per workgroup pass
 
 struct FlowFcn {
     int kind;            // VGX_FCN_GAUSSMIX or VGX_FCN_DIAGRIDGE
@@ -298,7 +299,7 @@ struct TileState {
     f32x4_t u[kFlowTiles][64];
     float l[kFlowTiles][64];
 };
-constexpr size_t kStateLds = 16 * sizeof(TileState);
+constexpr size_t kStateLds = kFlowWaves * sizeof(TileState);
 
 __global__ __launch_bounds__(kFlowThreads) void flow_sample_kernel(FlowArgs f, FlowFcn F, SampleArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(kFlowThreads) void flow_sample_kernel(FlowArgs f, F
     TileState* TS = reinterpret_cast<TileState*>(smem + ((sizeof(LayerLds) + 15) & ~(size_t)15)) +
                     (threadIdx.x >> 6);
     float* Lc = reinterpret_cast<float*>(smem + ((sizeof(LayerLds) + 15) & ~(size_t)15) + kStateLds);
-    __shared__ double red[16 * 4];
+    __shared__ double red[kFlowWaves * 4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4;
     const int ncp = F.kind == VGX_FCN_DIAGRIDGE ? 3 * F.ncomp : F.ncomp * (2 + f.D);
     for (int i = threadIdx.x; i < ncp; i += blockDim.x)
@@ -393,7 +394,7 @@ __global__ __launch_bounds__(kFlowThreads) void flow_sample_kernel(FlowArgs f, F
     __syncthreads();
     if (threadIdx.x == 0) {
         DD t1{0, 0}, t2{0, 0};
-        for (int w = 0; w < 16; ++w) {
+        for (int w = 0; w < kFlowWaves; ++w) {
             t1 = dd_sum(t1, DD{red[4 * w], red[4 * w + 1]});
             t2 = dd_sum(t2, DD{red[4 * w + 2], red[4 * w + 3]});
         }
