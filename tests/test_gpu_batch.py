@@ -188,7 +188,7 @@ def test_batch_nan_then_replay_in_place(va):
     assert np.all(np.isfinite(r.mean)) and np.all(r.mean > 0)
 
 
-def test_batch_4096_config_ranges(va):
+def test_batch_4096_config_ranges(va, orc):
     """The batched workload at its full size: 4096 integrals, dim U{2..10},
     centre U[0.2,0.8]^d, sigma log-U[0.01,0.1], neval log-U[1e4,1e6] (up to
     1e6 per integral), nitn 8 per graph launch, read after two warm-up calls
@@ -200,7 +200,13 @@ def test_batch_4096_config_ranges(va):
     converged integrals the pulls (estimate - exact) / sdev are a standard
     normal sample: at most 3 beyond 5 sigma, 95% +- 1.5% within 2 sigma.
     Every resolved integral beyond 5 sigma has Q < 1e-3 (vegas flags it).
-    The largest pulls are printed."""
+    The largest pulls are printed.  The integrand keeps the FP64 exponent
+    range (a point far from a narrow peak gives e^-300, not an FP32 0), so
+    the adaptation finds the narrow 8-10-D peaks from their tails as the
+    reference's FP64 evaluation does: at most 4 of the 4096 stay unresolved
+    after the 24 iterations (with FP32 exponentials about 150 did, all of
+    which the oracle resolves).  For those the oracle's own run (same 24
+    iterations, neval and increment cap) is printed beside the batch's."""
     from vegas_amd import workloads
     from vegas_amd.batch import BatchIntegrator, gaussian_params
     probs = workloads.batched_gaussians(4096)
@@ -226,7 +232,15 @@ def test_batch_4096_config_ranges(va):
     print("resolved %d of %d, converged %d; beyond 5 sigma: %d resolved, %d converged; within 2 sigma %.4f; "
           "RMS pull %.3f" % (resolved.sum(), len(exact), conv.sum(), (np.abs(pull_all) > 5).sum(),
                              (np.abs(pull) > 5).sum(), np.mean(np.abs(pull) < 2), math.sqrt(np.mean(pull ** 2))))
-    assert resolved.sum() > 3800 and conv.sum() > 3600
+    assert resolved.sum() >= 4092 and conv.sum() > 3900
     assert (np.abs(pull) > 5).sum() <= 3
     assert abs(np.mean(np.abs(pull) < 2) - 0.9545) < 0.015
     assert np.all(r.Q[resolved & (np.abs(pull_all) > 5)] < 1e-3) or (np.abs(pull) > 5).sum() <= 3
+    miss = [i for i in np.argsort(nev) if not resolved[i]][:8]
+    for i in miss:
+        g = orc.GaussMix([A[i]], [B[i]], [cs[i]])
+        o = orc.OracleIntegrator([(0, 1)] * int(dims[i]), neval=int(nev[i]), seed=1, maxinc_axis=500)
+        m, v, _ = o.run(g, 24)
+        om = orc.ravg(m[-8:], v[-8:])[0]
+        print("unresolved integral %d: dim %d sigma %.4f neval %d: batch %.3g, oracle %.3g, exact %.3g"
+              % (i, dims[i], sig[i], nev[i], r.mean[i], om, exact[i]))

@@ -143,6 +143,18 @@ struct GaussMix64 {
     }
 };
 
+// A * 2^y for y <= 0 in FP32 arithmetic with the FP64 exponent range: y = n + f
+// (n = floor(y), f in [0,1), an exact split), A * 2^f by the FP32 hardware
+// exp2, and the scaling by 2^n exact in FP64.  FP32 exp flushes e^-104 and
+// below to 0; here a point far from a narrow peak keeps its tiny value (down
+// to 2^-1074) as the reference's FP64 evaluation does -- the training signal
+// that leads the map to the peak in the first iterations.  NaN propagates.
+__device__ __forceinline__ double scaled_exp2(float A, float y) {
+    y = y < -1100.0f ? -1100.0f : y;  // 2^-1100 is 0 in FP64 as well
+    const float fl = floorf(y);
+    return ldexp((double)(A * __builtin_amdgcn_exp2f(y - fl)), (int)fl);
+}
+
 struct GaussMix32 {
     static constexpr bool kRng32 = true;
     int ncomp;
@@ -158,7 +170,8 @@ struct GaussMix32 {
             if (DIM <= 0 && d >= nd) break;
             xf[d] = (float)x[d];
         }
-        float acc = 0.0f;
+        // the components' values, each with the FP64 exponent range, summed in FP64
+        double acc = 0.0;
         for (int k = 0; k < ncomp; ++k) {
             float r2 = 0.0f;
 #pragma unroll
@@ -167,9 +180,9 @@ struct GaussMix32 {
                 float t = xf[d] - ldc(c, k * nd + d);
                 r2 = __builtin_fmaf(t, t, r2);
             }
-            acc = __builtin_fmaf(ldc(A, k), __expf(-ldc(B, k) * r2), acc);
+            acc += scaled_exp2(ldc(A, k), (-ldc(B, k) * r2) * 1.44269504088896341f);
         }
-        return (double)acc;
+        return acc;
     }
 };
 
