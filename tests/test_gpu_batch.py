@@ -204,8 +204,9 @@ def test_batch_4096_config_ranges(va, orc):
     range (a point far from a narrow peak gives e^-300, not an FP32 0), so
     the adaptation finds the narrow 8-10-D peaks from their tails as the
     reference's FP64 evaluation does: at most 4 of the 4096 stay unresolved
-    after the 24 iterations (with FP32 exponentials about 150 did, all of
-    which the oracle resolves).  For those the oracle's own run (same 24
+    after the 24 iterations (with FP32 exponentials about 150 did; the
+    oracle resolves the eight of those with the fewest samples).  For the
+    ones left the oracle's own run (same 24
     iterations, neval and increment cap) is printed beside the batch's."""
     from vegas_amd import workloads
     from vegas_amd.batch import BatchIntegrator, gaussian_params
