@@ -641,6 +641,10 @@ class Integrator:
         _lib.check(L.vgx_iter_begin(cx, C.byref(cfg), C.byref(st), 0, _lib.ptr(nh), C.byref(total),
                                     sp), 'iter_begin')
         ubuf = self._uniforms(torch, cfg, total.value)
+        # NaN in any component: flagged on the device batch by batch and read
+        # once, before the iteration closes (no adapt), instead of a host
+        # synchronisation per batch (pyx:2133-2134 raises at the batch)
+        nan_seen = torch.zeros((), dtype=torch.bool, device='cuda')
         for h0, h1, p0, p1 in _geometry.batches(nh.cpu().numpy(), self.min_neval_batch):
             n = p1 - p0
             self._draw(torch, ubuf, p0, p1)
@@ -653,8 +657,7 @@ class Integrator:
             f = std(x, jac)
             if f.shape[1] != m:
                 raise ValueError('integrand output changed shape (%d values, not %d)' % (f.shape[1], m))
-            if bool(torch.isnan(f).any()):
-                raise ValueError('integrand evaluates to nan')
+            nan_seen |= torch.isnan(f).any()
             # component 0 steers sigf and the training data (pyx:2173, 2185)
             f0 = f[:, 0].contiguous()
             _lib.check(L.vgx_iter_accumulate(cx, C.byref(cfg), C.byref(st), 0, h0, h1, p0, p1,
@@ -663,6 +666,8 @@ class Integrator:
                 _lib.check(L.vgx_iter_accumulate_cov(cx, C.byref(cfg), h0, h1, p0, p1, _lib.ptr(f), _lib.ptr(w),
                                                      m, corr, _lib.ptr(acc), _lib.ptr(cstat), sp),
                            'iter_accumulate_cov')
+        if bool(nan_seen):
+            raise ValueError('integrand evaluates to nan')
         _lib.check(L.vgx_iter_end(cx, C.byref(cfg), C.byref(st), 0, sp), 'iter_end')
         status, res = self._read_back(bufs)
         self._rng_iter += 1
