@@ -336,7 +336,9 @@ int vgx_flow_logq(vgx_flow* fl, const float* x, int64_t n, float* logq, float* u
  *                           fout) as f and log w = 0 (then w f^2 / q is
  *                           (f J)^2 / n: no log q is needed; likewise the KL
  *                           loss takes f J and log w = 0 for them).
- * Kept on the device for vgx_flow_adam (vgx_flow_get_params reads it).   */
+ * Kept on the device for vgx_flow_adam (vgx_flow_get_params reads it); the
+ * per-sample terms are summed in a fixed order, so the same inputs give the
+ * same bits.                                                              */
 enum { VGX_FLOW_LOSS_VARIANCE = 0, VGX_FLOW_LOSS_KL = 1, VGX_FLOW_LOSS_SQUARED = 2 };
 int vgx_flow_grad(vgx_flow* fl, const float* x, const float* f, const float* logw, int64_t n, int loss,
                   void* stream);
