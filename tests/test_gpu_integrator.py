@@ -118,6 +118,28 @@ def test_fp32_path_error_against_oracle(va, orc, wl):
     assert em <= 2e-5 and es <= 1e-3, (em, es)
 
 
+def test_fp32_path_far_tails_against_oracle(va, orc):
+    """The FP32 Gaussian functor keeps the FP64 exponent range: on a narrow
+    10-D peak (sigma 0.015) sampled by 2e4 uniform points, the first
+    iteration's estimate is made of far-tail values (the nearest points
+    near e^-500, below anything FP32 represents, so FP32 exp would give 0)
+    and still equals the FP64 oracle's on the same 24-bit points to the
+    FP32 rounding of the exponent (about 1e-7 of it: ~1e-4 relative here)."""
+    dim = 10
+    f = va.GaussianMixture.isotropic([[0.5] * dim], 0.015)
+    fo = orc.GaussMix(f.A, f.B, f.centers)
+    integ = va.Integrator([[0, 1]] * dim, neval=2e4, seed=29, precision='f32', alpha=0.0)
+    r = integ(f, nitn=1)
+    o = orc.OracleIntegrator([(0, 1)] * dim, neval=2e4, seed=29, alpha=0.0, rng_f32=True)
+    out, _, _ = o.iterate(fo)
+    assert integ.last_neval == out.last_neval
+    m = r.itn_results[0]
+    assert 0 < out.mean < 1e-46  # the estimate is all tails, below FP32's range
+    em = abs(m.mean - out.mean) / out.mean
+    print("far tails, FP32 path vs FP64 oracle: %.3g vs %.3g, %.2e relative" % (m.mean, out.mean, em))
+    assert em <= 1e-3, em
+
+
 @pytest.mark.parametrize("neval,scale", [(30000, 3.0), (2e5, 6.0)])
 def test_parity_variable_nh(va, orc, neval, scale):
     """Heavy-tailed sigf: n_h from 2 to thousands, hcubes spanning many wave
