@@ -231,3 +231,31 @@ def test_ravg_arrays_and_dicts():
     np.testing.assert_allclose(rd['b'].mean, [2.0, 3.0], rtol=1e-12)
     with pytest.raises(ValueError):
         va.ravg([])
+
+
+def test_ctypes_mirrors_match_the_c_abi(tmp_path):
+    """The ctypes structures the Python side passes through the C-ABI have
+    the size and field offsets of the C structures in include/vgx.h (a C
+    compiler's own layout, field by field)."""
+    from vegas_amd import _lib
+    from vegas_amd import batch as _batch
+    mirrors = {"vgx_config": _lib.VgxConfig, "vgx_integrand": _lib.VgxIntegrand, "vgx_state": _lib.VgxState,
+               "vgx_batch_config": _batch._BatchConfig, "vgx_batch_state": _batch._BatchState}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "vgx.h"', 'int main(void) {']
+    for cname, py in mirrors.items():
+        lines.append('  printf("%s sizeof %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('  printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, fname, cname, fname))
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    cmd = ["gcc", "-std=c11", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)]
+    if subprocess.run(cmd, capture_output=True).returncode != 0:
+        pytest.skip("no host C compiler")
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in mirrors.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
