@@ -181,6 +181,21 @@ __device__ __forceinline__ double dpp_d(double old, double x) {
     const uint32_t hi = (uint32_t)dpp_i<CTRL, ROWS>((int)(uint32_t)(o >> 32), (int)(uint32_t)(v >> 32));
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
+// Another lane's value with every row enabled and 0 where the lane has no
+// source (bound_ctrl): the value dpp_d(0.0, x) gives, without first writing
+// the zero into the destination (an FP64 consumer cannot take the DPP
+// modifier itself, so the old-value form costs two more moves per double).
+template <int CTRL>
+__device__ __forceinline__ int dpp0_i(int x) {
+    return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp0_d(double x) {
+    const uint64_t v = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)dpp0_i<CTRL>((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)dpp0_i<CTRL>((int)(uint32_t)(v >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 
 // The FP64 sum of the 64 lanes' values in lane 63, through DPP moves (no
 // LDS): quads (xor 1, xor 2), halves and rows (mirrors), then rows 1 and 3
@@ -188,10 +203,10 @@ __device__ __forceinline__ double dpp_d(double old, double x) {
 // 31).  FP64 addition is commutative, so the lanes of one row agree and the
 // result is a fixed function of the 64 inputs (a fixed summation tree).
 __device__ inline double wave_total(double v) {
-    v += dpp_d<0xB1>(0.0, v);    // quad_perm [1,0,3,2]
-    v += dpp_d<0x4E>(0.0, v);    // quad_perm [2,3,0,1]
-    v += dpp_d<0x141>(0.0, v);   // row_half_mirror
-    v += dpp_d<0x140>(0.0, v);   // row_mirror
+    v += dpp0_d<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp0_d<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp0_d<0x141>(v);   // row_half_mirror
+    v += dpp0_d<0x140>(v);   // row_mirror
     v += dpp_d<kDppRowBcast15, 0xA>(0.0, v);
     v += dpp_d<kDppRowBcast31, 0xC>(0.0, v);
     return v;
@@ -199,16 +214,22 @@ __device__ inline double wave_total(double v) {
 
 // One step of a segmented inclusive scan (v += v of the source lane unless a
 // segment starts at or before this lane since the source), for 1 or 2 values.
+// dpp_d(0.0, x) in its cheaper form where every row is enabled.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_z(double x) {
+    if constexpr (ROWS == 0xF) return dpp0_d<CTRL>(x);
+    else return dpp_d<CTRL, ROWS>(0.0, x);
+}
 template <int CTRL, int ROWS>
 __device__ __forceinline__ void segscan_step(double& v, int& f) {
-    const double pv = dpp_d<CTRL, ROWS>(0.0, v);
+    const double pv = dpp_z<CTRL, ROWS>(v);
     const int pf = dpp_i<CTRL, ROWS>(0, f);
     if (!f) v = pv + v;
     f |= pf;
 }
 template <int CTRL, int ROWS>
 __device__ __forceinline__ void segscan_step2(double& a, double& b, int& f) {
-    const double pa = dpp_d<CTRL, ROWS>(0.0, a), pb = dpp_d<CTRL, ROWS>(0.0, b);
+    const double pa = dpp_z<CTRL, ROWS>(a), pb = dpp_z<CTRL, ROWS>(b);
     const int pf = dpp_i<CTRL, ROWS>(0, f);
     if (!f) {
         a = pa + a;

@@ -63,6 +63,11 @@ __device__ __forceinline__ AxisC ldc(const AxisC* p, int i) {
     return AxisC{ldc(u, 0), ldc(u, 1), ldc(u, 2), ldc(u, 3), ldc(f, 2), ldc(u, 6), 0u};
 }
 
+// The LDS counts hold bins 2i and 2i+1 as the low and high halves of word i:
+// bin idx adds 1 << (16 (idx & 1)).  (idx << 4) & 31 is that shift, in the
+// form the 32-bit shift takes without a mask (it reads 5 bits of the count).
+__device__ __forceinline__ uint32_t pair_inc(int idx) { return 1u << (((uint32_t)idx << 4) & 31u); }
+
 // Stratum coordinates of hcube h, axis 0 fastest (pyx:1736-1738), each
 // premultiplied by the axis's increments per stratum (the first increment
 // index of the stratum, < 2^16).
@@ -281,7 +286,7 @@ __device__ __forceinline__ void seg_prefix2(const int (&key)[kK], double (&a)[kK
     const int prev_last = dpp_i<kDppWaveShr1>(-1, key[kK - 1]);  // lane 0: -1
     double va = a[kK - 1], vb = b[kK - 1];
     wave_segscan2(va, vb, key[kK - 1] != prev_last);
-    const double ca = dpp_d<kDppWaveShr1>(0.0, va), cb = dpp_d<kDppWaveShr1>(0.0, vb);
+    const double ca = dpp0_d<kDppWaveShr1>(va), cb = dpp0_d<kDppWaveShr1>(vb);
     const bool carry = lane > 0 && key[0] == prev_last;
 #pragma unroll
     for (int j = 0; j < kK; ++j)
@@ -301,7 +306,7 @@ __device__ __forceinline__ void seg_prefix(const int (&key)[kK], const double (&
     const int prev_last = dpp_i<kDppWaveShr1>(-1, key[kK - 1]);  // lane 0: -1
     const int head = key[kK - 1] != prev_last;
     const double incl = wave_segscan(P[kK - 1], head);
-    const double prev_incl = dpp_d<kDppWaveShr1>(0.0, incl);
+    const double prev_incl = dpp0_d<kDppWaveShr1>(incl);
     const double cin = (lane > 0 && key[0] == prev_last) ? prev_incl : 0.0;
 #pragma unroll
     for (int j = 0; j < kK; ++j)
@@ -473,7 +478,7 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
                             for (int d = 0; d < ND; ++d) {
                                 if (DIM <= 0 && d >= dim) break;
                                 const int idx = d * g.nb + kb[d];
-                                atomicAdd(&Lc[idx >> 1], 1u + (uint32_t)(idx & 1) * 65535u);
+                                atomicAdd(&Lc[idx >> 1], pair_inc(idx));
                             }
                         } else {
 #pragma unroll
@@ -482,7 +487,7 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
                                 const int k = kb[d];
                                 if (k < 0 || ((zm >> d) & 1)) continue;
                                 const int idx = d * g.nb + k;
-                                atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
+                                atomicAdd(&Lc[idx >> 1], pair_inc(idx));
                             }
                         }
                     }
@@ -493,7 +498,7 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
                         if (DIM <= 0 && d >= dim) break;
                         const int idx = d * g.nb + kb[d];
                         unsafeAtomicAdd(&Ls[idx], c2);
-                        atomicAdd(&Lc[idx >> 1], 1u + (uint32_t)(idx & 1) * 65535u);
+                        atomicAdd(&Lc[idx >> 1], pair_inc(idx));
                     }
                 } else {
                     const uint64_t zm = (PER && zmin == 0) ? per_zero_axes<DIM>(g, pc, pj - tp.pshift) : 0ull;
@@ -504,7 +509,7 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
                         if (k < 0 || ((zm >> d) & 1)) continue;
                         const int idx = d * g.nb + k;
                         unsafeAtomicAdd(&Ls[idx], c2);
-                        atomicAdd(&Lc[idx >> 1], (idx & 1) ? 65536u : 1u);
+                        atomicAdd(&Lc[idx >> 1], pair_inc(idx));
                     }
                 }
             }
@@ -538,20 +543,21 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
         if (key[j] == key[0]) fv = P[j];
     double rv = fv;
     int rf = (key[0] != key[kK - 1]) || endf[kK - 1];
-    // Within each 16-lane row by DPP (a lane with no source keeps its own
-    // value, which leaves it unchanged): lane i then holds the first flagged
-    // lane of [i, end of its row].  Across rows by permutes from the first
-    // lane of the row 1, then 2 rows ahead, which extends that to [i, 63].
+    // Within each 16-lane row by DPP (a lane with no source reads value and
+    // flag 0; a value is only used once its flag is set, so that is as if it
+    // kept its own): lane i then holds the first flagged lane of [i, end of
+    // its row].  Across rows by permutes from the first lane of the row 1,
+    // then 2 rows ahead, which extends that to [i, 63].
     auto back_step = [&](double nv, int nf) {
         if (!rf) {
             rv = nv;
             rf = nf;
         }
     };
-    back_step(dpp_d<kDppRowShl + 1>(rv, rv), dpp_i<kDppRowShl + 1>(rf, rf));
-    back_step(dpp_d<kDppRowShl + 2>(rv, rv), dpp_i<kDppRowShl + 2>(rf, rf));
-    back_step(dpp_d<kDppRowShl + 4>(rv, rv), dpp_i<kDppRowShl + 4>(rf, rf));
-    back_step(dpp_d<kDppRowShl + 8>(rv, rv), dpp_i<kDppRowShl + 8>(rf, rf));
+    back_step(dpp0_d<kDppRowShl + 1>(rv), dpp0_i<kDppRowShl + 1>(rf));
+    back_step(dpp0_d<kDppRowShl + 2>(rv), dpp0_i<kDppRowShl + 2>(rf));
+    back_step(dpp0_d<kDppRowShl + 4>(rv), dpp0_i<kDppRowShl + 4>(rf));
+    back_step(dpp0_d<kDppRowShl + 8>(rv), dpp0_i<kDppRowShl + 8>(rf));
 #pragma unroll
     for (int hop = 1; hop <= 2; hop <<= 1) {
         const int src = ((lane >> 4) + hop) << 4;
@@ -559,7 +565,7 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
         const int nf = __shfl(rf, src & (kWave - 1), kWave);
         if (src < kWave) back_step(nv, nf);
     }
-    const double next_rv = dpp_d<kDppWaveShl1>(0.0, rv);
+    const double next_rv = dpp0_d<kDppWaveShl1>(rv);
     const double vlast = endf[kK - 1] ? P[kK - 1] : next_rv;
     double S[kK];
     {
