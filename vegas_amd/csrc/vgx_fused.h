@@ -145,8 +145,9 @@ struct CoordWalk {
 //   (pyx:355-359).
 //   kbin[d]: the training bin of axis d, -1 where y is not inside (0,1)
 //   (pyx:459-463); with PER the bins are always given and zmin (min over the
-//   axes of c|u, to be initialised by the caller) is 0 iff some axis has
-//   y = 0 exactly -- per_zero_axes() then names the axes to skip.
+//   axes of the 24-bit uniform, to be initialised by the caller) is 0 when
+//   some axis may have y = 0 exactly (y = 0 needs u = 0 and stratum 0) --
+//   per_zero_axes() then names the axes to skip, if any.
 //   DIM > 0: that dimension; DIM <= 0: runtime g.dim, at most nd_max(DIM).
 template <int DIM, bool R32, bool PER>
 __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __restrict__ ax,
@@ -176,8 +177,8 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
                 const uint64_t prod = (uint64_t)ui * (uint64_t)(a.per & 0xFFFFFu);
                 k = (int)(c + (uint32_t)(prod >> 24));
                 dl = (double)((uint32_t)prod & 0xFFFFFFu) * (1.0 / 16777216.0);
-                kv = k;  // y = 0 (c = 0, u = 0) is flagged through zmin
-                zmin = min(zmin, c | ui);
+                kv = k;  // y = 0 (c = 0, u = 0) is screened through zmin
+                zmin = min(zmin, ui);
             } else {
                 const double yv = g.exact_div ? ((double)c + u24(wd)) / (double)a.nstrat
                                               : ((double)c + u24(wd)) * a.inv_s;
@@ -333,20 +334,28 @@ template <int DIM>
 __device__ __forceinline__ void store_record(const TileIO& io, int dim, uint32_t ri, const int* kb,
                                              uint64_t zm, double c2) {
     constexpr int ND = nd_max(DIM);
-    uint32_t h[2 * ((ND + 1) / 2)];
+    constexpr int NW = (ND + 1) / 2;
+    uint32_t wd[NW];
 #pragma unroll
-    for (int d = 0; d < 2 * ((ND + 1) / 2); ++d) h[d] = (d < ND && d < dim) ? (uint32_t)kb[d < ND ? d : 0] : kNoBin;
+    for (int k = 0; k < NW; ++k) {
+        const uint32_t lo = 2 * k < dim ? (uint32_t)kb[2 * k] : kNoBin;
+        const uint32_t hi = (2 * k + 1 < ND && 2 * k + 1 < dim) ? (uint32_t)kb[2 * k + 1 < ND ? 2 * k + 1 : 0] : kNoBin;
+        // low 16 bits of lo, then of hi (one v_perm_b32)
+        wd[k] = __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+    }
     if (zm) {  // y = 0 exactly on some axis (rare): those axes are not trained on
 #pragma unroll
-        for (int d = 0; d < ND; ++d)
-            if ((zm >> d) & 1) h[d] = kNoBin;
+        for (int k = 0; k < NW; ++k) {
+            if ((zm >> (2 * k)) & 1) wd[k] |= kNoBin;
+            if ((zm >> (2 * k + 1)) & 1) wd[k] |= kNoBin << 16;
+        }
     }
 #pragma unroll
-    for (int k = 0; k < (ND + 1) / 2; ++k) {
+    for (int k = 0; k < NW; ++k) {
         if (DIM <= 0 && 2 * k >= dim) break;
         // a uniform row base and a 32-bit lane offset (no 64-bit address math per lane)
         uint32_t* row = io.rec_bins + (size_t)k * io.rec_stride;
-        row[ri] = (h[2 * k] & 0xFFFFu) | (h[2 * k + 1] << 16);
+        row[ri] = wd[k];
     }
     io.rec_c2[ri] = c2;
 }
