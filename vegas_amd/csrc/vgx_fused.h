@@ -148,12 +148,16 @@ struct CoordWalk {
 //   axes of the 24-bit uniform, to be initialised by the caller) is 0 when
 //   some axis may have y = 0 exactly (y = 0 needs u = 0 and stratum 0) --
 //   per_zero_axes() then names the axes to skip, if any.
+//   perv (PER, optional): the axes' increments per stratum, held in vector
+//   registers by the caller so that no scalar load per point and axis sits
+//   between the grid reads (a scalar load shares the LDS reads' wait counter
+//   and returns out of order, which would wait out every grid read at once).
 //   DIM > 0: that dimension; DIM <= 0: runtime g.dim, at most nd_max(DIM).
 template <int DIM, bool R32, bool PER>
 __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __restrict__ ax,
                                            const double* __restrict__ G, const PackedCoords<DIM>& pc,
                                            uint32_t h, uint32_t p, double* x, double& J, int* kbin,
-                                           uint32_t& zmin) {
+                                           uint32_t& zmin, const uint32_t* perv = nullptr) {
     static_assert(R32 || !PER, "PER needs 24-bit uniforms");
     const int dim = DIM > 0 ? DIM : g.dim;
     U4 w{0, 0, 0, 0};
@@ -174,7 +178,8 @@ __device__ __forceinline__ void eval_point(const TileGeom& g, const AxisC* __res
             const uint32_t ui = wd >> 8;
             if (PER) {
                 // both factors < 2^24: the backend selects the 24-bit multiplies
-                const uint64_t prod = (uint64_t)ui * (uint64_t)(a.per & 0xFFFFFu);
+                const uint32_t pv = perv ? perv[d] : (a.per & 0xFFFFFu);
+                const uint64_t prod = (uint64_t)ui * (uint64_t)pv;
                 k = (int)(c + (uint32_t)(prod >> 24));
                 dl = (double)((uint32_t)prod & 0xFFFFFFu) * (1.0 / 16777216.0);
                 kv = k;  // y = 0 (c = 0, u = 0) is screened through zmin
@@ -443,6 +448,19 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
         wdiv = g.dv_y / (double)ncur;
         if (PER) pc.set(ax, dim, hf + li - tp.hbase);
     }
+    // PER with the dimension known (up to 8 axes; above, the registers would
+    // spill): each axis's increments per stratum in a vector register for
+    // the tile (the empty asm leaves it there)
+    constexpr bool kPinPer = PER && DIM > 0 && DIM <= 8;
+    uint32_t perv[ND];
+    if constexpr (kPinPer) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            uint32_t v = ldc(ax, d).per;
+            asm volatile("" : "+v"(v));
+            perv[d] = v & 0xFFFFFu;
+        }
+    }
 #pragma unroll 1
     for (int j = 0; j < kK; ++j) {
         const uint32_t pj = pl0 + j;
@@ -464,7 +482,8 @@ __device__ __forceinline__ void tile_body(const TileGeom& g, const TileIO& io, c
             int kb[ND];
             double J;
             uint32_t zmin = 0xFFFFFFFFu;
-            eval_point<DIM, R32, PER>(g, ax, G, pc, hf + cur_i - tp.hbase, pj - tp.pshift, x, J, kb, zmin);
+            eval_point<DIM, R32, PER>(g, ax, G, pc, hf + cur_i - tp.hbase, pj - tp.pshift, x, J, kb, zmin,
+                                      kPinPer ? perv : nullptr);
             const double w = J * wdiv;
             const double f = fcn.template eval<DIM>(x, dim, pj);
             if (f != f) acc.nan_seen = 1;
