@@ -372,6 +372,34 @@ def test_y_zero_left_out_of_training(va, orc):
     assert np.abs(integ.map.grid - g_ok).max() < gap / 100
 
 
+def test_zero_uniform_in_upper_stratum_trained(va, orc):
+    """A 24-bit uniform of 0 in stratum 1 of 2 gives y = 1/2, inside (0,1),
+    so the point is trained on (pyx:459-463 leaves out y = 0 only): the
+    sampler's screen for y = 0, which looks at the uniform first, must let it
+    through.  Seed 63, point 121083 (uniform 0 on axis 0) with nstrat [2, 1]:
+    the first iteration puts the same number of points in both hcubes, so the
+    point is in hcube 1, stratum 1 on axis 0, and every point of that hcube
+    lands in the upper half of axis 0's bins.  The device's training counts
+    equal the oracle's bit for bit (the oracle draws the same 24-bit
+    uniforms), and a point counted out would show as one short there."""
+    seed, p, neval = 63, 121083, 600000
+    assert orc.philox([p, 0, 0, 0], [seed, 0])[0] >> 8 == 0
+    region = [[0.0, 1.0], [0.0, 1.0]]
+    o = orc.OracleIntegrator(region, neval=neval, nstrat=[2, 1], seed=seed, rng_f32=True)
+    out, sf, nf = o.iterate(orc.GaussMix([1.0], [0.0], [[0.5, 0.5]]))
+    f = va.GaussianMixture([1.0], [0.0], [[0.5, 0.5]])
+    integ = va.Integrator(region, neval=neval, nstrat=[2, 1], seed=seed, precision='f32')
+    assert list(integ.map.ninc) == list(o.map.ninc) and integ.map.ninc[0] % 2 == 0
+    integ(f, nitn=1)
+    npts = out.last_neval
+    assert integ.last_neval == npts and npts // 2 <= p < npts
+    gs, gn = integ._last_training()
+    half = int(o.map.ninc[0]) // 2
+    assert round(nf[0, half:].sum()) == npts // 2
+    assert np.array_equal(gn, nf)
+    np.testing.assert_allclose(gs, sf, rtol=1e-12, atol=0)
+
+
 def test_fused_equals_staged(va, orc):
     """Same seed and precision: the device integrand and a numpy lbatch,
     rbatch, torch and non-batch integrand see the same points."""
