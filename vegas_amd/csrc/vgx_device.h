@@ -212,14 +212,15 @@ __device__ inline double wave_total(double v) {
     return v;
 }
 
-// One step of a segmented inclusive scan (v += v of the source lane unless a
-// segment starts at or before this lane since the source), for 1 or 2 values.
 // dpp_d(0.0, x) in its cheaper form where every row is enabled.
 template <int CTRL, int ROWS>
 __device__ __forceinline__ double dpp_z(double x) {
     if constexpr (ROWS == 0xF) return dpp0_d<CTRL>(x);
     else return dpp_d<CTRL, ROWS>(0.0, x);
 }
+
+// One step of a segmented inclusive scan (v += v of the source lane unless a
+// segment starts at or before this lane since the source), for 1 or 2 values.
 template <int CTRL, int ROWS>
 __device__ __forceinline__ void segscan_step(double& v, int& f) {
     const double pv = dpp_z<CTRL, ROWS>(v);
