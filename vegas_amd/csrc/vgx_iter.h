@@ -215,20 +215,21 @@ struct DiagRidge32 {
         }
         const float fd = (float)nd;
         float acc = 0.0f;
-        // four components per step with their constants loaded first (wide
+        // eight components per step with their constants loaded first (wide
         // scalar loads, one wait instead of two per component); the sum runs
         // in component order as in the remainder loop
+        constexpr int kStep = 8;
         int k = 0;
-        for (; k + 4 <= ncomp; k += 4) {
-            float cc[4], aa[4], bb[4];
+        for (; k + kStep <= ncomp; k += kStep) {
+            float cc[kStep], aa[kStep], bb[kStep];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < kStep; ++i) {
                 cc[i] = ldc(c, k + i);
                 aa[i] = ldc(A, k + i);
                 bb[i] = ldc(B, k + i);
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < kStep; ++i) {
                 float t = xbar - cc[i];
                 float r2 = __builtin_fmaf(fd * t, t, P);
                 acc = __builtin_fmaf(aa[i], __expf(-bb[i] * r2), acc);
